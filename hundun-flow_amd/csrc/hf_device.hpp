@@ -1,0 +1,161 @@
+// Device-side building blocks of the gfx950 pressure solver: storage types,
+// level descriptors, the 7-point stencil in the reference evaluation order,
+// and block reductions built on 64-lane wavefront shuffles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hf {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kXcds = 8;  // MI355X: 256 CUs in 8 XCDs, each with its own L2
+
+// BF16 storage only; arithmetic always widens to FP32.
+struct bf16 {
+  uint16_t bits;
+};
+
+template <class T> struct Compute { using type = float; };
+template <> struct Compute<double> { using type = double; };
+
+__device__ __forceinline__ double ld(double v) { return v; }
+__device__ __forceinline__ float ld(float v) { return v; }
+__device__ __forceinline__ float ld(bf16 v) {
+  return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+}
+
+template <class T> __device__ __forceinline__ T st(double v);
+template <> __device__ __forceinline__ double st<double>(double v) { return v; }
+template <> __device__ __forceinline__ float st<float>(double v) {
+  return static_cast<float>(v);
+}
+template <> __device__ __forceinline__ bf16 st<bf16>(double v) {
+  // round-to-nearest-even on the FP32 value, quiet NaN preserved
+  const float f = static_cast<float>(v);
+  uint32_t u = __float_as_uint(f);
+  bf16 out;
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu) != 0u) {
+    out.bits = 0x7fc0u;
+    return out;
+  }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  out.bits = static_cast<uint16_t>(u >> 16);
+  return out;
+}
+template <class T> __device__ __forceinline__ T st(float v);
+template <> __device__ __forceinline__ double st<double>(float v) { return v; }
+template <> __device__ __forceinline__ float st<float>(float v) { return v; }
+template <> __device__ __forceinline__ bf16 st<bf16>(float v) {
+  return st<bf16>(static_cast<double>(v));
+}
+
+// Boundary handling of one physical face of the point row
+// (src/solver_mg.cpp:938-963 and :2356-2368): 0 -> zero (Dirichlet),
+// 1 -> self (Neumann), 2 -> wrapped neighbour (periodic halo).
+struct Bc {
+  int8_t m[6];
+};
+
+// One multigrid level in storage type T.  Coefficient arrays hold the
+// masked operator; act marks fluid cells (fluid volume > 0).
+template <class T>
+struct Lv {
+  int nx, ny, nz;
+  int64_t cells;
+  const T* D;
+  const T* fx;
+  const T* fy;
+  const T* fz;
+  const uint8_t* act;
+  const double* vol;
+  T* x;
+  T* b;
+  T* r;
+};
+
+template <class C, class T>
+__device__ __forceinline__ C neighbour_or_boundary(int mode, C self,
+                                                   const T* x, int64_t wrapped) {
+  if (mode == 0) return C(0);
+  if (mode == 1) return self;
+  return ld(x[wrapped]);
+}
+
+// A x at cell (i,j,k) in the reference order: D*x, then x-lo, x-hi, y-lo,
+// y-hi, z-lo, z-hi, each product subtracted in turn (src/solver_mg.cpp:
+// 2405-2411).  Built with -ffp-contract=off so FP64 results are bitwise equal
+// to the sequential reference.
+template <class C, class T>
+__device__ __forceinline__ C stencil(const Lv<T>& L, const Bc& bc, const T* x,
+                                     int i, int j, int k, int64_t c, C xc) {
+  const int64_t nx = L.nx;
+  const int64_t plane = nx * L.ny;
+  const int64_t row = static_cast<int64_t>(j) + static_cast<int64_t>(L.ny) * k;
+  const C xl = i > 0 ? ld(x[c - 1])
+                     : neighbour_or_boundary<C>(bc.m[0], xc, x, c + (nx - 1));
+  const C xh = i + 1 < L.nx
+                   ? ld(x[c + 1])
+                   : neighbour_or_boundary<C>(bc.m[1], xc, x, c - (nx - 1));
+  const C yl = j > 0 ? ld(x[c - nx])
+                     : neighbour_or_boundary<C>(bc.m[2], xc, x,
+                                                c + (int64_t)(L.ny - 1) * nx);
+  const C yh = j + 1 < L.ny
+                   ? ld(x[c + nx])
+                   : neighbour_or_boundary<C>(bc.m[3], xc, x,
+                                              c - (int64_t)(L.ny - 1) * nx);
+  const C zl = k > 0 ? ld(x[c - plane])
+                     : neighbour_or_boundary<C>(bc.m[4], xc, x,
+                                                c + (int64_t)(L.nz - 1) * plane);
+  const C zh = k + 1 < L.nz
+                   ? ld(x[c + plane])
+                   : neighbour_or_boundary<C>(bc.m[5], xc, x,
+                                              c - (int64_t)(L.nz - 1) * plane);
+  const int64_t fyo = c + nx * k;
+  C v = C(ld(L.D[c])) * xc;
+  v -= C(ld(L.fx[c + row])) * xl;
+  v -= C(ld(L.fx[c + row + 1])) * xh;
+  v -= C(ld(L.fy[fyo])) * yl;
+  v -= C(ld(L.fy[fyo + nx])) * yh;
+  v -= C(ld(L.fz[c])) * zl;
+  v -= C(ld(L.fz[c + plane])) * zh;
+  return v;
+}
+
+// Block index -> logical tile such that each XCD owns one contiguous slab of
+// the domain: neighbouring rows and planes are then served by the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t block, int64_t blocks) {
+  const int64_t q = blocks / kXcds;
+  const int64_t rem = blocks % kXcds;
+  const int64_t xcd = block % kXcds;
+  const int64_t local = block / kXcds;
+  return xcd < rem ? xcd * (q + 1) + local
+                   : rem * (q + 1) + (xcd - rem) * q + local;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    v += __shfl_down(v, offset, kWave);
+  return v;
+}
+
+// Deterministic block sum (fixed shuffle tree, fixed wave order).  Result is
+// valid in thread 0.  `scratch` needs blockDim/64 doubles of LDS.
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  v = wave_sum(v);
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double total = 0.0;
+  if (threadIdx.x == 0) {
+    const int waves = (blockDim.x + kWave - 1) / kWave;
+    for (int w = 0; w < waves; ++w) total += scratch[w];
+  }
+  __syncthreads();
+  return total;
+}
+
+}  // namespace hf

@@ -1,0 +1,1094 @@
+// Host side of the gfx950 pressure solver: level planning, device hierarchy,
+// the multigrid cycle program, the PCG driver and the C ABI declared in
+// include/hundun_flow.h.  All device work of one plan is issued on the plan's
+// HIP stream; the host reads back only the scalars a Krylov decision needs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "hf_kernels.hpp"
+#include "hundun_flow.h"
+
+namespace hf {
+namespace {
+
+constexpr int kMaxLevels = 32;
+constexpr uint32_t kDetailPlan = 9101U;
+constexpr uint32_t kDetailAlloc = 9102U;
+constexpr uint32_t kDetailHip = 9103U;
+constexpr uint32_t kDetailNumeric = 9104U;
+constexpr uint32_t kDetailCoefficients = 9105U;
+constexpr uint32_t kDetailBreakdown = 9106U;
+constexpr uint32_t kDetailNonFinite = 9107U;
+
+hf_status ok() { return {HF_OK, 0U}; }
+hf_status fail(int code, uint32_t detail) { return {code, detail}; }
+
+int64_t blocks_for(int64_t work) {
+  return std::max<int64_t>(1, (work + kBlock - 1) / kBlock);
+}
+
+struct Level {
+  int n[3];
+  int64_t cells = 0, nfx = 0, nfy = 0, nfz = 0;
+  int coarsen_mask = 0;
+  int line_mask = 0;
+  int prec = HF_PRECISION_FP64;
+  int fused = 1;
+  bool tail = false;
+  double volume = 0.0;  // sum of fluid volumes (constant projection)
+  // FP64 hierarchy (level 0 doubles as the Krylov operator)
+  double* D64 = nullptr;
+  double* fx64 = nullptr;
+  double* fy64 = nullptr;
+  double* fz64 = nullptr;
+  double* vol = nullptr;
+  uint8_t* act = nullptr;
+  // storage-precision copies (alias the FP64 arrays when prec == FP64)
+  void* DT = nullptr;
+  void* fxT = nullptr;
+  void* fyT = nullptr;
+  void* fzT = nullptr;
+  // workspace slots in storage precision
+  void* x = nullptr;
+  void* b = nullptr;
+  void* r = nullptr;
+};
+
+// One step of the host-built cycle program (src/solver_mg.cpp:4192-4338).
+struct CycleOp {
+  int kind;  // T_SMOOTH, T_RESTRICT, T_PROLONG
+  int level;
+  int sweeps;
+  int reverse;
+  int retain;
+};
+
+// Executable item: one grid-level op, or a contiguous run of ops on tail
+// levels issued as one k_tail launch.
+struct ExecItem {
+  bool tail = false;
+  CycleOp op{};
+  int tail_begin = 0;  // index into tail_ops
+  int tail_count = 0;
+};
+
+size_t elem_size(int prec) {
+  return prec == HF_PRECISION_FP64 ? 8 : (prec == HF_PRECISION_FP32 ? 4 : 2);
+}
+
+template <class F>
+void dispatch(int prec, F&& f) {
+  if (prec == HF_PRECISION_FP64)
+    f(double{});
+  else if (prec == HF_PRECISION_FP32)
+    f(float{});
+  else
+    f(bf16{});
+}
+
+template <class T>
+Lv<T> view(const Level& L) {
+  Lv<T> v;
+  v.nx = L.n[0];
+  v.ny = L.n[1];
+  v.nz = L.n[2];
+  v.cells = L.cells;
+  v.D = static_cast<const T*>(L.DT);
+  v.fx = static_cast<const T*>(L.fxT);
+  v.fy = static_cast<const T*>(L.fyT);
+  v.fz = static_cast<const T*>(L.fzT);
+  v.act = L.act;
+  v.vol = L.vol;
+  v.x = static_cast<T*>(L.x);
+  v.b = static_cast<T*>(L.b);
+  v.r = static_cast<T*>(L.r);
+  return v;
+}
+
+Lv<double> op_view(const Level& L) {
+  Lv<double> v;
+  v.nx = L.n[0];
+  v.ny = L.n[1];
+  v.nz = L.n[2];
+  v.cells = L.cells;
+  v.D = L.D64;
+  v.fx = L.fx64;
+  v.fy = L.fy64;
+  v.fz = L.fz64;
+  v.act = L.act;
+  v.vol = L.vol;
+  v.x = nullptr;
+  v.b = nullptr;
+  v.r = nullptr;
+  return v;
+}
+
+}  // namespace
+}  // namespace hf
+
+using namespace hf;
+
+struct hf_plan {
+  hf_grid_desc grid{};
+  hf_policy policy{};
+  hipStream_t stream = nullptr;
+  int n[3] = {0, 0, 0};
+  int64_t cells = 0;
+  Bc bc{};
+  int periodic_mask = 0;
+  std::vector<double> faces[3];
+  std::vector<double> widths[3];
+  double* d_faces[3] = {nullptr, nullptr, nullptr};
+  double* d_widths[3] = {nullptr, nullptr, nullptr};
+  std::vector<Level> levels;
+  int tail_first = 0;  // first level handled by the tail kernel
+  std::vector<CycleOp> program;
+  std::vector<ExecItem> exec;
+  std::vector<TailOp> tail_ops_host;
+  TailOp* d_tail_ops = nullptr;
+  TailLevel* d_tail_levels = nullptr;
+  bool coefficients_ready = false;
+  bool has_activity = false;
+  // raw activity masks (copies) for the adjoint
+  uint8_t* fxa = nullptr;
+  uint8_t* fya = nullptr;
+  uint8_t* fza = nullptr;
+  // reductions
+  double* partials = nullptr;
+  int64_t partial_stride = 0;
+  double* scalars = nullptr;
+  double* host_scalars = nullptr;  // pinned
+  int* d_flag = nullptr;
+  // Krylov workspace (5 vectors, solve_pcg slots 0..4) + adjoint scratch
+  double* kx = nullptr;
+  double* kr = nullptr;
+  double* kz = nullptr;
+  double* kp = nullptr;
+  double* kq = nullptr;
+  double* adj_rhs = nullptr;
+  double* adj_lambda = nullptr;
+  uint64_t device_bytes = 0;
+  uint64_t reductions = 0;
+  std::vector<void*> allocations;
+};
+
+namespace {
+
+hf_status hip_check(hipError_t e) {
+  return e == hipSuccess ? ok() : fail(HF_INVALID_PLAN, kDetailHip);
+}
+
+template <class T>
+hf_status dalloc(hf_plan* p, T** out, size_t count) {
+  void* ptr = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (hipMalloc(&ptr, bytes) != hipSuccess) {
+    *out = nullptr;
+    return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  }
+  p->allocations.push_back(ptr);
+  p->device_bytes += bytes;
+  *out = static_cast<T*>(ptr);
+  return ok();
+}
+
+hf_status dalloc_raw(hf_plan* p, void** out, size_t bytes) {
+  char* c = nullptr;
+  const hf_status s = dalloc(p, &c, bytes);
+  *out = c;
+  return s;
+}
+
+#define HF_TRY(expr)                         \
+  do {                                       \
+    const hf_status hf_try_s = (expr);       \
+    if (hf_try_s.code != HF_OK) return hf_try_s; \
+  } while (0)
+
+#define HF_LAUNCH(kernel, grid, block, shared, stream, ...)             \
+  hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)),         \
+                     dim3(block), shared, stream, __VA_ARGS__)
+
+// Level planner, src/solver_mg_workspace.cpp:113-143, :181-199, :317-453.
+void plan_levels(hf_plan* p) {
+  double mn[3], mx[3], rep[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = INFINITY;
+    mx[a] = 0.0;
+    rep[a] = 0.0;
+    for (double w : p->widths[a]) {
+      mn[a] = std::min(mn[a], w);
+      mx[a] = std::max(mx[a], w);
+      rep[a] += w;
+    }
+    rep[a] /= static_cast<double>(p->widths[a].size());
+  }
+  const double largest = std::max({rep[0], rep[1], rep[2]});
+  int line_mask = 0;
+  for (int a = 0; a < 3; ++a) {
+    const bool internal = mn[a] > 0.0 &&
+                          mx[a] / mn[a] >= p->policy.anisotropy_threshold;
+    const bool cross = rep[a] > 0.0 &&
+                       largest / rep[a] >= p->policy.anisotropy_threshold;
+    if (internal || cross) line_mask |= 1 << a;
+  }
+  int preferred;
+  if ((line_mask & (line_mask - 1)) != 0)
+    preferred = 7;
+  else
+    preferred = (7 & ~line_mask) == 0 ? 7 : (7 & ~line_mask);
+  int g[3] = {p->n[0], p->n[1], p->n[2]};
+  p->levels.clear();
+  for (int l = 0; l < p->policy.maximum_levels; ++l) {
+    Level L;
+    std::memcpy(L.n, g, sizeof g);
+    L.line_mask = line_mask;
+    int next[3] = {g[0], g[1], g[2]};
+    int mask = 0;
+    for (int a = 0; a < 3; ++a) {
+      if ((preferred & (1 << a)) == 0) continue;
+      if (g[a] <= p->policy.minimum_coarse_extent) continue;
+      next[a] = (g[a] + 1) / 2;
+      mask |= 1 << a;
+    }
+    L.coarsen_mask = mask;
+    L.cells = (int64_t)g[0] * g[1] * g[2];
+    L.nfx = (int64_t)(g[0] + 1) * g[1] * g[2];
+    L.nfy = (int64_t)g[0] * (g[1] + 1) * g[2];
+    L.nfz = (int64_t)g[0] * g[1] * (g[2] + 1);
+    L.fused = 1;
+    for (int a = 0; a < 3; ++a)
+      if (p->bc.m[2 * a] == 2 && (g[a] & 1)) L.fused = 0;
+    p->levels.push_back(L);
+    if (mask == 0 || (int)p->levels.size() == p->policy.maximum_levels) break;
+    std::memcpy(g, next, sizeof g);
+  }
+}
+
+int64_t tail_threshold() {
+  const char* env = std::getenv("HF_TAIL_CELLS");
+  if (env != nullptr) {
+    const long long v = std::atoll(env);
+    if (v >= 0) return v;
+  }
+  return 8192;
+}
+
+// Build the cycle program exactly as mg_cycle recurses
+// (src/solver_mg.cpp:4192-4338).
+void emit_cycle(hf_plan* p, int level, int counter) {
+  const int last = static_cast<int>(p->levels.size()) - 1;
+  if (level == last) {
+    p->program.push_back({T_SMOOTH, level, p->policy.coarse_sweeps, 0, 0});
+    return;
+  }
+  p->program.push_back({T_SMOOTH, level, p->policy.pre_sweeps, 0, 1});
+  p->program.push_back({T_RESTRICT, level, 0, 0, 0});
+  emit_cycle(p, level + 1, counter);
+  if (counter > 1) emit_cycle(p, level + 1, counter - 1);
+  p->program.push_back({T_PROLONG, level, 0, 0, 0});
+  p->program.push_back({T_SMOOTH, level, p->policy.post_sweeps, 1, 0});
+}
+
+// An op belongs to the tail when every level it touches is a tail level.
+bool op_on_tail(const hf_plan* p, const CycleOp& op) {
+  const int lo = op.level;
+  return lo >= p->tail_first;
+}
+
+void build_exec(hf_plan* p) {
+  p->program.clear();
+  p->exec.clear();
+  p->tail_ops_host.clear();
+  emit_cycle(p, 0, p->policy.f_cycle ? 2 : 1);
+  size_t i = 0;
+  while (i < p->program.size()) {
+    if (!op_on_tail(p, p->program[i])) {
+      ExecItem e;
+      e.op = p->program[i];
+      p->exec.push_back(e);
+      ++i;
+      continue;
+    }
+    ExecItem e;
+    e.tail = true;
+    e.tail_begin = static_cast<int>(p->tail_ops_host.size());
+    while (i < p->program.size() && op_on_tail(p, p->program[i])) {
+      const CycleOp& op = p->program[i];
+      TailOp t{};
+      t.kind = static_cast<uint8_t>(op.kind);
+      t.level = static_cast<uint8_t>(op.level);
+      t.reverse = static_cast<uint8_t>(op.reverse);
+      t.retain = static_cast<uint8_t>(op.retain);
+      t.sweeps = static_cast<uint16_t>(op.sweeps);
+      p->tail_ops_host.push_back(t);
+      ++i;
+    }
+    e.tail_count = static_cast<int>(p->tail_ops_host.size()) - e.tail_begin;
+    p->exec.push_back(e);
+  }
+}
+
+// ---- reductions -----------------------------------------------------------
+void finalize(hf_plan* p, int64_t blocks, int nacc, int op, int slot,
+              double volume = 0.0, int unit_linear = 1) {
+  HF_LAUNCH(k_finalize, 1, 1024, 0, p->stream, p->partials, blocks,
+            p->partial_stride, nacc, p->scalars, op, slot, volume,
+            unit_linear);
+  ++p->reductions;
+}
+
+hf_status read_scalars(hf_plan* p) {
+  if (hipMemcpyAsync(p->host_scalars, p->scalars, S_COUNT * sizeof(double),
+                     hipMemcpyDeviceToHost, p->stream) != hipSuccess)
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  if (hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  return ok();
+}
+
+// ---- multigrid ------------------------------------------------------------
+template <class T>
+void project_level(hf_plan* p, int l, T* field) {
+  Level& L = p->levels[l];
+  const int64_t nb = blocks_for(L.cells);
+  HF_LAUNCH(k_vol_dot<T>, nb, kBlock, 0, p->stream, field, L.vol, L.cells,
+            p->partials);
+  finalize(p, nb, 1, F_MEAN, S_MEAN0 + l, L.volume);
+  HF_LAUNCH(k_project_apply<T>, nb, kBlock, 0, p->stream, field, L.vol,
+            L.cells, p->scalars + S_MEAN0 + l);
+}
+
+template <class T>
+void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain) {
+  Level& L = p->levels[l];
+  const Lv<T> v = view<T>(L);
+  const int64_t hx = (L.n[0] + 1) / 2;
+  const int64_t work = hx * L.n[1] * L.n[2];
+  const int64_t nbw = blocks_for(work);
+  const int64_t nbc = blocks_for(L.cells);
+  if (!L.fused) HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc, 0);
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    const int first = reverse ? 1 : 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int color = (first + pass) & 1;
+      if (L.fused)
+        HF_LAUNCH(k_rb<T>, nbw, kBlock, 0, p->stream, v, p->bc, color, work);
+      else
+        HF_LAUNCH(k_rb_separated<T>, nbw, kBlock, 0, p->stream, v, color, work);
+      const bool final_update = sweep + 1 == sweeps && pass == 1;
+      if (!final_update || retain) {
+        if (!L.fused || (final_update && retain))
+          HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc,
+                    (retain && final_update) ? 1 : 0);
+      }
+    }
+  }
+}
+
+void exec_grid_op(hf_plan* p, const CycleOp& op) {
+  Level& L = p->levels[op.level];
+  if (op.kind == T_SMOOTH) {
+    dispatch(L.prec, [&](auto t) {
+      using T = decltype(t);
+      grid_smooth<T>(p, op.level, op.sweeps, op.reverse, op.retain);
+    });
+    return;
+  }
+  Level& Cn = p->levels[op.level + 1];
+  if (op.kind == T_RESTRICT) {
+    const int64_t nb = blocks_for(Cn.cells);
+    const bool proj = p->policy.null_space_constant != 0;
+    dispatch(L.prec, [&](auto tf) {
+      using Tf = decltype(tf);
+      dispatch(Cn.prec, [&](auto tc) {
+        using Tc = decltype(tc);
+        HF_LAUNCH((k_restrict<Tf, Tc>), nb, kBlock, 0, p->stream, view<Tf>(L),
+                  view<Tc>(Cn), L.coarsen_mask,
+                  proj ? p->partials : static_cast<double*>(nullptr));
+        if (proj) {
+          finalize(p, nb, 1, F_MEAN, S_MEAN0 + op.level + 1, Cn.volume);
+          HF_LAUNCH(k_project_apply<Tc>, nb, kBlock, 0, p->stream,
+                    static_cast<Tc*>(Cn.b), Cn.vol, Cn.cells,
+                    p->scalars + S_MEAN0 + op.level + 1);
+        }
+      });
+    });
+    return;
+  }
+  const int64_t nb = blocks_for(L.cells);
+  dispatch(Cn.prec, [&](auto tc) {
+    using Tc = decltype(tc);
+    dispatch(L.prec, [&](auto tf) {
+      using Tf = decltype(tf);
+      HF_LAUNCH((k_prolong<Tc, Tf>), nb, kBlock, 0, p->stream, view<Tc>(Cn),
+                view<Tf>(L), L.coarsen_mask, p->periodic_mask);
+    });
+  });
+}
+
+void run_cycle(hf_plan* p) {
+  for (const ExecItem& e : p->exec) {
+    if (!e.tail) {
+      exec_grid_op(p, e.op);
+      continue;
+    }
+    HF_LAUNCH(k_tail, 1, 1024, 0, p->stream, p->d_tail_levels, p->tail_first,
+              p->d_tail_ops + e.tail_begin, e.tail_count, p->bc,
+              p->periodic_mask, static_cast<int>(p->policy.null_space_constant));
+  }
+}
+
+// z = M r on device arrays; optional fused Krylov dots (r.z, z.q).
+hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
+                   bool dots) {
+  Level& L0 = p->levels[0];
+  const int64_t nb = blocks_for(L0.cells);
+  dispatch(L0.prec, [&](auto t) {
+    using T = decltype(t);
+    T* b = static_cast<T*>(L0.b);
+    T* x = static_cast<T*>(L0.x);
+    HF_LAUNCH(k_mg_entry<T>, nb, kBlock, 0, p->stream, r, L0.act, b, x,
+              L0.cells);
+    if (p->policy.null_space_constant) project_level<T>(p, 0, b);
+  });
+  run_cycle(p);
+  const double* alpha = nullptr;
+  dispatch(L0.prec, [&](auto t) {
+    using T = decltype(t);
+    T* x = static_cast<T*>(L0.x);
+    if (p->policy.null_space_constant) project_level<T>(p, 0, x);
+    if (!p->policy.unit_linear) {
+      HF_LAUNCH(k_level_apply<T>, nb, kBlock, 0, p->stream, view<T>(L0), p->bc,
+                1);
+      HF_LAUNCH(k_mg_proj3<T>, nb, kBlock, 0, p->stream, r, L0.act,
+                static_cast<const T*>(L0.r), L0.cells, p->partials,
+                p->partial_stride);
+      finalize(p, nb, 3, F_MGALPHA, 0, 0.0, 0);
+      alpha = p->scalars + S_MGALPHA;
+    }
+    HF_LAUNCH(k_mg_exit<T>, nb, kBlock, 0, p->stream, x, L0.act, alpha, z,
+              dots ? r : static_cast<const double*>(nullptr), q, L0.cells,
+              dots ? p->partials : static_cast<double*>(nullptr),
+              p->partial_stride);
+  });
+  return ok();
+}
+
+hf_status upload_tail(hf_plan* p) {
+  const int L = static_cast<int>(p->levels.size());
+  const int count = L - p->tail_first;
+  if (count <= 0) return ok();
+  std::vector<TailLevel> host(count);
+  for (int l = p->tail_first; l < L; ++l) {
+    TailLevel& t = host[l - p->tail_first];
+    t.L = view<double>(p->levels[l]);
+    t.coarsen_mask = p->levels[l].coarsen_mask;
+    t.fused = p->levels[l].fused;
+    t.volume = p->levels[l].volume;
+  }
+  if (hipMemcpyAsync(p->d_tail_levels, host.data(),
+                     count * sizeof(TailLevel), hipMemcpyHostToDevice,
+                     p->stream) != hipSuccess)
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  if (!p->tail_ops_host.empty() &&
+      hipMemcpyAsync(p->d_tail_ops, p->tail_ops_host.data(),
+                     p->tail_ops_host.size() * sizeof(TailOp),
+                     hipMemcpyHostToDevice, p->stream) != hipSuccess)
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  return hip_check(hipStreamSynchronize(p->stream));
+}
+
+AxisGeom axis_geom(const hf_plan* p) {
+  AxisGeom g;
+  for (int a = 0; a < 3; ++a) {
+    g.faces[a] = p->d_faces[a];
+    g.n[a] = p->n[a];
+  }
+  for (int f = 0; f < 6; ++f) g.kind[f] = static_cast<int8_t>(p->grid.boundary[f]);
+  return g;
+}
+
+Lv<double> krylov_view(const hf_plan* p) { return op_view(p->levels[0]); }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+hf_status hf_policy_default(hf_policy* policy) {
+  if (policy == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  policy->anisotropy_threshold = 4.0;
+  policy->pre_sweeps = 2;
+  policy->post_sweeps = 2;
+  policy->maximum_levels = 32;
+  policy->minimum_coarse_extent = 3;
+  policy->coarse_sweeps = 24;
+  policy->f_cycle = 0;
+  policy->null_space_constant = 1;
+  policy->unit_linear = 1;
+  policy->precision = HF_PRECISION_FP64;
+  return ok();
+}
+
+hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
+                         void* stream, hf_plan** out) {
+  if (out == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  *out = nullptr;
+  if (grid == nullptr || policy == nullptr || grid->nx <= 0 || grid->ny <= 0 ||
+      grid->nz <= 0 || grid->faces_x == nullptr || grid->faces_y == nullptr ||
+      grid->faces_z == nullptr)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  for (int a = 0; a < 3; ++a) {
+    const int lo = grid->boundary[2 * a], hi = grid->boundary[2 * a + 1];
+    if (lo < 0 || lo > 2 || hi < 0 || hi > 2) return fail(HF_INVALID_PLAN, kDetailPlan);
+    if ((lo == HF_BC_PERIODIC) != (hi == HF_BC_PERIODIC))
+      return fail(HF_INVALID_PLAN, kDetailPlan);
+  }
+  if (policy->pre_sweeps == 0 || policy->post_sweeps == 0 ||
+      policy->coarse_sweeps == 0 || policy->maximum_levels < 2 ||
+      policy->maximum_levels > kMaxLevels || policy->minimum_coarse_extent < 2 ||
+      !(policy->anisotropy_threshold > 1.0) || policy->precision > 2)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  hf_plan* p = new (std::nothrow) hf_plan();
+  if (p == nullptr) return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  p->grid = *grid;
+  p->policy = *policy;
+  p->stream = static_cast<hipStream_t>(stream);
+  p->n[0] = grid->nx;
+  p->n[1] = grid->ny;
+  p->n[2] = grid->nz;
+  p->cells = (int64_t)grid->nx * grid->ny * grid->nz;
+  const double* f[3] = {grid->faces_x, grid->faces_y, grid->faces_z};
+  for (int a = 0; a < 3; ++a) {
+    p->faces[a].assign(f[a], f[a] + p->n[a] + 1);
+    p->widths[a].resize(p->n[a]);
+    for (int i = 0; i < p->n[a]; ++i) {
+      const double w = p->faces[a][i + 1] - p->faces[a][i];
+      if (!(w > 0.0) || !std::isfinite(w)) {
+        delete p;
+        return fail(HF_INVALID_PLAN, kDetailPlan);
+      }
+      p->widths[a][i] = w;
+    }
+  }
+  for (int face = 0; face < 6; ++face)
+    p->bc.m[face] = static_cast<int8_t>(grid->boundary[face] == HF_BC_DIRICHLET
+                                            ? 0
+                                            : (grid->boundary[face] == HF_BC_NEUMANN ? 1 : 2));
+  p->periodic_mask = (grid->boundary[0] == HF_BC_PERIODIC ? 1 : 0) |
+                     (grid->boundary[2] == HF_BC_PERIODIC ? 2 : 0) |
+                     (grid->boundary[4] == HF_BC_PERIODIC ? 4 : 0);
+  plan_levels(p);
+  if (p->levels.size() < 2 || p->levels[0].line_mask != 0) {
+    // line relaxation levels are not provided by this build
+    delete p;
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  }
+  const int64_t threshold = tail_threshold();
+  p->tail_first = static_cast<int>(p->levels.size());
+  for (int l = static_cast<int>(p->levels.size()) - 1; l >= 0; --l) {
+    if (p->levels[l].cells <= threshold)
+      p->tail_first = l;
+    else
+      break;
+  }
+  // the coarsest level is always FP64 and handled by the tail kernel
+  if (p->tail_first > static_cast<int>(p->levels.size()) - 1)
+    p->tail_first = static_cast<int>(p->levels.size()) - 1;
+  for (size_t l = 0; l < p->levels.size(); ++l) {
+    Level& L = p->levels[l];
+    L.tail = static_cast<int>(l) >= p->tail_first;
+    L.prec = L.tail ? HF_PRECISION_FP64 : policy->precision;
+  }
+  build_exec(p);
+
+  hf_status s = ok();
+  auto guard = [&](hf_status st) {
+    if (st.code != HF_OK && s.code == HF_OK) s = st;
+  };
+  for (int a = 0; a < 3; ++a) {
+    guard(dalloc(p, &p->d_faces[a], p->n[a] + 1));
+    guard(dalloc(p, &p->d_widths[a], p->n[a]));
+  }
+  for (Level& L : p->levels) {
+    guard(dalloc(p, &L.D64, L.cells));
+    guard(dalloc(p, &L.fx64, L.nfx));
+    guard(dalloc(p, &L.fy64, L.nfy));
+    guard(dalloc(p, &L.fz64, L.nfz));
+    guard(dalloc(p, &L.vol, L.cells));
+    guard(dalloc(p, &L.act, L.cells));
+    const size_t es = elem_size(L.prec);
+    if (L.prec == HF_PRECISION_FP64) {
+      L.DT = L.D64;
+      L.fxT = L.fx64;
+      L.fyT = L.fy64;
+      L.fzT = L.fz64;
+    } else {
+      guard(dalloc_raw(p, &L.DT, es * L.cells));
+      guard(dalloc_raw(p, &L.fxT, es * L.nfx));
+      guard(dalloc_raw(p, &L.fyT, es * L.nfy));
+      guard(dalloc_raw(p, &L.fzT, es * L.nfz));
+    }
+    guard(dalloc_raw(p, &L.x, es * L.cells));
+    guard(dalloc_raw(p, &L.b, es * L.cells));
+    guard(dalloc_raw(p, &L.r, es * L.cells));
+  }
+  const int64_t max_blocks = blocks_for(p->cells + 2 * (int64_t)p->n[0] * p->n[1]) + 8;
+  p->partial_stride = max_blocks;
+  guard(dalloc(p, &p->partials, 3 * max_blocks));
+  guard(dalloc(p, &p->scalars, S_COUNT));
+  guard(dalloc(p, &p->d_flag, 4));
+  guard(dalloc(p, &p->kx, p->cells));
+  guard(dalloc(p, &p->kr, p->cells));
+  guard(dalloc(p, &p->kz, p->cells));
+  guard(dalloc(p, &p->kp, p->cells));
+  guard(dalloc(p, &p->kq, p->cells));
+  const int tail_levels = static_cast<int>(p->levels.size()) - p->tail_first;
+  guard(dalloc(p, &p->d_tail_levels, std::max(tail_levels, 1)));
+  guard(dalloc(p, &p->d_tail_ops, std::max<size_t>(p->tail_ops_host.size(), 1)));
+  if (s.code == HF_OK &&
+      hipHostMalloc(reinterpret_cast<void**>(&p->host_scalars),
+                    S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    s = fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  if (s.code == HF_OK) {
+    for (int a = 0; a < 3; ++a) {
+      if (hipMemcpyAsync(p->d_faces[a], p->faces[a].data(),
+                         (p->n[a] + 1) * sizeof(double), hipMemcpyHostToDevice,
+                         p->stream) != hipSuccess ||
+          hipMemcpyAsync(p->d_widths[a], p->widths[a].data(),
+                         p->n[a] * sizeof(double), hipMemcpyHostToDevice,
+                         p->stream) != hipSuccess)
+        s = fail(HF_INVALID_PLAN, kDetailHip);
+    }
+    if (s.code == HF_OK) s = hip_check(hipStreamSynchronize(p->stream));
+  }
+  if (s.code != HF_OK) {
+    hf_plan_destroy(p);
+    return s;
+  }
+  *out = p;
+  return ok();
+}
+
+hf_status hf_plan_destroy(hf_plan* p) {
+  if (p == nullptr) return ok();
+  if (p->stream != nullptr) (void)hipStreamSynchronize(p->stream);
+  else (void)hipDeviceSynchronize();
+  for (void* ptr : p->allocations) (void)hipFree(ptr);
+  if (p->host_scalars != nullptr) (void)hipHostFree(p->host_scalars);
+  delete p;
+  return ok();
+}
+
+hf_status hf_assemble_coefficients(hf_plan* p, const double* mobility,
+                                   double* diag, double* bx, double* by,
+                                   double* bz) {
+  if (p == nullptr || mobility == nullptr || diag == nullptr || bx == nullptr ||
+      by == nullptr || bz == nullptr)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const AxisGeom g = axis_geom(p);
+  const Level& L = p->levels[0];
+  double* out[3] = {bx, by, bz};
+  const int64_t counts[3] = {L.nfx, L.nfy, L.nfz};
+  for (int a = 0; a < 3; ++a)
+    HF_LAUNCH(k_assemble_faces, blocks_for(counts[a]), kBlock, 0, p->stream, g,
+              a, mobility, out[a], counts[a]);
+  HF_LAUNCH(k_face_sum_diag, blocks_for(L.cells), kBlock, 0, p->stream,
+            p->n[0], p->n[1], p->n[2], bx, by, bz, diag);
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
+                                   const double* bx, const double* by,
+                                   const double* bz, const uint8_t* ca,
+                                   const uint8_t* fxa, const uint8_t* fya,
+                                   const uint8_t* fza) {
+  if (p == nullptr || diag == nullptr || bx == nullptr || by == nullptr ||
+      bz == nullptr)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const bool any = ca || fxa || fya || fza;
+  const bool all = ca && fxa && fya && fza;
+  if (any && !all) return fail(HF_INVALID_PLAN, kDetailPlan);
+  p->has_activity = all;
+  Level& L0 = p->levels[0];
+  if (all) {
+    if (p->fxa == nullptr) {
+      HF_TRY(dalloc(p, &p->fxa, L0.nfx));
+      HF_TRY(dalloc(p, &p->fya, L0.nfy));
+      HF_TRY(dalloc(p, &p->fza, L0.nfz));
+    }
+    HF_TRY(hip_check(hipMemcpyAsync(p->fxa, fxa, L0.nfx, hipMemcpyDeviceToDevice, p->stream)));
+    HF_TRY(hip_check(hipMemcpyAsync(p->fya, fya, L0.nfy, hipMemcpyDeviceToDevice, p->stream)));
+    HF_TRY(hip_check(hipMemcpyAsync(p->fza, fza, L0.nfz, hipMemcpyDeviceToDevice, p->stream)));
+  }
+  // level 0: masking
+  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfx), kBlock, 0, p->stream, bx, fxa, L0.fx64, L0.nfx);
+  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfy), kBlock, 0, p->stream, by, fya, L0.fy64, L0.nfy);
+  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfz), kBlock, 0, p->stream, bz, fza, L0.fz64, L0.nfz);
+  Finest f{};
+  f.nx = p->n[0];
+  f.ny = p->n[1];
+  f.nz = p->n[2];
+  f.D = diag;
+  f.bx = bx;
+  f.by = by;
+  f.bz = bz;
+  f.ca = ca;
+  f.fxa = fxa;
+  f.fya = fya;
+  f.fza = fza;
+  f.wx = p->d_widths[0];
+  f.wy = p->d_widths[1];
+  f.wz = p->d_widths[2];
+  f.oD = L0.D64;
+  f.ofx = L0.fx64;
+  f.ofy = L0.fy64;
+  f.ofz = L0.fz64;
+  f.vol = L0.vol;
+  f.act = L0.act;
+  HF_LAUNCH(k_mask_cells, blocks_for(L0.cells), kBlock, 0, p->stream, f);
+  // coarse levels
+  HF_TRY(hip_check(hipMemsetAsync(p->d_flag, 0, sizeof(int), p->stream)));
+  for (size_t l = 1; l < p->levels.size(); ++l) {
+    Level& F = p->levels[l - 1];
+    Level& C = p->levels[l];
+    Agg a;
+    for (int ax = 0; ax < 3; ++ax) {
+      a.fn[ax] = F.n[ax];
+      a.cn[ax] = C.n[ax];
+    }
+    a.mask = F.coarsen_mask;
+    Fine64 fine{F.D64, F.fx64, F.fy64, F.fz64, F.vol};
+    Coarse64 coarse{C.D64, C.fx64, C.fy64, C.fz64, C.vol, C.act};
+    HF_LAUNCH(k_coarse_cells, blocks_for(C.cells), kBlock, 0, p->stream, a, fine, coarse);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfx), kBlock, 0, p->stream, a, 0, F.fx64, C.fx64);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfy), kBlock, 0, p->stream, a, 1, F.fy64, C.fy64);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfz), kBlock, 0, p->stream, a, 2, F.fz64, C.fz64);
+    HF_LAUNCH(k_coarse_diag, blocks_for(C.cells), kBlock, 0, p->stream, C.n[0], C.n[1], C.n[2], coarse, p->d_flag);
+  }
+  for (Level& L : p->levels)
+    HF_LAUNCH(k_validate, blocks_for(L.cells), kBlock, 0, p->stream, L.n[0], L.n[1], L.n[2], L.D64, L.fx64, L.fy64, L.fz64, L.vol, p->d_flag);
+  // storage-precision copies
+  for (Level& L : p->levels) {
+    if (L.prec == HF_PRECISION_FP64) continue;
+    dispatch(L.prec, [&](auto t) {
+      using T = decltype(t);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.cells), kBlock, 0, p->stream, L.D64, static_cast<T*>(L.DT), L.cells);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.nfx), kBlock, 0, p->stream, L.fx64, static_cast<T*>(L.fxT), L.nfx);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.nfy), kBlock, 0, p->stream, L.fy64, static_cast<T*>(L.fyT), L.nfy);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.nfz), kBlock, 0, p->stream, L.fz64, static_cast<T*>(L.fzT), L.nfz);
+    });
+  }
+  // projection volumes (sum of fluid volumes per level)
+  for (size_t l = 0; l < p->levels.size(); ++l) {
+    Level& L = p->levels[l];
+    const int64_t nb = blocks_for(L.cells);
+    HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, L.vol, L.cells, p->partials,
+              static_cast<const uint8_t*>(nullptr));
+    finalize(p, nb, 1, F_STORE, S_SUM);
+    HF_TRY(read_scalars(p));
+    L.volume = p->host_scalars[S_SUM];
+  }
+  int flag = 0;
+  HF_TRY(hip_check(hipMemcpyAsync(&flag, p->d_flag, sizeof(int), hipMemcpyDeviceToHost, p->stream)));
+  HF_TRY(hip_check(hipStreamSynchronize(p->stream)));
+  if (flag != 0) return fail(HF_NUMERICAL_FAILURE, kDetailCoefficients);
+  if (p->policy.null_space_constant && !(p->levels[0].volume > 0.0))
+    return fail(HF_NUMERICAL_FAILURE, kDetailCoefficients);
+  HF_TRY(upload_tail(p));
+  p->coefficients_ready = true;
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_operator_apply(hf_plan* p, const double* x, double* y) {
+  if (p == nullptr || x == nullptr || y == nullptr || !p->coefficients_ready)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  HF_LAUNCH(k_op_dot, blocks_for(p->cells), kBlock, 0, p->stream,
+            krylov_view(p), p->bc, x, y, static_cast<double*>(nullptr));
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_mg_apply(hf_plan* p, const double* r, double* z) {
+  if (p == nullptr || r == nullptr || z == nullptr || !p->coefficients_ready)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  HF_TRY(mg_apply(p, r, z, nullptr, false));
+  return hip_check(hipGetLastError());
+}
+
+static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
+                          const hf_solve_control* ctl, hf_solve_result* res) {
+  const int64_t N = p->cells;
+  const int64_t nb = blocks_for(N);
+  const Lv<double> A = krylov_view(p);
+  double* x = p->kx;
+  double* r = p->kr;
+  double* z = p->kz;
+  double* pp = p->kp;
+  double* q = p->kq;
+  const uint64_t red0 = p->reductions;
+  auto finish = [&](hf_status st, int term) {
+    res->status = st;
+    res->termination = term;
+    res->reduction_calls = p->reductions - red0;
+    return st;
+  };
+  // ||b||
+  HF_LAUNCH(k_sumsq, nb, kBlock, 0, p->stream, rhs, N, p->partials);
+  finalize(p, nb, 1, F_STORE, S_SUM);
+  HF_TRY(read_scalars(p));
+  const double bnorm = std::sqrt(p->host_scalars[S_SUM]);
+  res->rhs_norm = bnorm;
+  if (!std::isfinite(bnorm))
+    return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
+  const double tol = std::max(ctl->absolute_tolerance, ctl->relative_tolerance * bnorm);
+  res->true_residual_limit = tol;
+  if (bnorm == 0.0) {
+    HF_TRY(hip_check(hipMemsetAsync(solution, 0, N * sizeof(double), p->stream)));
+    return finish(ok(), HF_TERM_ZERO_RHS);
+  }
+  HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, x, solution, N);
+  HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x, r, p->partials);
+  finalize(p, nb, 1, F_STORE, S_RNORM2);
+  res->operator_applies++;
+  HF_TRY(read_scalars(p));
+  res->initial_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
+  res->final_true_residual = res->initial_true_residual;
+  res->recursive_residual = res->initial_true_residual;
+  if (!std::isfinite(res->initial_true_residual))
+    return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_OPERATOR_FAILURE);
+  if (res->initial_true_residual <= tol) {
+    HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, solution, x, N);
+    return finish(ok(), HF_TERM_CONVERGED);
+  }
+  auto precondition_restart = [&]() -> hf_status {
+    HF_TRY(mg_apply(p, r, z, nullptr, true));
+    res->preconditioner_applies++;
+    finalize(p, nb, 1, F_RHO, 0);
+    HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, pp, z, N);
+    return ok();
+  };
+  HF_TRY(precondition_restart());
+  HF_TRY(read_scalars(p));
+  {
+    const double rho = p->host_scalars[S_RHO];
+    if (!(rho > 0.0) || !std::isfinite(rho))
+      return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown), HF_TERM_BREAKDOWN);
+  }
+  while (res->iterations < ctl->maximum_iterations) {
+    HF_LAUNCH(k_op_dot, nb, kBlock, 0, p->stream, A, p->bc, pp, q, p->partials);
+    res->operator_applies++;
+    finalize(p, nb, 1, F_ALPHA, 0);
+    HF_LAUNCH(k_update_xr, nb, kBlock, 0, p->stream, x, pp, r, q,
+              p->scalars + S_ALPHA, N, p->partials);
+    finalize(p, nb, 1, F_STORE, S_RNORM2);
+    res->iterations++;
+    HF_TRY(read_scalars(p));
+    const double pap = p->host_scalars[S_PAP];
+    if (!(pap > 0.0) || !std::isfinite(pap))
+      return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown), HF_TERM_BREAKDOWN);
+    if (!std::isfinite(p->host_scalars[S_ALPHA]))
+      return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
+    const double recursive = std::sqrt(p->host_scalars[S_RNORM2]);
+    if (!std::isfinite(recursive))
+      return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
+    res->recursive_residual = recursive;
+    const bool verify = recursive <= tol ||
+                        res->iterations % ctl->true_residual_interval == 0 ||
+                        res->iterations == ctl->maximum_iterations;
+    if (verify) {
+      HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x, r,
+                p->partials);
+      finalize(p, nb, 1, F_STORE, S_RNORM2);
+      res->operator_applies++;
+      HF_TRY(read_scalars(p));
+      res->final_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
+      res->recursive_residual = res->final_true_residual;
+      if (!std::isfinite(res->final_true_residual))
+        return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_OPERATOR_FAILURE);
+      if (res->final_true_residual <= tol) {
+        HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, solution, x, N);
+        return finish(ok(), HF_TERM_CONVERGED);
+      }
+      if (res->iterations == ctl->maximum_iterations)
+        return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
+      HF_TRY(precondition_restart());
+      continue;
+    }
+    HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true));
+    res->preconditioner_applies++;
+    finalize(p, nb, ctl->flexible ? 2 : 1, ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
+    HF_LAUNCH(k_update_p, nb, kBlock, 0, p->stream, pp, z, p->scalars + S_BETA, N);
+  }
+  return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
+}
+
+hf_status hf_pcg_solve(hf_plan* p, const double* rhs, double* solution,
+                       const hf_solve_control* ctl, hf_solve_result* res) {
+  if (res != nullptr) std::memset(res, 0, sizeof *res);
+  if (p == nullptr || rhs == nullptr || solution == nullptr || ctl == nullptr ||
+      res == nullptr || !p->coefficients_ready)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  if (!std::isfinite(ctl->absolute_tolerance) ||
+      !std::isfinite(ctl->relative_tolerance) || ctl->absolute_tolerance < 0.0 ||
+      ctl->relative_tolerance < 0.0 ||
+      (ctl->absolute_tolerance == 0.0 && ctl->relative_tolerance == 0.0) ||
+      ctl->maximum_iterations == 0 || ctl->true_residual_interval == 0) {
+    res->status = fail(HF_INVALID_PLAN, kDetailPlan);
+    res->termination = HF_TERM_INVALID_PLAN;
+    return res->status;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, p->stream);
+  const hf_status st = pcg_impl(p, rhs, solution, ctl, res);
+  (void)hipEventRecord(e1, p->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  res->solve_milliseconds = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const hipError_t last = hipGetLastError();
+  if (st.code == HF_OK && last != hipSuccess) {
+    res->status = fail(HF_INVALID_PLAN, kDetailHip);
+    return res->status;
+  }
+  return st;
+}
+
+hf_status hf_project_mean(hf_plan* p, double* field) {
+  if (p == nullptr || field == nullptr || !p->coefficients_ready)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const int64_t nb = blocks_for(p->cells);
+  const uint8_t* act = p->levels[0].act;
+  HF_LAUNCH(k_count, nb, kBlock, 0, p->stream, act, p->cells, p->partials);
+  finalize(p, nb, 1, F_STORE, S_SUM2);
+  HF_TRY(read_scalars(p));
+  const double count = p->host_scalars[S_SUM2];
+  if (!(count > 0.0)) return fail(HF_NUMERICAL_FAILURE, kDetailNumeric);
+  HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, field, p->cells, p->partials, act);
+  finalize(p, nb, 1, F_MEAN, S_SUM, count);
+  HF_LAUNCH(k_subtract_mean, nb, kBlock, 0, p->stream, field, p->cells,
+            p->scalars + S_SUM, act);
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_adjoint_gradients(hf_plan* p, const double* mobility,
+                               const double* pressure, const double* grad_p,
+                               const hf_solve_control* ctl, double* grad_b,
+                               double* grad_logrho, hf_solve_result* res) {
+  if (res != nullptr) std::memset(res, 0, sizeof *res);
+  if (p == nullptr || pressure == nullptr || grad_p == nullptr || ctl == nullptr ||
+      res == nullptr || !p->coefficients_ready ||
+      (grad_logrho != nullptr && mobility == nullptr))
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const int64_t N = p->cells;
+  const int64_t nb = blocks_for(N);
+  if (p->adj_rhs == nullptr) {
+    HF_TRY(dalloc(p, &p->adj_rhs, N));
+    HF_TRY(dalloc(p, &p->adj_lambda, N));
+  }
+  // adjoint rhs: P grad_p (solids zero, zero fluid mean)
+  HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, p->adj_rhs, grad_p, N);
+  HF_TRY(hf_project_mean(p, p->adj_rhs));
+  HF_TRY(hip_check(hipMemsetAsync(p->adj_lambda, 0, N * sizeof(double), p->stream)));
+  const hf_status st = hf_pcg_solve(p, p->adj_rhs, p->adj_lambda, ctl, res);
+  if (st.code != HF_OK) return st;
+  HF_TRY(hf_project_mean(p, p->adj_lambda));
+  if (grad_b != nullptr)
+    HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, grad_b, p->adj_lambda, N);
+  if (grad_logrho != nullptr) {
+    HF_LAUNCH(k_grad_logrho, nb, kBlock, 0, p->stream, axis_geom(p), mobility,
+              pressure, p->adj_lambda,
+              p->has_activity ? p->fxa : static_cast<uint8_t*>(nullptr),
+              p->has_activity ? p->fya : static_cast<uint8_t*>(nullptr),
+              p->has_activity ? p->fza : static_cast<uint8_t*>(nullptr),
+              grad_logrho);
+  }
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_plan_level_count(const hf_plan* p, int32_t* levels) {
+  if (p == nullptr || levels == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  *levels = static_cast<int32_t>(p->levels.size());
+  return ok();
+}
+
+hf_status hf_plan_level_info(const hf_plan* p, int32_t level,
+                             hf_level_info* info) {
+  if (p == nullptr || info == nullptr || level < 0 ||
+      level >= static_cast<int32_t>(p->levels.size()))
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Level& L = p->levels[level];
+  info->nx = L.n[0];
+  info->ny = L.n[1];
+  info->nz = L.n[2];
+  info->coarsen_mask = static_cast<uint8_t>(L.coarsen_mask);
+  info->line_mask = static_cast<uint8_t>(L.line_mask);
+  info->precision = static_cast<uint8_t>(L.prec);
+  info->on_tail = L.tail ? 1 : 0;
+  return ok();
+}
+
+hf_status hf_plan_level_array(const hf_plan* p, int32_t level, int32_t which,
+                              double* out) {
+  if (p == nullptr || out == nullptr || level < 0 ||
+      level >= static_cast<int32_t>(p->levels.size()) || which < 0 || which > 4)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Level& L = p->levels[level];
+  const double* src[5] = {L.D64, L.fx64, L.fy64, L.fz64, L.vol};
+  const int64_t count[5] = {L.cells, L.nfx, L.nfy, L.nfz, L.cells};
+  if (hipMemcpyAsync(out, src[which], count[which] * sizeof(double),
+                     hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  return hip_check(hipStreamSynchronize(p->stream));
+}
+
+hf_status hf_plan_device_bytes(const hf_plan* p, uint64_t* bytes) {
+  if (p == nullptr || bytes == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  *bytes = p->device_bytes;
+  return ok();
+}
+
+hf_status hf_plan_set_profiling(hf_plan* p, int32_t enabled) {
+  (void)enabled;
+  if (p == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  return ok();
+}
+
+hf_status hf_plan_profile(const hf_plan* p, double* ms, uint64_t* launches,
+                          int32_t capacity, int32_t* count) {
+  (void)ms;
+  (void)launches;
+  (void)capacity;
+  if (p == nullptr || count == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  *count = 0;
+  return ok();
+}
+
+const char* hf_status_message(hf_status s) {
+  switch (s.code) {
+    case HF_OK: return "ok";
+    case HF_INVALID_CASE: return "invalid case";
+    case HF_INVALID_PLAN: return "invalid plan";
+    case HF_ALLOCATION_FAILURE: return "allocation failure";
+    case HF_MPI_FAILURE: return "mpi failure";
+    case HF_NUMERICAL_FAILURE: return "numerical failure";
+    case HF_REJECTED_STEP: return "rejected step";
+    case HF_IO_FAILURE: return "io failure";
+    default: return "unknown status";
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,232 @@
+/*
+ * hundun_flow.h -- C ABI of the MI355X (gfx950) pressure-projection solver.
+ *
+ * The solver replaces the CPU hot path of Hundun-Flow's low-Mach pressure
+ * step: the variable-coefficient (1/rho) 7-point cell-centred Poisson
+ * operator with immersed-boundary (IBM) masked cells, preconditioned by the
+ * native Cartesian geometric multigrid, inside a conjugate-gradient solve.
+ * Reference interfaces are cited per entry point (paths relative to
+ * versions/v0.4/ of the reference tree).
+ *
+ * Conventions
+ *  - Every array argument of a compute entry point is a DEVICE pointer
+ *    (HBM resident).  Host pointers are only taken by hf_plan_create (grid
+ *    face coordinates) and by result/out structs.
+ *  - Cell arrays are x-fastest, no ghosts: index i + nx*(j + ny*k).
+ *    x-faces: i + (nx+1)*(j + ny*k); y-faces: i + nx*(j + (ny+1)*k);
+ *    z-faces: i + nx*(j + ny*k) with k in [0, nz].
+ *  - Every call returns hf_status{code, detail}; code values equal the
+ *    reference StatusCode (include/hundun/v04_status.hpp:13-22).  Calls never
+ *    throw and never publish a solution on failure.
+ *  - One plan = one HIP stream (passed at creation, NULL = default stream).
+ *    Plans are not thread safe; calls on one plan are serialised on its
+ *    stream.
+ */
+#ifndef HUNDUN_FLOW_H
+#define HUNDUN_FLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* StatusCode, include/hundun/v04_status.hpp:13-22 */
+enum hf_status_code {
+  HF_OK = 0,
+  HF_INVALID_CASE = 1,
+  HF_INVALID_PLAN = 2,
+  HF_ALLOCATION_FAILURE = 3,
+  HF_MPI_FAILURE = 4, /* kept for numbering parity; never produced */
+  HF_NUMERICAL_FAILURE = 5,
+  HF_REJECTED_STEP = 6,
+  HF_IO_FAILURE = 7
+};
+
+typedef struct hf_status {
+  int32_t code;
+  uint32_t detail;
+} hf_status;
+
+/* MgBoundaryKind, include/hundun/v04_linear.hpp:100-104 */
+enum hf_boundary_kind {
+  HF_BC_DIRICHLET = 0,
+  HF_BC_NEUMANN = 1,
+  HF_BC_PERIODIC = 2
+};
+
+/* Storage precision of the multigrid V-cycle.  The outer Krylov loop, its
+ * residual, its inner products and the coarsest-level solve are FP64 in
+ * every mode. */
+enum hf_precision {
+  HF_PRECISION_FP64 = 0,
+  HF_PRECISION_FP32 = 1,
+  HF_PRECISION_BF16 = 2 /* BF16 storage, FP32 arithmetic */
+};
+
+/* LinearTermination, include/hundun/v04_linear.hpp:75-86 */
+enum hf_termination {
+  HF_TERM_CONVERGED = 0,
+  HF_TERM_ZERO_RHS = 1,
+  HF_TERM_MAXIMUM_ITERATIONS = 2,
+  HF_TERM_BREAKDOWN = 3,
+  HF_TERM_NON_FINITE = 4,
+  HF_TERM_OPERATOR_FAILURE = 5,
+  HF_TERM_PRECONDITIONER_FAILURE = 6,
+  HF_TERM_COLLECTIVE_FAILURE = 7,
+  HF_TERM_INVALID_PLAN = 8
+};
+
+/* Geometry: CartesianGeometryPlan axis metrics from explicit face
+ * coordinates (runtime_axes_v1, include/hundun/v04_case.hpp:28-32) and the
+ * six physical boundary kinds (MgBoundarySet, v04_linear.hpp:136-143). */
+typedef struct hf_grid_desc {
+  int32_t nx, ny, nz;
+  const double* faces_x; /* host, nx+1 increasing coordinates */
+  const double* faces_y; /* host, ny+1 */
+  const double* faces_z; /* host, nz+1 */
+  int32_t boundary[6];   /* x_min, x_max, y_min, y_max, z_min, z_max */
+} hf_grid_desc;
+
+/* MgHierarchyPolicy (v04_linear.hpp:115-131) plus the spec fields of
+ * NativeCartesianMgSpec (v04_linear.hpp:329-342) that change arithmetic. */
+typedef struct hf_policy {
+  double anisotropy_threshold;  /* 4.0; >= max/min spacing ratio keeps point RB */
+  uint8_t pre_sweeps;           /* 2 */
+  uint8_t post_sweeps;          /* 2 */
+  uint8_t maximum_levels;       /* 32 */
+  uint8_t minimum_coarse_extent;/* 3 */
+  uint16_t coarse_sweeps;       /* 24 */
+  uint8_t f_cycle;              /* 0 V-cycle, 1 F-cycle (MgCycleKind) */
+  uint8_t null_space_constant;  /* MgNullSpace::constant */
+  uint8_t unit_linear;          /* MgCorrectionScaling::unit_linear */
+  uint8_t precision;            /* enum hf_precision */
+} hf_policy;
+
+/* LinearSolveControl (v04_linear.hpp:454-466) restricted to PCG. */
+typedef struct hf_solve_control {
+  double absolute_tolerance;
+  double relative_tolerance;
+  uint32_t maximum_iterations;
+  uint32_t true_residual_interval;
+  uint8_t flexible; /* 0: beta = rho'/rho (reference); 1: Polak-Ribiere */
+} hf_solve_control;
+
+/* LinearSolveResult subset (v04_linear.hpp:530-579). */
+typedef struct hf_solve_result {
+  hf_status status;
+  int32_t termination; /* enum hf_termination */
+  uint32_t iterations;
+  double initial_true_residual;
+  double final_true_residual;
+  double recursive_residual;
+  double rhs_norm;
+  double true_residual_limit;
+  uint64_t operator_applies;
+  uint64_t preconditioner_applies;
+  uint64_t reduction_calls;
+  double solve_milliseconds; /* device time of the whole solve */
+} hf_solve_result;
+
+typedef struct hf_level_info {
+  int32_t nx, ny, nz;
+  uint8_t coarsen_mask; /* bit0 x, bit1 y, bit2 z */
+  uint8_t line_mask;
+  uint8_t precision;    /* storage precision of this level */
+  uint8_t on_tail;      /* level handled by the single-workgroup tail */
+} hf_level_info;
+
+typedef struct hf_plan hf_plan;
+
+/* Defaults of MgHierarchyPolicy (v04_linear.hpp:115-131), unit_linear,
+ * constant null space, FP64. */
+hf_status hf_policy_default(hf_policy* policy);
+
+/* Level planning + device allocation.  Replaces make_mg_workspace_requirements
+ * (src/solver_mg_workspace.cpp:317-453) and the symbolic half of
+ * NativeCartesianMgPlan::compile (src/solver_mg.cpp:4361-4602).
+ * stream: hipStream_t or NULL. */
+hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
+                         void* stream, hf_plan** out);
+hf_status hf_plan_destroy(hf_plan* plan);
+
+/* Face coefficient rule beta_f = A_f / (d_L/m_L + d_R/m_R) and D = sum of
+ * faces from a per-cell mobility m (= 1/rho).  Replaces face_pressure_
+ * coefficient / fill_pressure_coefficients (src/solver_piso.cpp:1060-1160).
+ * Outputs are device arrays sized as in the layout note. */
+hf_status hf_assemble_coefficients(hf_plan* plan, const double* mobility,
+                                   double* diag, double* bx, double* by,
+                                   double* bz);
+
+/* Numeric setup: IBM masking of level 0 (src/solver_mg.cpp:1469-1570),
+ * coarse aggregation (src/solver_mg.cpp:1706-1962) and the certified
+ * hierarchy check (src/solver_mg.cpp:1965-2019).  Replaces the numeric half of
+ * NativeCartesianMgPlan::compile and update_coefficients
+ * (src/solver_mg.cpp:4604-4795).  Activity arrays are 0/1 bytes; pass all
+ * four or all NULL (full Cartesian domain). */
+hf_status hf_plan_set_coefficients(hf_plan* plan, const double* diag,
+                                   const double* bx, const double* by,
+                                   const double* bz,
+                                   const uint8_t* cell_active,
+                                   const uint8_t* x_face_active,
+                                   const uint8_t* y_face_active,
+                                   const uint8_t* z_face_active);
+
+/* y = A x with the exact IBM pressure operator (solid rows identity, link
+ * faces removed).  Replaces IbmPressureOperator::apply
+ * (src/solver_ibm_pressure.cpp:256-273) over PressureLinearOperator::apply
+ * (src/solver_pressure.cpp:969-991). */
+hf_status hf_operator_apply(hf_plan* plan, const double* x, double* y);
+
+/* z = M r, one multigrid application.  Replaces NativeCartesianMgPlan::apply
+ * (src/solver_mg.cpp:5124-5131 -> apply_impl :4814-5122). */
+hf_status hf_mg_apply(hf_plan* plan, const double* residual,
+                      double* correction);
+
+/* PCG solve, x holds the initial guess on entry.  Replaces solve_pcg
+ * (src/solver_krylov.cpp:1299-1555) with the MG plan as preconditioner. */
+hf_status hf_pcg_solve(hf_plan* plan, const double* rhs, double* solution,
+                       const hf_solve_control* control,
+                       hf_solve_result* result);
+
+/* Adjoint gradient of a scalar loss L(p) with p the gauge-fixed pressure
+ * solution of A(rho) p = b (zero fluid mean).  Given dL/dp (grad_p) and the
+ * forward solution p, solves A lambda = P grad_p (A is symmetric, so the
+ * transpose solve is the same solve) and returns
+ *   grad_b      = P lambda                      (zero on solid cells)
+ *   grad_logrho = -sum_f (lambda_L-lambda_R)(p_L-p_R) d beta_f / d log rho
+ * using the mobility m = 1/rho that built the current coefficients.
+ * Either output may be NULL.  The hierarchy is recomputed, not stored. */
+hf_status hf_adjoint_gradients(hf_plan* plan, const double* mobility,
+                               const double* pressure, const double* grad_p,
+                               const hf_solve_control* control,
+                               double* grad_b, double* grad_logrho,
+                               hf_solve_result* result);
+
+/* Gauge: subtract the Euclidean mean over fluid cells (solids stay 0). */
+hf_status hf_project_mean(hf_plan* plan, double* field);
+
+hf_status hf_plan_level_count(const hf_plan* plan, int32_t* levels);
+hf_status hf_plan_level_info(const hf_plan* plan, int32_t level,
+                             hf_level_info* info);
+/* Copies one level array of the FP64 hierarchy to a device buffer:
+ * which = 0 D, 1 x-faces, 2 y-faces, 3 z-faces, 4 fluid volume. */
+hf_status hf_plan_level_array(const hf_plan* plan, int32_t level,
+                              int32_t which, double* out);
+/* Device bytes held by the plan (hierarchy + workspaces). */
+hf_status hf_plan_device_bytes(const hf_plan* plan, uint64_t* bytes);
+
+/* Per-kernel-class device timing of the last hf_pcg_solve when enabled
+ * (HIP events on the plan stream). */
+hf_status hf_plan_set_profiling(hf_plan* plan, int32_t enabled);
+hf_status hf_plan_profile(const hf_plan* plan, double* milliseconds,
+                          uint64_t* launches, int32_t capacity,
+                          int32_t* count);
+
+const char* hf_status_message(hf_status status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,949 @@
+/*
+ * CPU oracle for the pressure-projection hot path -- TEST INFRASTRUCTURE.
+ *
+ * A single-rank plain-C restatement of the reference algorithm (paths are
+ * relative to versions/v0.4/ of the reference tree).  It is the checker for
+ * the HIP product path and the "port" CPU baseline; it is never shipped in or
+ * called by the product library.  Build with -ffp-contract=off so that every
+ * arithmetic step rounds exactly like the reference release build.
+ *
+ * Single rank means the halo engine reduces to periodic wrap-around and the
+ * reduction engine to a local sum; both are restated as index arithmetic.
+ */
+#include "hf_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { ST_OK = 0, ST_INVALID_PLAN = 2, ST_NUMERICAL = 5, ST_REJECTED = 6 };
+enum {
+  TERM_CONVERGED = 0,
+  TERM_ZERO_RHS = 1,
+  TERM_MAXIMUM_ITERATIONS = 2,
+  TERM_BREAKDOWN = 3,
+  TERM_NON_FINITE = 4
+};
+
+#define OR_MAX_LEVELS 32
+
+typedef struct or_level {
+  int n[3];
+  int coarsen_mask; /* bit0 x, bit1 y, bit2 z: axes coarsened toward next */
+  int line_mask;
+  size_t cells, nfx, nfy, nfz;
+  double *D, *fx, *fy, *fz, *vol;
+  double *x, *b, *r, *t;
+} or_level;
+
+struct or_mg {
+  int bc[6];
+  or_policy policy;
+  int levels;
+  or_level level[OR_MAX_LEVELS];
+  uint8_t* cell_active; /* copy, level-0 cells (NULL = all active) */
+};
+
+static size_t cidx(const int n[3], int i, int j, int k) {
+  return (size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k);
+}
+static size_t xfidx(const int n[3], int i, int j, int k) {
+  return (size_t)i + (size_t)(n[0] + 1) * ((size_t)j + (size_t)n[1] * (size_t)k);
+}
+static size_t yfidx(const int n[3], int i, int j, int k) {
+  return (size_t)i + (size_t)n[0] * ((size_t)j + (size_t)(n[1] + 1) * (size_t)k);
+}
+static size_t zfidx(const int n[3], int i, int j, int k) {
+  return (size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k);
+}
+
+/* ------------------------------------------------------------------ */
+/* Face coefficient rule: src/solver_piso.cpp:1060-1118.               */
+/* Geometry metrics: src/mesh_focus.cpp:598-608 (centre = f + 0.5 w).  */
+/* Face area: src/solver_cartesian.cpp:319-331.                        */
+/* ------------------------------------------------------------------ */
+static double face_rule(const double* faces, int count, int normal, int kind,
+                        double area, double m_left, double m_right) {
+  /* m_left / m_right are the mobilities of the cells on either side; for a
+   * periodic physical face they are the wrapped cells (halo-filled). */
+  if ((normal == 0 || normal == count) && kind == OR_NEUMANN) return 0.0;
+  if ((normal == 0 || normal == count) && kind == OR_DIRICHLET) {
+    /* owner = inside cell; ghost state mobility taken equal to the owner's
+     * (homogeneous EOS-closed ghost). */
+    const double owner = normal == 0 ? m_right : m_left;
+    double distance;
+    if (normal == 0) {
+      const double w0 = faces[1] - faces[0];
+      distance = (faces[0] + 0.5 * w0) - faces[0];
+    } else {
+      const double wl = faces[count] - faces[count - 1];
+      distance = faces[count] - (faces[count - 1] + 0.5 * wl);
+    }
+    return 2.0 * area / (distance / owner + distance / owner);
+  }
+  double left_distance, right_distance;
+  if (normal == 0 || normal == count) { /* periodic physical face */
+    left_distance = 0.5 * (faces[count] - faces[count - 1]);
+    right_distance = 0.5 * (faces[1] - faces[0]);
+  } else {
+    const double location = faces[normal];
+    const double wl = faces[normal] - faces[normal - 1];
+    const double wr = faces[normal + 1] - faces[normal];
+    left_distance = location - (faces[normal - 1] + 0.5 * wl);
+    right_distance = (faces[normal] + 0.5 * wr) - location;
+  }
+  return area / (left_distance / m_left + right_distance / m_right);
+}
+
+int or_assemble(const int n[3], const double* faces_x, const double* faces_y,
+                const double* faces_z, const int bc[6], const double* m,
+                double* bx, double* by, double* bz, double* diag) {
+  const int nx = n[0], ny = n[1], nz = n[2];
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const double area = (faces_y[j + 1] - faces_y[j]) *
+                            (faces_z[k + 1] - faces_z[k]);
+        const int il = i == 0 ? nx - 1 : i - 1;
+        const int ir = i == nx ? 0 : i;
+        const int kind = i == 0 ? bc[0] : bc[1];
+        const double v = face_rule(faces_x, nx, i, kind, area,
+                                   m[cidx(n, il, j, k)], m[cidx(n, ir, j, k)]);
+        if (!isfinite(v) || v < 0.0) return ST_NUMERICAL;
+        bx[xfidx(n, i, j, k)] = v;
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double area = (faces_x[i + 1] - faces_x[i]) *
+                            (faces_z[k + 1] - faces_z[k]);
+        const int jl = j == 0 ? ny - 1 : j - 1;
+        const int jr = j == ny ? 0 : j;
+        const int kind = j == 0 ? bc[2] : bc[3];
+        const double v = face_rule(faces_y, ny, j, kind, area,
+                                   m[cidx(n, i, jl, k)], m[cidx(n, i, jr, k)]);
+        if (!isfinite(v) || v < 0.0) return ST_NUMERICAL;
+        by[yfidx(n, i, j, k)] = v;
+      }
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double area = (faces_x[i + 1] - faces_x[i]) *
+                            (faces_y[j + 1] - faces_y[j]);
+        const int kl = k == 0 ? nz - 1 : k - 1;
+        const int kr = k == nz ? 0 : k;
+        const int kind = k == 0 ? bc[4] : bc[5];
+        const double v = face_rule(faces_z, nz, k, kind, area,
+                                   m[cidx(n, i, j, kl)], m[cidx(n, i, j, kr)]);
+        if (!isfinite(v) || v < 0.0) return ST_NUMERICAL;
+        bz[zfidx(n, i, j, k)] = v;
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        diag[cidx(n, i, j, k)] =
+            bx[xfidx(n, i, j, k)] + bx[xfidx(n, i + 1, j, k)] +
+            by[yfidx(n, i, j, k)] + by[yfidx(n, i, j + 1, k)] +
+            bz[zfidx(n, i, j, k)] + bz[zfidx(n, i, j, k + 1)];
+  return ST_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Out-of-patch neighbour: src/solver_piso.cpp:1553-1595 (operator)     */
+/* and src/solver_mg.cpp:2237-2272 / :2356-2368 (MG point rows).        */
+/* ------------------------------------------------------------------ */
+static double neighbour(const double* x, const int n[3], const int bc[6],
+                        int i, int j, int k, int axis, int dir) {
+  int c[3] = {i, j, k};
+  c[axis] += dir;
+  if (c[axis] >= 0 && c[axis] < n[axis]) return x[cidx(n, c[0], c[1], c[2])];
+  const int kind = bc[2 * axis + (dir > 0 ? 1 : 0)];
+  if (kind == OR_DIRICHLET) return 0.0;
+  if (kind == OR_NEUMANN) return x[cidx(n, i, j, k)];
+  c[axis] = dir < 0 ? n[axis] - 1 : 0;
+  return x[cidx(n, c[0], c[1], c[2])];
+}
+
+/* Stencil in the reference evaluation order (src/solver_mg.cpp:2405-2411,
+ * src/solver_pressure.cpp:757-798): D*x, then x-lo, x-hi, y-lo, y-hi, z-lo,
+ * z-hi, each subtracted in turn. */
+static double row_value(const int n[3], const int bc[6], const double* D,
+                        const double* fx, const double* fy, const double* fz,
+                        const double* x, int i, int j, int k) {
+  double v = D[cidx(n, i, j, k)] * x[cidx(n, i, j, k)];
+  v -= fx[xfidx(n, i, j, k)] * neighbour(x, n, bc, i, j, k, 0, -1);
+  v -= fx[xfidx(n, i + 1, j, k)] * neighbour(x, n, bc, i, j, k, 0, 1);
+  v -= fy[yfidx(n, i, j, k)] * neighbour(x, n, bc, i, j, k, 1, -1);
+  v -= fy[yfidx(n, i, j + 1, k)] * neighbour(x, n, bc, i, j, k, 1, 1);
+  v -= fz[zfidx(n, i, j, k)] * neighbour(x, n, bc, i, j, k, 2, -1);
+  v -= fz[zfidx(n, i, j, k + 1)] * neighbour(x, n, bc, i, j, k, 2, 1);
+  return v;
+}
+
+void or_operator_apply(const int n[3], const int bc[6], const double* diag,
+                       const double* bx, const double* by, const double* bz,
+                       const uint8_t* act, const double* x, double* y) {
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i)
+        y[cidx(n, i, j, k)] = row_value(n, bc, diag, bx, by, bz, x, i, j, k);
+  if (act == NULL) return;
+  /* IBM decoration, src/solver_ibm_pressure.cpp:347-385: solid rows become
+   * identity; each fluid/solid link adds beta*(x_solid - x_fluid) to the
+   * fluid row.  Links are enumerated per solid cell and axis direction. */
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        const size_t s = cidx(n, i, j, k);
+        if (act[s]) continue;
+        y[s] = x[s];
+      }
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        const size_t f = cidx(n, i, j, k);
+        if (!act[f]) continue;
+        for (int axis = 0; axis < 3; ++axis)
+          for (int dir = -1; dir <= 1; dir += 2) {
+            int c[3] = {i, j, k};
+            c[axis] += dir;
+            if (c[axis] < 0 || c[axis] >= n[axis]) {
+              if (bc[2 * axis + (dir > 0)] != OR_PERIODIC) continue;
+              c[axis] = dir < 0 ? n[axis] - 1 : 0;
+            }
+            const size_t s = cidx(n, c[0], c[1], c[2]);
+            if (act[s]) continue;
+            double face;
+            if (axis == 0)
+              face = bx[xfidx(n, dir < 0 ? i : i + 1, j, k)];
+            else if (axis == 1)
+              face = by[yfidx(n, i, dir < 0 ? j : j + 1, k)];
+            else
+              face = bz[zfidx(n, i, j, dir < 0 ? k : k + 1)];
+            y[f] += face * (x[s] - x[f]);
+          }
+      }
+}
+
+/* ------------------------------------------------------------------ */
+/* Level planner: src/solver_mg_workspace.cpp:113-143 (strategy mask),  */
+/* :181-199 (coarsen_axis), :317-453 (level loop).                      */
+/* ------------------------------------------------------------------ */
+static void spacing(const double* w, int count, double* mn, double* mx,
+                    double* rep) {
+  *mn = INFINITY;
+  *mx = 0.0;
+  *rep = 0.0;
+  for (int i = 0; i < count; ++i) {
+    if (w[i] < *mn) *mn = w[i];
+    if (w[i] > *mx) *mx = w[i];
+    *rep += w[i];
+  }
+  *rep /= (double)count;
+}
+
+static int strategy_mask(const int n[3], const double* const w[3],
+                         double threshold, int* line_mask) {
+  double mn[3], mx[3], rep[3];
+  for (int a = 0; a < 3; ++a) spacing(w[a], n[a], &mn[a], &mx[a], &rep[a]);
+  double largest = rep[0];
+  if (rep[1] > largest) largest = rep[1];
+  if (rep[2] > largest) largest = rep[2];
+  *line_mask = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int internally = mn[a] > 0.0 && mx[a] / mn[a] >= threshold;
+    const int cross = rep[a] > 0.0 && largest / rep[a] >= threshold;
+    if (internally || cross) *line_mask |= 1 << a;
+  }
+  if ((*line_mask & (*line_mask - 1)) != 0) return 7;
+  const int candidate = 7 & ~*line_mask;
+  return candidate == 0 ? 7 : candidate;
+}
+
+/* ------------------------------------------------------------------ */
+/* Hierarchy coefficients.                                              */
+/* ------------------------------------------------------------------ */
+static double faces_sum(const or_level* L, const double* fx, const double* fy,
+                        const double* fz, int i, int j, int k) {
+  return fx[xfidx(L->n, i, j, k)] + fx[xfidx(L->n, i + 1, j, k)] +
+         fy[yfidx(L->n, i, j, k)] + fy[yfidx(L->n, i, j + 1, k)] +
+         fz[zfidx(L->n, i, j, k)] + fz[zfidx(L->n, i, j, k + 1)];
+}
+
+/* src/solver_mg.cpp:1469-1570 */
+static void copy_finest(or_mg* mg, const double* const w[3], const double* D,
+                        const double* bx, const double* by, const double* bz,
+                        const uint8_t* ca, const uint8_t* fxa,
+                        const uint8_t* fya, const uint8_t* fza) {
+  or_level* L = &mg->level[0];
+  const int* n = L->n;
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        const size_t c = cidx(n, i, j, k);
+        L->D[c] = D[c];
+        L->vol[c] = w[0][i] * w[1][j] * w[2][k];
+        if (ca != NULL && ca[c] == 0) L->vol[c] = 0.0;
+      }
+  for (size_t f = 0; f < L->nfx; ++f)
+    L->fx[f] = bx[f] * (fxa == NULL ? 1.0 : (double)fxa[f]);
+  for (size_t f = 0; f < L->nfy; ++f)
+    L->fy[f] = by[f] * (fya == NULL ? 1.0 : (double)fya[f]);
+  for (size_t f = 0; f < L->nfz; ++f)
+    L->fz[f] = bz[f] * (fza == NULL ? 1.0 : (double)fza[f]);
+  if (ca == NULL) return;
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        const size_t c = cidx(n, i, j, k);
+        if (ca[c] == 0) {
+          L->D[c] = 1.0;
+          continue;
+        }
+        const double original =
+            bx[xfidx(n, i, j, k)] + bx[xfidx(n, i + 1, j, k)] +
+            by[yfidx(n, i, j, k)] + by[yfidx(n, i, j + 1, k)] +
+            bz[zfidx(n, i, j, k)] + bz[zfidx(n, i, j, k + 1)];
+        const double active = faces_sum(L, L->fx, L->fy, L->fz, i, j, k);
+        L->D[c] = L->D[c] - original + active;
+      }
+}
+
+/* Channel aggregation, src/solver_mg.cpp:1706-1856 (single rank):
+ * x -> y -> z over the coarsened axes; cells and tangential faces sum their
+ * two children, normal faces take the aligned fine face min(2c, n). */
+static void aggregate(const or_level* F, int face_axis, const double* src,
+                      double* dst, double* s1, double* s2) {
+  int ext[3] = {F->n[0], F->n[1], F->n[2]};
+  int cells[3] = {F->n[0], F->n[1], F->n[2]};
+  if (face_axis >= 0) ext[face_axis] += 1;
+  size_t count = (size_t)ext[0] * ext[1] * ext[2];
+  memcpy(s1, src, count * sizeof(double));
+  double* cur = s1;
+  double* nxt = s2;
+  for (int axis = 0; axis < 3; ++axis) {
+    if ((F->coarsen_mask & (1 << axis)) == 0) continue;
+    const int global_extent = cells[axis];
+    int ncells[3] = {cells[0], cells[1], cells[2]};
+    ncells[axis] = (cells[axis] + 1) / 2;
+    int next[3] = {ncells[0], ncells[1], ncells[2]};
+    if (face_axis >= 0) next[face_axis] += 1;
+    for (int k = 0; k < next[2]; ++k)
+      for (int j = 0; j < next[1]; ++j)
+        for (int i = 0; i < next[0]; ++i) {
+          int o[3] = {i, j, k};
+          double value = 0.0;
+          if (face_axis == axis) {
+            int s[3] = {i, j, k};
+            s[axis] = 2 * o[axis] < global_extent ? 2 * o[axis] : global_extent;
+            value = cur[(size_t)s[0] + (size_t)ext[0] * ((size_t)s[1] + (size_t)ext[1] * s[2])];
+          } else {
+            for (int child = 0; child < 2; ++child) {
+              const int fg = 2 * o[axis] + child;
+              if (fg >= global_extent) continue;
+              int s[3] = {i, j, k};
+              s[axis] = fg;
+              value += cur[(size_t)s[0] + (size_t)ext[0] * ((size_t)s[1] + (size_t)ext[1] * s[2])];
+            }
+          }
+          nxt[(size_t)i + (size_t)next[0] * ((size_t)j + (size_t)next[1] * k)] = value;
+        }
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+    memcpy(cells, ncells, sizeof cells);
+    memcpy(ext, next, sizeof ext);
+  }
+  count = (size_t)ext[0] * ext[1] * ext[2];
+  memcpy(dst, cur, count * sizeof(double));
+}
+
+/* src/solver_mg.cpp:1875-1962 */
+static int build_coarse(or_mg* mg) {
+  size_t maxv = 0;
+  for (int l = 0; l < mg->levels; ++l) {
+    const or_level* L = &mg->level[l];
+    if (L->cells > maxv) maxv = L->cells;
+    if (L->nfx > maxv) maxv = L->nfx;
+    if (L->nfy > maxv) maxv = L->nfy;
+    if (L->nfz > maxv) maxv = L->nfz;
+  }
+  double* s1 = (double*)malloc(maxv * sizeof(double));
+  double* s2 = (double*)malloc(maxv * sizeof(double));
+  double* tmp = (double*)malloc(maxv * sizeof(double));
+  int status = ST_OK;
+  for (int l = 1; l < mg->levels && status == ST_OK; ++l) {
+    const or_level* F = &mg->level[l - 1];
+    or_level* C = &mg->level[l];
+    aggregate(F, -1, F->vol, C->vol, s1, s2);
+    for (int k = 0; k < F->n[2]; ++k)
+      for (int j = 0; j < F->n[1]; ++j)
+        for (int i = 0; i < F->n[0]; ++i) {
+          const size_t c = cidx(F->n, i, j, k);
+          if (F->vol[c] == 0.0) {
+            tmp[c] = 0.0;
+            continue;
+          }
+          const double faces = faces_sum(F, F->fx, F->fy, F->fz, i, j, k);
+          const double reaction = F->D[c] - faces;
+          tmp[c] = reaction > 0.0 ? reaction : 0.0;
+        }
+    aggregate(F, -1, tmp, C->D, s1, s2);
+    aggregate(F, 0, F->fx, C->fx, s1, s2);
+    aggregate(F, 1, F->fy, C->fy, s1, s2);
+    aggregate(F, 2, F->fz, C->fz, s1, s2);
+    for (int k = 0; k < C->n[2]; ++k)
+      for (int j = 0; j < C->n[1]; ++j)
+        for (int i = 0; i < C->n[0]; ++i) {
+          const size_t c = cidx(C->n, i, j, k);
+          C->D[c] += faces_sum(C, C->fx, C->fy, C->fz, i, j, k);
+          if (C->vol[c] == 0.0) C->D[c] = 1.0;
+          if (C->vol[c] < 0.0 || !isfinite(C->vol[c]) || !(C->D[c] > 0.0) ||
+              !isfinite(C->D[c]))
+            status = ST_NUMERICAL;
+        }
+  }
+  free(s1);
+  free(s2);
+  free(tmp);
+  return status;
+}
+
+or_mg* or_mg_create(const int n[3], const double* wx, const double* wy,
+                    const double* wz, const int bc[6], const or_policy* policy,
+                    const double* diag, const double* bx, const double* by,
+                    const double* bz, const uint8_t* ca, const uint8_t* fxa,
+                    const uint8_t* fya, const uint8_t* fza, int* status) {
+  *status = ST_INVALID_PLAN;
+  if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return NULL;
+  for (int a = 0; a < 3; ++a)
+    if ((bc[2 * a] == OR_PERIODIC) != (bc[2 * a + 1] == OR_PERIODIC))
+      return NULL;
+  if (policy->pre_sweeps <= 0 || policy->post_sweeps <= 0 ||
+      policy->coarse_sweeps <= 0 || policy->maximum_levels < 2 ||
+      policy->maximum_levels > OR_MAX_LEVELS ||
+      policy->minimum_coarse_extent < 2 || !(policy->anisotropy_threshold > 1.0))
+    return NULL;
+  or_mg* mg = (or_mg*)calloc(1, sizeof(or_mg));
+  memcpy(mg->bc, bc, sizeof mg->bc);
+  mg->policy = *policy;
+  const double* w[3] = {wx, wy, wz};
+  int line_mask = 0;
+  const int preferred = strategy_mask(n, w, policy->anisotropy_threshold,
+                                      &line_mask);
+  int g[3] = {n[0], n[1], n[2]};
+  for (int l = 0; l < policy->maximum_levels; ++l) {
+    or_level* L = &mg->level[l];
+    memcpy(L->n, g, sizeof g);
+    L->line_mask = line_mask;
+    int next[3] = {g[0], g[1], g[2]};
+    int mask = 0;
+    for (int a = 0; a < 3; ++a) {
+      if ((preferred & (1 << a)) == 0) continue;
+      if (g[a] <= policy->minimum_coarse_extent) continue;
+      next[a] = (g[a] + 1) / 2;
+      mask |= 1 << a;
+    }
+    L->coarsen_mask = mask;
+    mg->levels = l + 1;
+    if (mask == 0 || mg->levels == policy->maximum_levels) break;
+    memcpy(g, next, sizeof g);
+  }
+  if (mg->levels < 2 || line_mask != 0) {
+    /* Line relaxation levels are not restated by this oracle. */
+    free(mg);
+    return NULL;
+  }
+  for (int l = 0; l < mg->levels; ++l) {
+    or_level* L = &mg->level[l];
+    L->cells = (size_t)L->n[0] * L->n[1] * L->n[2];
+    L->nfx = (size_t)(L->n[0] + 1) * L->n[1] * L->n[2];
+    L->nfy = (size_t)L->n[0] * (L->n[1] + 1) * L->n[2];
+    L->nfz = (size_t)L->n[0] * L->n[1] * (L->n[2] + 1);
+    L->D = (double*)calloc(L->cells, sizeof(double));
+    L->vol = (double*)calloc(L->cells, sizeof(double));
+    L->fx = (double*)calloc(L->nfx, sizeof(double));
+    L->fy = (double*)calloc(L->nfy, sizeof(double));
+    L->fz = (double*)calloc(L->nfz, sizeof(double));
+    L->x = (double*)calloc(L->cells, sizeof(double));
+    L->b = (double*)calloc(L->cells, sizeof(double));
+    L->r = (double*)calloc(L->cells, sizeof(double));
+    L->t = (double*)calloc(L->cells, sizeof(double));
+  }
+  if (ca != NULL) {
+    mg->cell_active = (uint8_t*)malloc(mg->level[0].cells);
+    memcpy(mg->cell_active, ca, mg->level[0].cells);
+  }
+  copy_finest(mg, w, diag, bx, by, bz, ca, fxa, fya, fza);
+  *status = build_coarse(mg);
+  if (*status != ST_OK) {
+    or_mg_destroy(mg);
+    return NULL;
+  }
+  return mg;
+}
+
+void or_mg_destroy(or_mg* mg) {
+  if (mg == NULL) return;
+  for (int l = 0; l < mg->levels; ++l) {
+    or_level* L = &mg->level[l];
+    free(L->D); free(L->vol); free(L->fx); free(L->fy); free(L->fz);
+    free(L->x); free(L->b); free(L->r); free(L->t);
+  }
+  free(mg->cell_active);
+  free(mg);
+}
+
+int or_mg_level_count(const or_mg* mg) { return mg->levels; }
+
+int or_mg_level_info(const or_mg* mg, int level, int* shape, int* cm,
+                     int* lm) {
+  if (level < 0 || level >= mg->levels) return ST_INVALID_PLAN;
+  memcpy(shape, mg->level[level].n, 3 * sizeof(int));
+  *cm = mg->level[level].coarsen_mask;
+  *lm = mg->level[level].line_mask;
+  return ST_OK;
+}
+
+const double* or_mg_level_array(const or_mg* mg, int level, int which) {
+  const or_level* L = &mg->level[level];
+  switch (which) {
+    case 0: return L->D;
+    case 1: return L->fx;
+    case 2: return L->fy;
+    case 3: return L->fz;
+    case 4: return L->vol;
+    case 5: return L->x;
+    case 6: return L->b;
+    case 7: return L->r;
+    default: return NULL;
+  }
+}
+
+double* or_mg_level_slot(or_mg* mg, int level, int which) {
+  return (double*)or_mg_level_array(mg, level, which);
+}
+
+static void level_matrix(const or_mg* mg, const or_level* L, const double* x,
+                         double* out) {
+  for (int k = 0; k < L->n[2]; ++k)
+    for (int j = 0; j < L->n[1]; ++j)
+      for (int i = 0; i < L->n[0]; ++i)
+        out[cidx(L->n, i, j, k)] =
+            row_value(L->n, mg->bc, L->D, L->fx, L->fy, L->fz, x, i, j, k);
+}
+
+static void level_defect(const or_mg* mg, const or_level* L, const double* x,
+                         const double* b, double* out) {
+  for (int k = 0; k < L->n[2]; ++k)
+    for (int j = 0; j < L->n[1]; ++j)
+      for (int i = 0; i < L->n[0]; ++i) {
+        const size_t c = cidx(L->n, i, j, k);
+        out[c] = b[c] - row_value(L->n, mg->bc, L->D, L->fx, L->fy, L->fz, x,
+                                  i, j, k);
+      }
+}
+
+void or_masked_apply(const or_mg* mg, const double* x, double* y) {
+  level_matrix(mg, &mg->level[0], x, y);
+}
+
+/* src/solver_mg.cpp:3112-3126: the fused colour update is allowed unless a
+ * periodic axis has an odd extent on this level. */
+static int fused_allowed(const or_mg* mg, const or_level* L) {
+  for (int a = 0; a < 3; ++a)
+    if (mg->bc[2 * a] == OR_PERIODIC && (L->n[a] & 1)) return 0;
+  return 1;
+}
+
+/* Red-black point smoother, src/solver_mg.cpp:3149-3349 (omega 0.72). */
+static void point_smooth(or_mg* mg, int level, int sweeps, int reverse,
+                         int retain) {
+  or_level* L = &mg->level[level];
+  const double omega = 0.72;
+  const int fused = fused_allowed(mg, L);
+  const int* n = L->n;
+  if (!fused || sweeps == 0) {
+    if (retain && sweeps == 0)
+      level_defect(mg, L, L->x, L->b, L->r);
+    else
+      level_matrix(mg, L, L->x, L->r);
+  }
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    const int first = reverse ? 1 : 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int color = (first + pass) & 1;
+      /* Same-colour cells never neighbour under the fused-allowed rule, so
+       * the update is order independent; the reference loop order is kept. */
+      for (int kk = 0; kk < n[2]; ++kk) {
+        const int k = reverse ? n[2] - 1 - kk : kk;
+        for (int jj = 0; jj < n[1]; ++jj) {
+          const int j = reverse ? n[1] - 1 - jj : jj;
+          for (int ii = 0; ii < n[0]; ++ii) {
+            const int i = reverse ? n[0] - 1 - ii : ii;
+            if (((i + j + k) & 1) != color) continue;
+            const size_t c = cidx(n, i, j, k);
+            const double value =
+                fused ? row_value(n, mg->bc, L->D, L->fx, L->fy, L->fz, L->x,
+                                  i, j, k)
+                      : L->r[c];
+            L->x[c] += omega * (L->b[c] - value) / L->D[c];
+          }
+        }
+      }
+      const int final_update = sweep + 1 == sweeps && pass == 1;
+      if (!final_update || retain) {
+        if (!fused || (final_update && retain)) {
+          if (retain && final_update)
+            level_defect(mg, L, L->x, L->b, L->r);
+          else
+            level_matrix(mg, L, L->x, L->r);
+        }
+      }
+    }
+  }
+}
+
+int or_mg_smooth(or_mg* mg, int level, int sweeps, int reverse, int retain) {
+  if (level < 0 || level >= mg->levels) return ST_INVALID_PLAN;
+  point_smooth(mg, level, sweeps, reverse, retain);
+  return ST_OK;
+}
+
+/* src/solver_mg.cpp:1412-1454: volume-weighted mean removal. */
+static int project_constant(const or_level* L, double* field) {
+  double weighted = 0.0, volume = 0.0;
+  for (size_t c = 0; c < L->cells; ++c) {
+    weighted += L->vol[c] * field[c];
+    volume += L->vol[c];
+  }
+  if (!(volume > 0.0) || !isfinite(weighted) || !isfinite(volume))
+    return ST_NUMERICAL;
+  const double mean = weighted / volume;
+  for (size_t c = 0; c < L->cells; ++c)
+    field[c] = L->vol[c] > 0.0 ? field[c] - mean : 0.0;
+  return ST_OK;
+}
+
+/* src/solver_mg.cpp:3633-3720: coarse rhs = sum of children (dz, dy, dx
+ * order), coarse x := 0, optional constant projection. */
+int or_mg_restrict(or_mg* mg, int fine) {
+  const or_level* F = &mg->level[fine];
+  or_level* C = &mg->level[fine + 1];
+  const int mx = F->coarsen_mask & 1, my = (F->coarsen_mask >> 1) & 1,
+            mz = (F->coarsen_mask >> 2) & 1;
+  for (int ck = 0; ck < C->n[2]; ++ck)
+    for (int cj = 0; cj < C->n[1]; ++cj)
+      for (int ci = 0; ci < C->n[0]; ++ci) {
+        const int xb = mx ? 2 * ci : ci, yb = my ? 2 * cj : cj,
+                  zb = mz ? 2 * ck : ck;
+        double sum = 0.0;
+        for (int dz = 0; dz < (mz ? 2 : 1); ++dz)
+          for (int dy = 0; dy < (my ? 2 : 1); ++dy)
+            for (int dx = 0; dx < (mx ? 2 : 1); ++dx) {
+              const int fx = xb + dx, fy = yb + dy, fz = zb + dz;
+              if (fx >= F->n[0] || fy >= F->n[1] || fz >= F->n[2]) continue;
+              sum += F->r[cidx(F->n, fx, fy, fz)];
+            }
+        const size_t c = cidx(C->n, ci, cj, ck);
+        C->b[c] = sum;
+        C->x[c] = 0.0;
+      }
+  if (mg->policy.null_space_constant) return project_constant(C, C->b);
+  return ST_OK;
+}
+
+static void interpolation(int fine_global, int coarsened, int coarse_extent,
+                          int periodic, int* left, int* right, double* rw) {
+  if (!coarsened) {
+    *left = fine_global;
+    *right = fine_global;
+    *rw = 0.0;
+    return;
+  }
+  const int parent = fine_global / 2;
+  if ((fine_global & 1) == 0) {
+    *left = parent - 1;
+    *right = parent;
+    *rw = 0.75;
+  } else {
+    *left = parent;
+    *right = parent + 1;
+    *rw = 0.25;
+  }
+  if (!periodic && *left < 0) {
+    *left = 0;
+    *right = 0;
+    *rw = 0.0;
+  } else if (!periodic && *right >= coarse_extent) {
+    *left = coarse_extent - 1;
+    *right = coarse_extent - 1;
+    *rw = 0.0;
+  }
+}
+
+static int wrap(int v, int extent) {
+  if (v < 0) return v + extent;
+  if (v >= extent) return v - extent;
+  return v;
+}
+
+/* src/solver_mg.cpp:3893-3993: trilinear (0.75/0.25) add, periodic via the
+ * wrapped halo, clamp at physical faces, inactive fine cells set to 0. */
+int or_mg_prolongate(or_mg* mg, int fine) {
+  or_level* F = &mg->level[fine];
+  const or_level* C = &mg->level[fine + 1];
+  const int m[3] = {F->coarsen_mask & 1, (F->coarsen_mask >> 1) & 1,
+                    (F->coarsen_mask >> 2) & 1};
+  const int per[3] = {mg->bc[0] == OR_PERIODIC, mg->bc[2] == OR_PERIODIC,
+                      mg->bc[4] == OR_PERIODIC};
+  for (int k = 0; k < F->n[2]; ++k)
+    for (int j = 0; j < F->n[1]; ++j)
+      for (int i = 0; i < F->n[0]; ++i) {
+        const size_t c = cidx(F->n, i, j, k);
+        if (F->vol[c] == 0.0) {
+          F->x[c] = 0.0;
+          continue;
+        }
+        int xl, xr, yl, yr, zl, zr;
+        double xw, yw, zw;
+        interpolation(i, m[0], C->n[0], per[0], &xl, &xr, &xw);
+        interpolation(j, m[1], C->n[1], per[1], &yl, &yr, &yw);
+        interpolation(k, m[2], C->n[2], per[2], &zl, &zr, &zw);
+        double value = 0.0;
+        for (int az = 0; az < (zl == zr ? 1 : 2); ++az) {
+          const int gz = az == 0 ? zl : zr;
+          const double wz = az == 0 ? 1.0 - zw : zw;
+          for (int ay = 0; ay < (yl == yr ? 1 : 2); ++ay) {
+            const int gy = ay == 0 ? yl : yr;
+            const double wy = ay == 0 ? 1.0 - yw : yw;
+            for (int ax = 0; ax < (xl == xr ? 1 : 2); ++ax) {
+              const int gx = ax == 0 ? xl : xr;
+              const double wx = ax == 0 ? 1.0 - xw : xw;
+              value += wx * wy * wz *
+                       C->x[cidx(C->n, wrap(gx, C->n[0]), wrap(gy, C->n[1]),
+                                 wrap(gz, C->n[2]))];
+            }
+          }
+        }
+        F->x[c] += value;
+      }
+  return ST_OK;
+}
+
+/* src/solver_mg.cpp:4192-4338 */
+static int mg_cycle(or_mg* mg, int level, int counter) {
+  if (level + 1 == mg->levels) {
+    point_smooth(mg, level, mg->policy.coarse_sweeps, 0, 0);
+    return ST_OK;
+  }
+  point_smooth(mg, level, mg->policy.pre_sweeps, 0, 1);
+  int st = or_mg_restrict(mg, level);
+  if (st != ST_OK) return st;
+  st = mg_cycle(mg, level + 1, counter);
+  if (st != ST_OK) return st;
+  if (counter > 1) {
+    st = mg_cycle(mg, level + 1, counter - 1);
+    if (st != ST_OK) return st;
+  }
+  or_mg_prolongate(mg, level);
+  point_smooth(mg, level, mg->policy.post_sweeps, 1, 0);
+  return ST_OK;
+}
+
+/* NativeCartesianMgPlan::apply_impl, src/solver_mg.cpp:4814-5122. */
+int or_mg_apply(or_mg* mg, const double* residual, double* correction) {
+  or_level* L = &mg->level[0];
+  const uint8_t* act = mg->cell_active;
+  for (size_t c = 0; c < L->cells; ++c) {
+    L->x[c] = 0.0;
+    L->b[c] = (act == NULL || act[c]) ? residual[c] : 0.0;
+  }
+  int st;
+  if (mg->policy.null_space_constant) {
+    st = project_constant(L, L->b);
+    if (st != ST_OK) return st;
+  }
+  st = mg_cycle(mg, 0, mg->policy.f_cycle ? 2 : 1);
+  if (st != ST_OK) return st;
+  if (mg->policy.null_space_constant) {
+    st = project_constant(L, L->x);
+    if (st != ST_OK) return st;
+  }
+  level_defect(mg, L, L->x, L->b, L->r);
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  for (size_t c = 0; c < L->cells; ++c) {
+    const double r = (act == NULL || act[c]) ? residual[c] : 0.0;
+    const double az = r - L->r[c];
+    p0 += r * r;
+    p1 += r * az;
+    p2 += az * az;
+  }
+  if (!isfinite(p0) || !isfinite(p1) || !isfinite(p2) || p0 < 0.0 || p2 < 0.0)
+    return ST_NUMERICAL;
+  const double tiny = DBL_EPSILON * p0;
+  const double alpha = mg->policy.unit_linear ? 1.0
+                                              : (p2 > tiny ? p1 / p2 : 0.0);
+  if (!isfinite(alpha)) return ST_NUMERICAL;
+  for (size_t c = 0; c < L->cells; ++c) {
+    const int a = act == NULL || act[c];
+    L->x[c] = a ? L->x[c] * alpha : 0.0;
+    const double r = a ? residual[c] : 0.0;
+    L->r[c] = r - alpha * (r - L->r[c]);
+  }
+  memcpy(correction, L->x, L->cells * sizeof(double));
+  return ST_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Krylov vector kernels, src/solver_krylov.cpp:176-204 and :650-711.   */
+/* ------------------------------------------------------------------ */
+double or_dot(const double* a, const double* b, size_t n) {
+  double sum = 0.0, correction = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double product = a[i] * b[i];
+    const double next = sum + product;
+    if (fabs(sum) >= fabs(product))
+      correction += (sum - next) + product;
+    else
+      correction += (product - next) + sum;
+    sum = next;
+  }
+  return sum + correction;
+}
+
+double or_norm2(const double* v, size_t n) {
+  double scale = 0.0;
+  for (size_t i = 0; i < n; ++i)
+    if (fabs(v[i]) > scale) scale = fabs(v[i]);
+  if (scale == 0.0) return 0.0;
+  double squares = 0.0, correction = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double s = v[i] / scale;
+    const double sq = s * s;
+    const double next = squares + sq;
+    if (fabs(squares) >= fabs(sq))
+      correction += (squares - next) + sq;
+    else
+      correction += (sq - next) + squares;
+    squares = next;
+  }
+  squares += correction;
+  return scale * sqrt(squares);
+}
+
+/* solve_pcg, src/solver_krylov.cpp:1299-1555: true residual every
+ * true_residual_interval iterations (and at the recursive gate); an
+ * unconverged verification restarts the direction p = M r. */
+int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
+                 const double* bx, const double* by, const double* bz,
+                 const uint8_t* act, or_mg* mg, const double* rhs,
+                 double* solution, const or_control* ctl, or_result* res) {
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  memset(res, 0, sizeof *res);
+  double* x = (double*)malloc(N * sizeof(double));
+  double* r = (double*)malloc(N * sizeof(double));
+  double* z = (double*)malloc(N * sizeof(double));
+  double* p = (double*)malloc(N * sizeof(double));
+  double* ap = (double*)malloc(N * sizeof(double));
+  int st = ST_OK;
+#define OR_FAIL(code, term) \
+  do {                      \
+    st = (code);            \
+    res->termination = (term); \
+    goto done;              \
+  } while (0)
+  const double rhs_norm = or_norm2(rhs, N);
+  res->rhs_norm = rhs_norm;
+  const double tol = fmax(ctl->absolute_tolerance,
+                          ctl->relative_tolerance * rhs_norm);
+  if (rhs_norm == 0.0) {
+    memset(solution, 0, N * sizeof(double));
+    res->termination = TERM_ZERO_RHS;
+    goto done;
+  }
+  memcpy(x, solution, N * sizeof(double));
+  or_operator_apply(n, bc, diag, bx, by, bz, act, x, ap);
+  res->operator_applies++;
+  for (size_t i = 0; i < N; ++i) r[i] = 1.0 * rhs[i] + -1.0 * ap[i];
+  res->initial_true_residual = or_norm2(r, N);
+  res->final_true_residual = res->initial_true_residual;
+  if (res->initial_true_residual <= tol) {
+    memcpy(solution, x, N * sizeof(double));
+    res->termination = TERM_CONVERGED;
+    goto done;
+  }
+#define OR_PRECONDITION()                                             \
+  do {                                                                \
+    if (mg != NULL) {                                                 \
+      const int pst = or_mg_apply(mg, r, z);                          \
+      if (pst != ST_OK) OR_FAIL(pst, TERM_NON_FINITE);                \
+    } else {                                                          \
+      memcpy(z, r, N * sizeof(double));                               \
+    }                                                                 \
+    res->preconditioner_applies++;                                    \
+  } while (0)
+  OR_PRECONDITION();
+  double rho = or_dot(r, z, N);
+  if (!(rho > 0.0) || !isfinite(rho)) OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
+  memcpy(p, z, N * sizeof(double));
+  while (res->iterations < ctl->maximum_iterations) {
+    or_operator_apply(n, bc, diag, bx, by, bz, act, p, ap);
+    res->operator_applies++;
+    const double denominator = or_dot(p, ap, N);
+    if (!(denominator > 0.0) || !isfinite(denominator))
+      OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
+    const double alpha = rho / denominator;
+    if (!isfinite(alpha)) OR_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+    for (size_t i = 0; i < N; ++i) x[i] = x[i] + alpha * p[i];
+    for (size_t i = 0; i < N; ++i) r[i] = r[i] + -alpha * ap[i];
+    res->iterations++;
+    const double recursive = or_norm2(r, N);
+    const int verify = recursive <= tol ||
+                       res->iterations % ctl->true_residual_interval == 0 ||
+                       res->iterations == ctl->maximum_iterations;
+    if (verify) {
+      or_operator_apply(n, bc, diag, bx, by, bz, act, x, ap);
+      res->operator_applies++;
+      for (size_t i = 0; i < N; ++i) r[i] = 1.0 * rhs[i] + -1.0 * ap[i];
+      res->final_true_residual = or_norm2(r, N);
+      if (res->final_true_residual <= tol) {
+        memcpy(solution, x, N * sizeof(double));
+        res->termination = TERM_CONVERGED;
+        goto done;
+      }
+      if (res->iterations == ctl->maximum_iterations)
+        OR_FAIL(ST_REJECTED, TERM_MAXIMUM_ITERATIONS);
+      OR_PRECONDITION();
+      rho = or_dot(r, z, N);
+      if (!(rho > 0.0) || !isfinite(rho)) OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
+      memcpy(p, z, N * sizeof(double));
+      continue;
+    }
+    OR_PRECONDITION();
+    const double rho_next = or_dot(r, z, N);
+    if (!(rho_next > 0.0) || !isfinite(rho_next))
+      OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
+    double beta;
+    if (ctl->flexible) {
+      /* Polak-Ribiere: z+ . (r+ - r) / rho, with r+ - r = -alpha * Ap. */
+      beta = -alpha * or_dot(z, ap, N) / rho;
+    } else {
+      beta = rho_next / rho;
+    }
+    for (size_t i = 0; i < N; ++i) p[i] = 1.0 * z[i] + beta * p[i];
+    rho = rho_next;
+  }
+  OR_FAIL(ST_REJECTED, TERM_MAXIMUM_ITERATIONS);
+done:
+  res->status = st;
+  free(x);
+  free(r);
+  free(z);
+  free(p);
+  free(ap);
+  return st;
+#undef OR_FAIL
+#undef OR_PRECONDITION
+}

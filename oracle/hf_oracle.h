@@ -1,0 +1,121 @@
+/*
+ * CPU oracle for the pressure-projection hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (Hundun-Flow v0.4 solver_* pressure path) used as the checker for
+ * the HIP implementation.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product library never links
+ * or calls it.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * versions/v0.4/ of the reference tree).
+ *
+ * Layout convention (identical to the product C ABI):
+ *   cell arrays   : x + nx*(y + ny*z), no ghosts
+ *   x-face arrays : (nx+1)*ny*nz, index i + (nx+1)*(j + ny*k)
+ *   y-face arrays : nx*(ny+1)*nz, index i + nx*(j + (ny+1)*k)
+ *   z-face arrays : nx*ny*(nz+1), index i + nx*(j + ny*k)
+ */
+#ifndef HF_ORACLE_H
+#define HF_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_DIRICHLET = 0, OR_NEUMANN = 1, OR_PERIODIC = 2 };
+
+typedef struct or_policy {
+  double anisotropy_threshold;  /* 4.0  */
+  int pre_sweeps;               /* 2    */
+  int post_sweeps;              /* 2    */
+  int maximum_levels;           /* 32   */
+  int coarse_sweeps;            /* 24   */
+  int minimum_coarse_extent;    /* 3    */
+  int f_cycle;                  /* 0 = V, 1 = F */
+  int null_space_constant;      /* 0 none, 1 constant */
+  int unit_linear;              /* 1 unit_linear, 0 residual_minimizing */
+} or_policy;
+
+typedef struct or_control {
+  double absolute_tolerance;
+  double relative_tolerance;
+  uint32_t maximum_iterations;
+  uint32_t true_residual_interval;
+  int flexible; /* 0: reference beta = rho'/rho; 1: Polak-Ribiere beta */
+} or_control;
+
+typedef struct or_result {
+  int status;       /* 0 ok, otherwise StatusCode value */
+  int termination;  /* LinearTermination value */
+  uint32_t iterations;
+  double initial_true_residual;
+  double final_true_residual;
+  double rhs_norm;
+  uint64_t operator_applies;
+  uint64_t preconditioner_applies;
+} or_result;
+
+typedef struct or_mg or_mg;
+
+/* Face coefficient rule, src/solver_piso.cpp:1060-1118 with mobility m per
+ * cell, and the diagonal D = sum of the six faces (pure Poisson rows). */
+int or_assemble(const int n[3], const double* faces_x, const double* faces_y,
+                const double* faces_z, const int bc[6], const double* mobility,
+                double* bx, double* by, double* bz, double* diag);
+
+/* Exact IBM pressure operator: regular 7-point action
+ * (src/solver_pressure.cpp:746-811 with src/solver_piso.cpp:1553-1595) then
+ * IBM decoration (src/solver_ibm_pressure.cpp:347-385).  cell_active may be
+ * NULL (no IBM). */
+void or_operator_apply(const int n[3], const int bc[6], const double* diag,
+                       const double* bx, const double* by, const double* bz,
+                       const uint8_t* cell_active, const double* x,
+                       double* y);
+
+/* Masked level-0 operator action used by the multigrid (same matrix as the
+ * IBM operator, src/solver_mg.cpp:1469-1570 + :2405-2411). */
+void or_masked_apply(const or_mg* mg, const double* x, double* y);
+
+or_mg* or_mg_create(const int n[3], const double* widths_x,
+                    const double* widths_y, const double* widths_z,
+                    const int bc[6], const or_policy* policy,
+                    const double* diag, const double* bx, const double* by,
+                    const double* bz, const uint8_t* cell_active,
+                    const uint8_t* fx_active, const uint8_t* fy_active,
+                    const uint8_t* fz_active, int* status);
+void or_mg_destroy(or_mg* mg);
+int or_mg_apply(or_mg* mg, const double* residual, double* correction);
+int or_mg_level_count(const or_mg* mg);
+/* shape[3], coarsen_mask, line_mask */
+int or_mg_level_info(const or_mg* mg, int level, int* shape, int* coarsen_mask,
+                     int* line_mask);
+/* pointers to level storage: 0 D, 1 fx, 2 fy, 3 fz, 4 vol, 5 x, 6 b, 7 r */
+const double* or_mg_level_array(const or_mg* mg, int level, int which);
+/* One smoother call on a level with the level's current x/b (test hook). */
+int or_mg_smooth(or_mg* mg, int level, int sweeps, int reverse,
+                 int retain_defect);
+int or_mg_restrict(or_mg* mg, int fine_level);
+int or_mg_prolongate(or_mg* mg, int fine_level);
+double* or_mg_level_slot(or_mg* mg, int level, int which);
+
+/* solve_pcg restatement, src/solver_krylov.cpp:1299-1555.  Operator is the
+ * exact IBM operator (or_operator_apply); the preconditioner is the MG plan
+ * (mg may be NULL: identity). */
+int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
+                 const double* bx, const double* by, const double* bz,
+                 const uint8_t* cell_active, or_mg* mg, const double* rhs,
+                 double* solution, const or_control* control,
+                 or_result* result);
+
+/* Vector kernels, src/solver_krylov.cpp:176-204 and :650-711. */
+double or_dot(const double* a, const double* b, size_t n);
+double or_norm2(const double* v, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,256 @@
+"""ctypes binding of the CPU oracle (liboracle_hf.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  It is the checker, never part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "liboracle_hf.so")
+
+
+class or_policy(ctypes.Structure):
+    _fields_ = [("anisotropy_threshold", ctypes.c_double),
+                ("pre_sweeps", ctypes.c_int), ("post_sweeps", ctypes.c_int),
+                ("maximum_levels", ctypes.c_int),
+                ("coarse_sweeps", ctypes.c_int),
+                ("minimum_coarse_extent", ctypes.c_int),
+                ("f_cycle", ctypes.c_int),
+                ("null_space_constant", ctypes.c_int),
+                ("unit_linear", ctypes.c_int)]
+
+
+class or_control(ctypes.Structure):
+    _fields_ = [("absolute_tolerance", ctypes.c_double),
+                ("relative_tolerance", ctypes.c_double),
+                ("maximum_iterations", ctypes.c_uint32),
+                ("true_residual_interval", ctypes.c_uint32),
+                ("flexible", ctypes.c_int)]
+
+
+class or_result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("termination", ctypes.c_int),
+                ("iterations", ctypes.c_uint32),
+                ("initial_true_residual", ctypes.c_double),
+                ("final_true_residual", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double),
+                ("operator_applies", ctypes.c_uint64),
+                ("preconditioner_applies", ctypes.c_uint64)]
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise RuntimeError("oracle not built: make -C oracle")
+        L = ctypes.CDLL(LIBRARY_PATH)
+        L.or_assemble.argtypes = [_I, _D, _D, _D, _I, _D, _D, _D, _D, _D]
+        L.or_assemble.restype = ctypes.c_int
+        L.or_operator_apply.argtypes = [_I, _I, _D, _D, _D, _D, _U8, _D, _D]
+        L.or_operator_apply.restype = None
+        L.or_masked_apply.argtypes = [ctypes.c_void_p, _D, _D]
+        L.or_masked_apply.restype = None
+        L.or_mg_create.argtypes = [_I, _D, _D, _D, _I,
+                                   ctypes.POINTER(or_policy), _D, _D, _D, _D,
+                                   _U8, _U8, _U8, _U8, _I]
+        L.or_mg_create.restype = ctypes.c_void_p
+        L.or_mg_destroy.argtypes = [ctypes.c_void_p]
+        L.or_mg_destroy.restype = None
+        L.or_mg_apply.argtypes = [ctypes.c_void_p, _D, _D]
+        L.or_mg_apply.restype = ctypes.c_int
+        L.or_mg_level_count.argtypes = [ctypes.c_void_p]
+        L.or_mg_level_count.restype = ctypes.c_int
+        L.or_mg_level_info.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I, _I]
+        L.or_mg_level_info.restype = ctypes.c_int
+        L.or_mg_level_array.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.or_mg_level_array.restype = _D
+        L.or_mg_level_slot.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.or_mg_level_slot.restype = _D
+        L.or_mg_smooth.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int]
+        L.or_mg_smooth.restype = ctypes.c_int
+        L.or_mg_restrict.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_mg_restrict.restype = ctypes.c_int
+        L.or_mg_prolongate.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_mg_prolongate.restype = ctypes.c_int
+        L.or_pcg_solve.argtypes = [_I, _I, _D, _D, _D, _D, _U8, ctypes.c_void_p,
+                                   _D, _D, ctypes.POINTER(or_control),
+                                   ctypes.POINTER(or_result)]
+        L.or_pcg_solve.restype = ctypes.c_int
+        L.or_dot.argtypes = [_D, _D, ctypes.c_size_t]
+        L.or_dot.restype = ctypes.c_double
+        L.or_norm2.argtypes = [_D, ctypes.c_size_t]
+        L.or_norm2.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _u8(a):
+    return None if a is None else a.ctypes.data_as(_U8)
+
+
+def _i(seq):
+    arr = (ctypes.c_int * len(seq))(*[int(v) for v in seq])
+    return ctypes.cast(arr, _I), arr
+
+
+def face_counts(n):
+    nx, ny, nz = n
+    return (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
+
+
+def assemble(n, faces, bc, mobility):
+    fx, fy, fz = face_counts(n)
+    cells = n[0] * n[1] * n[2]
+    bx, by, bz = np.empty(fx), np.empty(fy), np.empty(fz)
+    diag = np.empty(cells)
+    pn, _k1 = _i(n)
+    pb, _k2 = _i(bc)
+    f = [np.ascontiguousarray(v, dtype=np.float64) for v in faces]
+    m = np.ascontiguousarray(mobility, dtype=np.float64)
+    st = lib().or_assemble(pn, _d(f[0]), _d(f[1]), _d(f[2]), pb, _d(m),
+                           _d(bx), _d(by), _d(bz), _d(diag))
+    if st != 0:
+        raise RuntimeError(f"or_assemble status {st}")
+    return diag, bx, by, bz
+
+
+def operator_apply(n, bc, diag, bx, by, bz, cell_active, x):
+    y = np.empty(n[0] * n[1] * n[2])
+    pn, _k1 = _i(n)
+    pb, _k2 = _i(bc)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().or_operator_apply(pn, pb, _d(diag), _d(bx), _d(by), _d(bz),
+                            _u8(cell_active), _d(x), _d(y))
+    return y
+
+
+def policy(**kw) -> or_policy:
+    p = or_policy(anisotropy_threshold=4.0, pre_sweeps=2, post_sweeps=2,
+                  maximum_levels=32, coarse_sweeps=24,
+                  minimum_coarse_extent=3, f_cycle=0, null_space_constant=1,
+                  unit_linear=1)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class MG:
+    """Oracle multigrid plan (NativeCartesianMgPlan restatement)."""
+
+    def __init__(self, n, faces, bc, pol: or_policy, diag, bx, by, bz,
+                 cell_active=None, x_active=None, y_active=None,
+                 z_active=None):
+        self.n = tuple(n)
+        self.cells = n[0] * n[1] * n[2]
+        w = [np.ascontiguousarray(np.diff(f), dtype=np.float64) for f in faces]
+        self._keep = [w, diag, bx, by, bz, cell_active, x_active, y_active,
+                      z_active]
+        pn, self._k1 = _i(n)
+        pb, self._k2 = _i(bc)
+        st = ctypes.c_int()
+        self.pol = pol
+        self.h = lib().or_mg_create(pn, _d(w[0]), _d(w[1]), _d(w[2]), pb,
+                                    ctypes.byref(pol), _d(diag), _d(bx),
+                                    _d(by), _d(bz), _u8(cell_active),
+                                    _u8(x_active), _u8(y_active),
+                                    _u8(z_active), ctypes.byref(st))
+        if not self.h:
+            raise RuntimeError(f"or_mg_create status {st.value}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mg_destroy(self.h)
+            self.h = None
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty(self.cells)
+        st = lib().or_mg_apply(self.h, _d(r), _d(z))
+        if st != 0:
+            raise RuntimeError(f"or_mg_apply status {st}")
+        return z
+
+    def masked_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.cells)
+        lib().or_masked_apply(self.h, _d(x), _d(y))
+        return y
+
+    def level_count(self):
+        return lib().or_mg_level_count(self.h)
+
+    def level_info(self, level):
+        shape = (ctypes.c_int * 3)()
+        cm = ctypes.c_int()
+        lm = ctypes.c_int()
+        lib().or_mg_level_info(self.h, level, ctypes.cast(shape, _I),
+                               ctypes.byref(cm), ctypes.byref(lm))
+        return {"shape": tuple(shape), "coarsen_mask": cm.value,
+                "line_mask": lm.value}
+
+    def level_array(self, level, which):
+        nx, ny, nz = self.level_info(level)["shape"]
+        counts = [nx * ny * nz, (nx + 1) * ny * nz, nx * (ny + 1) * nz,
+                  nx * ny * (nz + 1), nx * ny * nz, nx * ny * nz,
+                  nx * ny * nz, nx * ny * nz]
+        ptr = lib().or_mg_level_array(self.h, level, which)
+        return np.ctypeslib.as_array(ptr, shape=(counts[which],)).copy()
+
+    def slot(self, level, which):
+        nx, ny, nz = self.level_info(level)["shape"]
+        ptr = lib().or_mg_level_slot(self.h, level, which)
+        return np.ctypeslib.as_array(ptr, shape=(nx * ny * nz,))
+
+
+def pcg(n, bc, diag, bx, by, bz, cell_active, mg, rhs, x0=None, rtol=1e-10,
+        atol=0.0, maximum_iterations=400, true_residual_interval=4,
+        flexible=False):
+    cells = n[0] * n[1] * n[2]
+    x = np.zeros(cells) if x0 is None else np.array(x0, dtype=np.float64)
+    ctl = or_control(atol, rtol, maximum_iterations, true_residual_interval,
+                     1 if flexible else 0)
+    res = or_result()
+    pn, _k1 = _i(n)
+    pb, _k2 = _i(bc)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().or_pcg_solve(pn, pb, _d(diag), _d(bx), _d(by), _d(bz),
+                       _u8(cell_active), None if mg is None else mg.h,
+                       _d(rhs), _d(x), ctypes.byref(ctl), ctypes.byref(res))
+    return x, res
+
+
+def dot(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return lib().or_dot(_d(a), _d(b), a.size)
+
+
+def norm2(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().or_norm2(_d(v), v.size)
+
+
+def from_case(case, pol: or_policy | None = None):
+    """Assemble coefficients and build the oracle MG for a case dict."""
+    diag, bx, by, bz = assemble(case["n"], case["faces"], case["bc"],
+                                case["mobility"])
+    mg = MG(case["n"], case["faces"], case["bc"], pol or policy(), diag, bx,
+            by, bz, case["cell_active"], case["x_active"], case["y_active"],
+            case["z_active"])
+    return (diag, bx, by, bz), mg

@@ -1,0 +1,59 @@
+"""CPU checks of the C ABI: the library is built, loads, and exports every
+function declared in include/hundun_flow.h (no compute without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hundun_flow.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hf_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_abi():
+    import hundun_flow as H
+    assert declared_functions() == sorted(H.ABI_FUNCTIONS)
+
+
+def test_library_exports_every_symbol():
+    import hundun_flow as H
+    lib = H.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header():
+    import hundun_flow as H
+    # sizes follow the C declarations (natural alignment, x86-64)
+    assert ctypes.sizeof(H.hf_status) == 8
+    assert ctypes.sizeof(H.hf_policy) == 24
+    assert ctypes.sizeof(H.hf_solve_control) == 32
+    assert ctypes.sizeof(H.hf_grid_desc) == 64
+    assert ctypes.sizeof(H.hf_level_info) == 16
+
+
+def test_policy_defaults_match_reference():
+    import hundun_flow as H
+    p = H.default_policy()
+    # MgHierarchyPolicy defaults, include/hundun/v04_linear.hpp:115-131
+    assert p.anisotropy_threshold == 4.0
+    assert (p.pre_sweeps, p.post_sweeps, p.maximum_levels) == (2, 2, 32)
+    assert (p.coarse_sweeps, p.minimum_coarse_extent) == (24, 3)
+    assert p.f_cycle == 0
+
+
+def test_invalid_arguments_are_rejected_without_gpu():
+    import hundun_flow as H
+    lib = H.load_library()
+    # null plan pointers fail with invalid_plan before any device work
+    st = lib.hf_operator_apply(None, None, None)
+    assert st.code == 2
+    st = lib.hf_plan_create(None, None, None, None)
+    assert st.code == 2
+    assert lib.hf_status_message(H.hf_status(5, 0)) == b"numerical failure"

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: bit-exact where the arithmetic order is the reference's own (face
+coefficients, hierarchy, masked operator, the FP64 V-cycle without a null
+space); documented relative tolerances where a parallel reduction replaces a
+sequential one (projection means, Krylov dots).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import hf_cases
+import hf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def mixed_case(n=(24, 20, 16), seed=11, periodic_z=True, dirichlet_x=True,
+               radius=0.22):
+    """Dirichlet x walls (no null space), Neumann y, periodic z, sphere
+    mask, lognormal rho."""
+    case = hf_cases.lognormal_sphere(n=n, seed=seed, radius=radius)
+    bc = [0 if dirichlet_x else 1, 0 if dirichlet_x else 1, 1, 1,
+          2 if periodic_z else 1, 2 if periodic_z else 1]
+    act = case["cell_active"].reshape(n[2], n[1], n[0])
+    fx, fy, fz = hf_cases.face_activity(act, bc)
+    case.update(bc=bc, x_active=fx, y_active=fy, z_active=fz)
+    return case
+
+
+CASES = {
+    "sphere32": lambda: hf_cases.lognormal_sphere(n=(32, 32, 32), seed=7),
+    "stretched24": lambda: hf_cases.flame_cylinder(n=(24, 32, 16), seed=8),
+    "step": lambda: hf_cases.flame_step(n=(48, 24, 16), seed=9),
+    "mixed": lambda: mixed_case(),
+    "odd_periodic": lambda: mixed_case(n=(20, 12, 18)),
+}
+
+
+def _policy(case_name, **kw):
+    # stretched grids select line relaxation in the reference planner; a
+    # threshold above the spacing ratio keeps point red-black + full
+    # coarsening on both sides
+    thr = 16.0 if case_name == "stretched24" else 4.0
+    null = 0 if case_name in ("mixed", "odd_periodic") else 1
+    base = dict(anisotropy_threshold=thr, null_space_constant=null)
+    base.update(kw)
+    return base
+
+
+def _gpu_plan(torch, case, **policy):
+    import hundun_flow as H
+    pol = H.default_policy(**policy)
+    plan = H.plan_from_case(case, pol)
+    return plan
+
+
+def _t(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_assemble_bitwise(torch_cuda, name):
+    torch = torch_cuda
+    _say(f"[assemble] case={name}")
+    case = CASES[name]()
+    import hundun_flow as H
+    plan = H.Plan(case["n"], case["faces"], case["bc"],
+                  H.default_policy(**_policy(name)))
+    diag, bx, by, bz = plan.assemble(_t(torch, case["mobility"]))
+    od, obx, oby, obz = O.assemble(case["n"], case["faces"], case["bc"],
+                                   case["mobility"])
+    for g, o in ((diag, od), (bx, obx), (by, oby), (bz, obz)):
+        assert np.array_equal(g.cpu().numpy(), o)
+    plan.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_hierarchy_bitwise(torch_cuda, name):
+    torch = torch_cuda
+    _say(f"[hierarchy] case={name}")
+    case = CASES[name]()
+    pol = _policy(name)
+    plan = _gpu_plan(torch, case, **pol)
+    _, mg = O.from_case(case, O.policy(**pol))
+    assert plan.level_count() == mg.level_count()
+    for level in range(mg.level_count()):
+        gi, oi = plan.level_info(level), mg.level_info(level)
+        assert gi["shape"] == oi["shape"]
+        assert gi["coarsen_mask"] == oi["coarsen_mask"]
+        for which in range(5):
+            g = plan.level_array(level, which).cpu().numpy()
+            o = mg.level_array(level, which)
+            assert np.array_equal(g, o), (level, which)
+    plan.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_operator(torch_cuda, name):
+    torch = torch_cuda
+    _say(f"[operator] case={name}")
+    case = CASES[name]()
+    pol = _policy(name)
+    plan = _gpu_plan(torch, case, **pol)
+    (d, bx, by, bz), mg = O.from_case(case, O.policy(**pol))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(plan.cells)
+    y = plan.operator_apply(_t(torch, x)).cpu().numpy()
+    # bitwise against the masked level-0 matrix of the oracle MG
+    assert np.array_equal(y, mg.masked_apply(x))
+    # and within rounding of the reference IBM-decorated operator
+    ref = O.operator_apply(case["n"], case["bc"], d, bx, by, bz,
+                           case["cell_active"], x)
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    solid = case["cell_active"] == 0
+    assert np.array_equal(y[solid], x[solid])  # identity rows
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["mixed", "odd_periodic"])
+@pytest.mark.parametrize("f_cycle", [0, 1])
+@pytest.mark.parametrize("tail", ["default", "0"])
+def test_mg_apply_bitwise(torch_cuda, monkeypatch, name, f_cycle, tail):
+    """Without a null-space projection every V/F-cycle step is a fixed-order
+    FP64 computation, so the GPU cycle equals the oracle bit for bit, both
+    through the single-workgroup tail and through grid-level kernels."""
+    torch = torch_cuda
+    _say(f"[mg bitwise] case={name} f_cycle={f_cycle} tail={tail}")
+    if tail != "default":
+        monkeypatch.setenv("HF_TAIL_CELLS", tail)
+    case = CASES[name]()
+    pol = _policy(name, f_cycle=f_cycle)
+    plan = _gpu_plan(torch, case, **pol)
+    _, mg = O.from_case(case, O.policy(**pol))
+    r = np.random.default_rng(5).standard_normal(plan.cells)
+    z = plan.mg_apply(_t(torch, r)).cpu().numpy()
+    zo = mg.apply(r)
+    assert np.array_equal(z, zo), np.max(np.abs(z - zo))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["sphere32", "step", "stretched24"])
+@pytest.mark.parametrize("unit_linear", [1, 0])
+@pytest.mark.parametrize("tail", ["default", "0"])
+def test_mg_apply_projected(torch_cuda, monkeypatch, name, unit_linear, tail):
+    """With the constant null space the projection means come from a
+    parallel reduction: agreement to 1e-12 relative."""
+    torch = torch_cuda
+    _say(f"[mg projected] case={name} unit_linear={unit_linear} tail={tail}")
+    if tail != "default":
+        monkeypatch.setenv("HF_TAIL_CELLS", tail)
+    case = CASES[name]()
+    pol = _policy(name, unit_linear=unit_linear)
+    plan = _gpu_plan(torch, case, **pol)
+    _, mg = O.from_case(case, O.policy(**pol))
+    r = case["rhs"]
+    z = plan.mg_apply(_t(torch, r)).cpu().numpy()
+    zo = mg.apply(r)
+    rel = np.linalg.norm(z - zo) / np.linalg.norm(zo)
+    _say(f"  rel={rel:.3e}")
+    assert rel <= 1e-12
+    solid = case["cell_active"] == 0
+    assert np.all(z[solid] == 0.0)
+    plan.close()
+
+
+def _gauge(x, act):
+    fluid = act != 0
+    y = x.copy()
+    y[fluid] -= y[fluid].mean()
+    y[~fluid] = 0.0
+    return y
+
+
+@pytest.mark.parametrize("name", ["sphere32", "step", "stretched24", "mixed"])
+@pytest.mark.parametrize("flexible", [False, True])
+def test_pcg_fp64_matches_oracle(torch_cuda, name, flexible):
+    torch = torch_cuda
+    _say(f"[pcg fp64] case={name} flexible={flexible}")
+    case = CASES[name]()
+    pol = _policy(name)
+    plan = _gpu_plan(torch, case, **pol)
+    (d, bx, by, bz), mg = O.from_case(case, O.policy(**pol))
+    b = case["rhs"]
+    xg, res = plan.solve_pcg(_t(torch, b), rtol=1e-12, flexible=flexible)
+    xo, ores = O.pcg(case["n"], case["bc"], d, bx, by, bz,
+                     case["cell_active"], mg, b, rtol=1e-12,
+                     flexible=flexible)
+    assert res.termination == "converged" and ores.status == 0
+    xg = xg.cpu().numpy()
+    act = case["cell_active"]
+    # independent residual check with the reference-form operator
+    Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
+    true_rel = np.linalg.norm(b - Ax) / np.linalg.norm(b)
+    ga, go = _gauge(xg, act), _gauge(xo, act)
+    diff = np.linalg.norm(ga - go) / np.linalg.norm(go)
+    _say(f"  it gpu={res.iterations} oracle={ores.iterations} "
+         f"true_rel={true_rel:.2e} diff={diff:.2e}")
+    assert true_rel <= 1.01e-12
+    assert abs(res.iterations - ores.iterations) <= 2
+    assert diff <= 1e-9
+    assert np.all(xg[act == 0] == 0.0)
+    plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("name", ["sphere32", "step"])
+def test_mixed_precision_reaches_tolerance(torch_cuda, precision, name):
+    """FP32 / BF16-storage V-cycle inside the FP64 CG: the FP64 true residual
+    reaches 1e-10 and the solution agrees with the FP64 oracle to the
+    conditioning bound."""
+    torch = torch_cuda
+    _say(f"[mixed precision] case={name} precision={precision}")
+    case = CASES[name]()
+    pol = _policy(name, precision=precision)
+    plan = _gpu_plan(torch, case, **pol)
+    (d, bx, by, bz), mg = O.from_case(case, O.policy(**_policy(name)))
+    b = case["rhs"]
+    xg, res = plan.solve_pcg(_t(torch, b), rtol=1e-10, flexible=True)
+    xo, ores = O.pcg(case["n"], case["bc"], d, bx, by, bz,
+                     case["cell_active"], mg, b, rtol=1e-12)
+    xg = xg.cpu().numpy()
+    act = case["cell_active"]
+    Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
+    true_rel = np.linalg.norm(b - Ax) / np.linalg.norm(b)
+    diff = (np.linalg.norm(_gauge(xg, act) - _gauge(xo, act))
+            / np.linalg.norm(_gauge(xo, act)))
+    _say(f"  it={res.iterations} (fp64 oracle {ores.iterations}) "
+         f"true_rel={true_rel:.2e} err_vs_fp64={diff:.2e}")
+    assert res.termination == "converged"
+    assert true_rel <= 1.01e-10
+    assert diff <= 1e-6
+    assert np.all(xg[act == 0] == 0.0)
+    plan.close()
