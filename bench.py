@@ -1,0 +1,262 @@
+"""Pressure-solve benchmark on one MI355X.
+
+Workload (default): the 512x256x256 precision case -- flame-sheet rho (1:7)
+with 5% lognormal noise, a backward-facing step masked, Neumann walls and a
+periodic span; rhs ~ N(0,1) projected to zero fluid mean.  One step = one
+complete pressure solve from x0 = 0 to a relative FP64 true residual of 1e-10,
+with every input already resident in HBM.  Timed: the FP32-storage V(2,2)
+multigrid inside the FP64 CG (the value), next to the plain FP64 solve.
+
+Prints one JSON line.  `value` is ms per solve (lower is better);
+`roofline` is the dominant kernel's algorithmic bytes per launch over its
+average launch time (HIP events on the plan stream, measured in this run);
+`cpu_baseline` times the CPU oracle on a bounded sample of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("hundun-flow_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def storage_bytes(precision: str) -> int:
+    return {"fp64": 8, "fp32": 4, "bf16": 2}[precision]
+
+
+# Algorithmic bytes per cell of one launch, per kernel class (each array
+# streamed once at its storage width s; neighbour reuse assumed perfect).
+def class_bytes_per_cell(cls: str, s: int):
+    return {
+        # half-sweep: read x (all), b and D (active half), 3 face arrays,
+        # write x (active half)
+        "rb_finest": 5.5 * s,
+        # Krylov q = A p + dot: p, D, 3 faces, q (FP64)
+        "operator": 6 * 8,
+        # x += a p, r -= a q: read x p r q, write x r
+        "update_xr": 6 * 8,
+        # r = b - A x: x, b, D, 3 faces, r
+        "true_residual": 7 * 8,
+        # p = z + beta p: read z p, write p
+        "update_p": 3 * 8,
+    }.get(cls)
+
+
+def make_case(name, nx, ny, nz):
+    import hf_cases
+    if name == "flame_step":
+        return hf_cases.flame_step(n=(nx, ny, nz))
+    if name == "flame_cylinder":
+        return hf_cases.flame_cylinder(n=(nx, ny, nz))
+    if name == "lognormal_sphere":
+        return hf_cases.lognormal_sphere(n=(nx, ny, nz))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def gpu_solves(plan, b, steps, warmup, flexible, torch):
+    x = torch.zeros_like(b)
+    result = None
+    for _ in range(warmup):
+        x.zero_()
+        x, result = plan.solve_pcg(b, x, rtol=1e-10, maximum_iterations=1000,
+                                   flexible=flexible)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, result = plan.solve_pcg(b, x, rtol=1e-10, maximum_iterations=1000,
+                                   flexible=flexible)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return x, result, times
+
+
+def cpu_baseline(case, iterations):
+    """Oracle (single-thread C restatement) on the full grid: one V(2,2)
+    application + one operator application + three dot passes, i.e. one PCG
+    iteration, scaled by the FP64 iteration count."""
+    import numpy as np
+
+    import hf_oracle as O
+    t_setup = time.perf_counter()
+    (d, bx, by, bz), mg = O.from_case(case, O.policy())
+    setup_s = time.perf_counter() - t_setup
+    r = case["rhs"]
+    t0 = time.perf_counter()
+    z = mg.apply(r)
+    t_mg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.operator_apply(case["n"], case["bc"], d, bx, by, bz,
+                     case["cell_active"], z)
+    t_op = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(3):
+        O.dot(r, z)
+    t_vec = time.perf_counter() - t0
+    per_iteration = t_mg + t_op + t_vec
+    del mg
+    return {
+        "value": round(per_iteration * iterations * 1e3, 1),
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"1 PCG iteration of the C oracle on the full grid "
+                   f"(V(2,2) {t_mg:.2f}s + operator {t_op:.2f}s + 3 dots "
+                   f"{t_vec:.2f}s; setup {setup_s:.1f}s untimed) x "
+                   f"{iterations} FP64 iterations"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="flame_step")
+    ap.add_argument("--shape", default="512,256,256")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build measures the "
+              "single-GPU share of the pressure solve (multi-GPU sharding is "
+              "a design note, not a measured path).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import hundun_flow as H
+
+    nx, ny, nz = (int(v) for v in args.shape.split(","))
+    log(f"[bench] generating {args.workload} {nx}x{ny}x{nz}")
+    case = make_case(args.workload, nx, ny, nz)
+    cells = nx * ny * nz
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    b = tt(case["rhs"])
+    act = case["cell_active"]
+
+    results = {}
+    plans = {}
+    for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
+        log(f"[bench] plan {prec}")
+        pol = H.default_policy(precision=prec)
+        plan = H.plan_from_case(case, pol, dev)
+        plans[prec] = plan
+        flexible = prec != "fp64"
+        x, res, times = gpu_solves(plan, b, args.steps, args.warmup, flexible,
+                                   torch)
+        # independent FP64 check of the published solution
+        ax = plan.operator_apply(x)
+        true_rel = float(torch.linalg.norm(b - ax) / torch.linalg.norm(b))
+        solid_max = float(x[torch.from_numpy(act == 0).to(dev)].abs().max()) \
+            if (act == 0).any() else 0.0
+        results[prec] = dict(x=x, res=res, times=times, true_rel=true_rel,
+                             solid_max=solid_max)
+        log(f"[bench] {prec}: iterations={res.iterations} "
+            f"ms={['%.2f' % t for t in times]} true_rel={true_rel:.3e} "
+            f"solid_max={solid_max}")
+        assert res.termination == "converged", res
+        assert true_rel <= 1.01e-10, true_rel
+        assert solid_max == 0.0
+
+    main_prec = args.precision
+    main = results[main_prec]
+    ms = sum(main["times"]) / len(main["times"])
+
+    # dominant kernel: one more profiled solve of the timed configuration
+    plan = plans[main_prec]
+    plan.set_profiling(True)
+    x = torch.zeros_like(b)
+    plan.solve_pcg(b, x, rtol=1e-10, maximum_iterations=1000, flexible=True)
+    prof = plan.profile()
+    plan.set_profiling(False)
+    s = storage_bytes(main_prec)
+    ranked = sorted(prof.items(), key=lambda kv: -kv[1][0])
+    log("[bench] kernel classes (ms, launches): " +
+        ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in ranked if v[1]))
+    dominant, (dom_ms, dom_n) = ranked[0]
+    per_cell = class_bytes_per_cell(dominant, s)
+    roofline = None
+    if per_cell is not None and dom_n:
+        avg_ms = dom_ms / dom_n
+        bytes_per_launch = per_cell * cells
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None, "kernel": dominant,
+                    "avg_launch_ms": round(avg_ms, 4),
+                    "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+    extra = {}
+    if "fp64" in results:
+        r64 = results["fp64"]
+        ms64 = sum(r64["times"]) / len(r64["times"])
+        fluid = torch.from_numpy(act != 0).to(dev)
+
+        def gauge(v):
+            v = v.clone()
+            v[fluid] -= v[fluid].mean()
+            v[~fluid] = 0.0
+            return v
+        err = float(torch.linalg.norm(gauge(main["x"]) - gauge(r64["x"]))
+                    / torch.linalg.norm(gauge(r64["x"])))
+        extra = {"fp64_ms_per_solve": round(ms64, 3),
+                 "fp64_iterations": r64["res"].iterations,
+                 "speedup_vs_fp64": round(ms64 / ms, 3),
+                 "error_vs_fp64_solution": err}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        its = results["fp64"]["res"].iterations if "fp64" in results \
+            else main["res"].iterations
+        log("[bench] cpu baseline (oracle, full grid, 1 iteration)")
+        cpu = cpu_baseline(case, its)
+
+    line = {
+        "metric": ("Pressure solve ms to 1e-10 rel residual, FP32-MG vs "
+                   "FP64-MG; % HBM roofline"),
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{main_prec} V-cycle storage / f64 Krylov",
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": f"{args.workload} {nx}x{ny}x{nz}",
+                   "cells": cells, "cycle": "V(2,2) red-black, 24 coarse "
+                   "sweeps, constant null space", "krylov": "PCG, flexible "
+                   "(Polak-Ribiere) beta", "tolerance": 1e-10},
+        "iterations": main["res"].iterations,
+        "mcell_solves_per_s": round(cells / (ms * 1e-3) / 1e6, 2),
+        "true_relative_residual": main["true_rel"],
+        **extra,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -19,6 +19,25 @@ namespace hf {
 namespace {
 
 constexpr int kMaxLevels = 32;
+
+// Kernel classes reported by hf_plan_profile.
+enum ProfileClass : int {
+  P_RB_FINEST = 0,    // red-black colour pass on level 0
+  P_RB_COARSE = 1,    // red-black colour pass on grid levels >= 1
+  P_DEFECT = 2,       // level stencil / defect
+  P_RESTRICT = 3,
+  P_PROLONG = 4,
+  P_TAIL = 5,         // single-workgroup coarse levels
+  P_OPERATOR = 6,     // Krylov A p + dot
+  P_UPDATE_XR = 7,    // x += a p, r -= a q, |r|^2
+  P_TRUE_RESIDUAL = 8,
+  P_MG_ENTRY_EXIT = 9,
+  P_PROJECT = 10,
+  P_UPDATE_P = 11,
+  P_FINALIZE = 12,
+  P_OTHER = 13,
+  kProfileClasses = 14
+};
 constexpr uint32_t kDetailPlan = 9101U;
 constexpr uint32_t kDetailAlloc = 9102U;
 constexpr uint32_t kDetailHip = 9103U;
@@ -135,6 +154,16 @@ Lv<double> op_view(const Level& L) {
 
 using namespace hf;
 
+namespace {
+// Brackets the enclosed launch with a pair of events when profiling is on.
+struct ProfScope {
+  hf_plan* p;
+  explicit ProfScope(hf_plan* plan, int cls);
+  ~ProfScope();
+};
+#define HF_PROF(plan, cls) ProfScope hf_prof_scope_(plan, cls)
+}  // namespace
+
 struct hf_plan {
   hf_grid_desc grid{};
   hf_policy policy{};
@@ -176,10 +205,53 @@ struct hf_plan {
   double* adj_lambda = nullptr;
   uint64_t device_bytes = 0;
   uint64_t reductions = 0;
+  // kernel-class timing with HIP events on the plan stream
+  bool profiling = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> event_class;  // class of the launch starting at 2*i
+  size_t event_pairs = 0;
+  double class_ms[kProfileClasses] = {};
+  uint64_t class_launches[kProfileClasses] = {};
   std::vector<void*> allocations;
 };
 
 namespace {
+
+ProfScope::ProfScope(hf_plan* plan, int cls) : p(plan) {
+  if (!p->profiling) return;
+  const size_t need = 2 * (p->event_pairs + 1);
+  while (p->events.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      p->profiling = false;
+      return;
+    }
+    p->events.push_back(e);
+  }
+  p->event_class.resize(p->event_pairs + 1);
+  p->event_class[p->event_pairs] = cls;
+  (void)hipEventRecord(p->events[2 * p->event_pairs], p->stream);
+}
+
+ProfScope::~ProfScope() {
+  if (!p->profiling) return;
+  (void)hipEventRecord(p->events[2 * p->event_pairs + 1], p->stream);
+  ++p->event_pairs;
+}
+
+// Accumulates the recorded launches (the stream must be idle).
+void profile_collect(hf_plan* p) {
+  if (!p->profiling) return;
+  for (size_t i = 0; i < p->event_pairs; ++i) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p->events[2 * i], p->events[2 * i + 1]) ==
+        hipSuccess) {
+      p->class_ms[p->event_class[i]] += ms;
+      p->class_launches[p->event_class[i]] += 1;
+    }
+  }
+  p->event_pairs = 0;
+}
 
 hf_status hip_check(hipError_t e) {
   return e == hipSuccess ? ok() : fail(HF_INVALID_PLAN, kDetailHip);
@@ -215,6 +287,7 @@ hf_status dalloc_raw(hf_plan* p, void** out, size_t bytes) {
 #define HF_LAUNCH(kernel, grid, block, shared, stream, ...)             \
   hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)),         \
                      dim3(block), shared, stream, __VA_ARGS__)
+
 
 // Level planner, src/solver_mg_workspace.cpp:113-143, :181-199, :317-453.
 void plan_levels(hf_plan* p) {
@@ -339,6 +412,7 @@ void build_exec(hf_plan* p) {
 // ---- reductions -----------------------------------------------------------
 void finalize(hf_plan* p, int64_t blocks, int nacc, int op, int slot,
               double volume = 0.0, int unit_linear = 1) {
+  HF_PROF(p, P_FINALIZE);
   HF_LAUNCH(k_finalize, 1, 1024, 0, p->stream, p->partials, blocks,
             p->partial_stride, nacc, p->scalars, op, slot, volume,
             unit_linear);
@@ -359,9 +433,13 @@ template <class T>
 void project_level(hf_plan* p, int l, T* field) {
   Level& L = p->levels[l];
   const int64_t nb = blocks_for(L.cells);
-  HF_LAUNCH(k_vol_dot<T>, nb, kBlock, 0, p->stream, field, L.vol, L.cells,
-            p->partials);
+  {
+    HF_PROF(p, P_PROJECT);
+    HF_LAUNCH(k_vol_dot<T>, nb, kBlock, 0, p->stream, field, L.vol, L.cells,
+              p->partials);
+  }
   finalize(p, nb, 1, F_MEAN, S_MEAN0 + l, L.volume);
+  HF_PROF(p, P_PROJECT);
   HF_LAUNCH(k_project_apply<T>, nb, kBlock, 0, p->stream, field, L.vol,
             L.cells, p->scalars + S_MEAN0 + l);
 }
@@ -374,20 +452,30 @@ void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain) {
   const int64_t work = hx * L.n[1] * L.n[2];
   const int64_t nbw = blocks_for(work);
   const int64_t nbc = blocks_for(L.cells);
-  if (!L.fused) HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc, 0);
+  const int rb_class = l == 0 ? P_RB_FINEST : P_RB_COARSE;
+  if (!L.fused) {
+    HF_PROF(p, P_DEFECT);
+    HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc, 0);
+  }
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     const int first = reverse ? 1 : 0;
     for (int pass = 0; pass < 2; ++pass) {
       const int color = (first + pass) & 1;
-      if (L.fused)
-        HF_LAUNCH(k_rb<T>, nbw, kBlock, 0, p->stream, v, p->bc, color, work);
-      else
-        HF_LAUNCH(k_rb_separated<T>, nbw, kBlock, 0, p->stream, v, color, work);
+      {
+        HF_PROF(p, rb_class);
+        if (L.fused)
+          HF_LAUNCH(k_rb<T>, nbw, kBlock, 0, p->stream, v, p->bc, color, work);
+        else
+          HF_LAUNCH(k_rb_separated<T>, nbw, kBlock, 0, p->stream, v, color,
+                    work);
+      }
       const bool final_update = sweep + 1 == sweeps && pass == 1;
       if (!final_update || retain) {
-        if (!L.fused || (final_update && retain))
+        if (!L.fused || (final_update && retain)) {
+          HF_PROF(p, P_DEFECT);
           HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc,
                     (retain && final_update) ? 1 : 0);
+        }
       }
     }
   }
@@ -410,11 +498,15 @@ void exec_grid_op(hf_plan* p, const CycleOp& op) {
       using Tf = decltype(tf);
       dispatch(Cn.prec, [&](auto tc) {
         using Tc = decltype(tc);
-        HF_LAUNCH((k_restrict<Tf, Tc>), nb, kBlock, 0, p->stream, view<Tf>(L),
-                  view<Tc>(Cn), L.coarsen_mask,
-                  proj ? p->partials : static_cast<double*>(nullptr));
+        {
+          HF_PROF(p, P_RESTRICT);
+          HF_LAUNCH((k_restrict<Tf, Tc>), nb, kBlock, 0, p->stream,
+                    view<Tf>(L), view<Tc>(Cn), L.coarsen_mask,
+                    proj ? p->partials : static_cast<double*>(nullptr));
+        }
         if (proj) {
           finalize(p, nb, 1, F_MEAN, S_MEAN0 + op.level + 1, Cn.volume);
+          HF_PROF(p, P_PROJECT);
           HF_LAUNCH(k_project_apply<Tc>, nb, kBlock, 0, p->stream,
                     static_cast<Tc*>(Cn.b), Cn.vol, Cn.cells,
                     p->scalars + S_MEAN0 + op.level + 1);
@@ -428,6 +520,7 @@ void exec_grid_op(hf_plan* p, const CycleOp& op) {
     using Tc = decltype(tc);
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
+      HF_PROF(p, P_PROLONG);
       HF_LAUNCH((k_prolong<Tc, Tf>), nb, kBlock, 0, p->stream, view<Tc>(Cn),
                 view<Tf>(L), L.coarsen_mask, p->periodic_mask);
     });
@@ -440,6 +533,7 @@ void run_cycle(hf_plan* p) {
       exec_grid_op(p, e.op);
       continue;
     }
+    HF_PROF(p, P_TAIL);
     HF_LAUNCH(k_tail, 1, 1024, 0, p->stream, p->d_tail_levels, p->tail_first,
               p->d_tail_ops + e.tail_begin, e.tail_count, p->bc,
               p->periodic_mask, static_cast<int>(p->policy.null_space_constant));
@@ -455,8 +549,11 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
     T* x = static_cast<T*>(L0.x);
-    HF_LAUNCH(k_mg_entry<T>, nb, kBlock, 0, p->stream, r, L0.act, b, x,
-              L0.cells);
+    {
+      HF_PROF(p, P_MG_ENTRY_EXIT);
+      HF_LAUNCH(k_mg_entry<T>, nb, kBlock, 0, p->stream, r, L0.act, b, x,
+                L0.cells);
+    }
     if (p->policy.null_space_constant) project_level<T>(p, 0, b);
   });
   run_cycle(p);
@@ -474,6 +571,7 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
       finalize(p, nb, 3, F_MGALPHA, 0, 0.0, 0);
       alpha = p->scalars + S_MGALPHA;
     }
+    HF_PROF(p, P_MG_ENTRY_EXIT);
     HF_LAUNCH(k_mg_exit<T>, nb, kBlock, 0, p->stream, x, L0.act, alpha, z,
               dots ? r : static_cast<const double*>(nullptr), q, L0.cells,
               dots ? p->partials : static_cast<double*>(nullptr),
@@ -685,6 +783,7 @@ hf_status hf_plan_destroy(hf_plan* p) {
   if (p->stream != nullptr) (void)hipStreamSynchronize(p->stream);
   else (void)hipDeviceSynchronize();
   for (void* ptr : p->allocations) (void)hipFree(ptr);
+  for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   if (p->host_scalars != nullptr) (void)hipHostFree(p->host_scalars);
   delete p;
   return ok();
@@ -857,7 +956,11 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     return finish(ok(), HF_TERM_ZERO_RHS);
   }
   HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, x, solution, N);
-  HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x, r, p->partials);
+  {
+    HF_PROF(p, P_TRUE_RESIDUAL);
+    HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x, r,
+              p->partials);
+  }
   finalize(p, nb, 1, F_STORE, S_RNORM2);
   res->operator_applies++;
   HF_TRY(read_scalars(p));
@@ -885,11 +988,18 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown), HF_TERM_BREAKDOWN);
   }
   while (res->iterations < ctl->maximum_iterations) {
-    HF_LAUNCH(k_op_dot, nb, kBlock, 0, p->stream, A, p->bc, pp, q, p->partials);
+    {
+      HF_PROF(p, P_OPERATOR);
+      HF_LAUNCH(k_op_dot, nb, kBlock, 0, p->stream, A, p->bc, pp, q,
+                p->partials);
+    }
     res->operator_applies++;
     finalize(p, nb, 1, F_ALPHA, 0);
-    HF_LAUNCH(k_update_xr, nb, kBlock, 0, p->stream, x, pp, r, q,
-              p->scalars + S_ALPHA, N, p->partials);
+    {
+      HF_PROF(p, P_UPDATE_XR);
+      HF_LAUNCH(k_update_xr, nb, kBlock, 0, p->stream, x, pp, r, q,
+                p->scalars + S_ALPHA, N, p->partials);
+    }
     finalize(p, nb, 1, F_STORE, S_RNORM2);
     res->iterations++;
     HF_TRY(read_scalars(p));
@@ -906,8 +1016,11 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                         res->iterations % ctl->true_residual_interval == 0 ||
                         res->iterations == ctl->maximum_iterations;
     if (verify) {
-      HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x, r,
-                p->partials);
+      {
+        HF_PROF(p, P_TRUE_RESIDUAL);
+        HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x,
+                  r, p->partials);
+      }
       finalize(p, nb, 1, F_STORE, S_RNORM2);
       res->operator_applies++;
       HF_TRY(read_scalars(p));
@@ -927,7 +1040,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true));
     res->preconditioner_applies++;
     finalize(p, nb, ctl->flexible ? 2 : 1, ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
-    HF_LAUNCH(k_update_p, nb, kBlock, 0, p->stream, pp, z, p->scalars + S_BETA, N);
+    HF_PROF(p, P_UPDATE_P);
+    HF_LAUNCH(k_update_p, nb, kBlock, 0, p->stream, pp, z, p->scalars + S_BETA,
+              N);
   }
   return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
 }
@@ -954,6 +1069,7 @@ hf_status hf_pcg_solve(hf_plan* p, const double* rhs, double* solution,
   const hf_status st = pcg_impl(p, rhs, solution, ctl, res);
   (void)hipEventRecord(e1, p->stream);
   (void)hipEventSynchronize(e1);
+  profile_collect(p);
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   res->solve_milliseconds = ms;
@@ -1062,18 +1178,26 @@ hf_status hf_plan_device_bytes(const hf_plan* p, uint64_t* bytes) {
 }
 
 hf_status hf_plan_set_profiling(hf_plan* p, int32_t enabled) {
-  (void)enabled;
   if (p == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  HF_TRY(hip_check(hipStreamSynchronize(p->stream)));
+  profile_collect(p);
+  p->profiling = enabled != 0;
+  p->event_pairs = 0;
+  for (int c = 0; c < kProfileClasses; ++c) {
+    p->class_ms[c] = 0.0;
+    p->class_launches[c] = 0;
+  }
   return ok();
 }
 
 hf_status hf_plan_profile(const hf_plan* p, double* ms, uint64_t* launches,
                           int32_t capacity, int32_t* count) {
-  (void)ms;
-  (void)launches;
-  (void)capacity;
   if (p == nullptr || count == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
-  *count = 0;
+  *count = kProfileClasses;
+  for (int c = 0; c < kProfileClasses && c < capacity; ++c) {
+    if (ms != nullptr) ms[c] = p->class_ms[c];
+    if (launches != nullptr) launches[c] = p->class_launches[c];
+  }
   return ok();
 }
 

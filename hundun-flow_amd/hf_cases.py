@@ -32,8 +32,14 @@ def _rng(seed: int) -> np.random.Generator:
     return np.random.Generator(np.random.Philox(seed))
 
 
+def real4(faces: np.ndarray) -> np.ndarray:
+    """Face coordinates as REAL4 values: the reference's runtime-axes
+    geometry stores face coordinates in single precision."""
+    return faces.astype(np.float32).astype(np.float64)
+
+
 def uniform_faces(n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
-    return lo + (hi - lo) * np.arange(n + 1, dtype=np.float64) / n
+    return real4(lo + (hi - lo) * np.arange(n + 1, dtype=np.float64) / n)
 
 
 def tanh_faces(n: int, ratio: float) -> np.ndarray:
@@ -45,7 +51,7 @@ def tanh_faces(n: int, ratio: float) -> np.ndarray:
     f = 0.5 + 0.5 * np.tanh(s * (2.0 * eta - 1.0)) / math.tanh(s)
     f[0] = 0.0
     f[-1] = 1.0
-    return f
+    return real4(f)
 
 
 def centres(faces: np.ndarray) -> np.ndarray:

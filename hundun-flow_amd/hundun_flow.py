@@ -27,6 +27,11 @@ TERMINATIONS = ["converged", "zero_rhs", "maximum_iterations", "breakdown",
                 "non_finite", "operator_failure", "preconditioner_failure",
                 "collective_failure", "invalid_plan"]
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+# kernel classes of hf_plan_profile, in order
+PROFILE_CLASSES = ["rb_finest", "rb_coarse", "defect", "restrict", "prolong",
+                   "tail", "operator", "update_xr", "true_residual",
+                   "mg_entry_exit", "project", "update_p", "finalize",
+                   "other"]
 
 ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",
@@ -353,6 +358,21 @@ class Plan:
                self.lib.hf_plan_level_array(self.handle, level, which,
                                             _ptr(out)))
         return out
+
+    def set_profiling(self, enabled: bool):
+        _check("hf_plan_set_profiling",
+               self.lib.hf_plan_set_profiling(self.handle, 1 if enabled else 0))
+
+    def profile(self) -> dict:
+        cap = len(PROFILE_CLASSES)
+        ms = (ctypes.c_double * cap)()
+        launches = (ctypes.c_uint64 * cap)()
+        count = ctypes.c_int32()
+        _check("hf_plan_profile",
+               self.lib.hf_plan_profile(self.handle, ms, launches, cap,
+                                        ctypes.byref(count)))
+        return {PROFILE_CLASSES[i]: (ms[i], launches[i])
+                for i in range(min(cap, count.value))}
 
     def device_bytes(self) -> int:
         v = ctypes.c_uint64()
