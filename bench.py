@@ -53,6 +53,29 @@ def class_bytes_per_cell(cls: str, s: int):
     }.get(cls)
 
 
+# kernel class -> symbol of its kernel for the rocprof traffic lookup
+CLASS_KERNELS = {
+    "rb_finest": "void hf::k_rb<{t}>(hf::Lv<{t}>, hf::Bc, int, long)",
+    "operator": "hf::k_op_dot(hf::Lv<double>, hf::Bc, double const*, double*, double*)",
+}
+
+
+def measured_traffic(workload_tag, cls, precision, grid):
+    """HBM bytes per launch of the class's kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload_tag}_hbm_traffic.json")
+    if not os.path.exists(path) or cls not in CLASS_KERNELS:
+        return None
+    t = {"fp32": "float", "bf16": "hf::bf16", "fp64": "double"}[precision]
+    name = CLASS_KERNELS[cls].format(t=t)
+    with open(path) as f:
+        data = json.load(f)
+    entry = data["kernels"].get(f"{name}|{grid}")
+    if entry is None:
+        return None
+    return int(entry["fetch_bytes"] + entry["write_bytes"])
+
+
 def make_case(name, nx, ny, nz):
     import hf_cases
     if name == "flame_step":
@@ -130,6 +153,10 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cycle", default="v", choices=["v", "f"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of this many MiB (x10) as "
+                         "the known-size reference for HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the "
@@ -141,6 +168,16 @@ def main():
     import torch
 
     import hundun_flow as H
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 8 * n, "launches": 10}))
+        return
 
     nx, ny, nz = (int(v) for v in args.shape.split(","))
     log(f"[bench] generating {args.workload} {nx}x{ny}x{nz}")
@@ -156,7 +193,8 @@ def main():
     plans = {}
     for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
         log(f"[bench] plan {prec}")
-        pol = H.default_policy(precision=prec)
+        pol = H.default_policy(precision=prec,
+                               f_cycle=1 if args.cycle == "f" else 0)
         plan = H.plan_from_case(case, pol, dev)
         plans[prec] = plan
         flexible = prec != "fp64"
@@ -198,10 +236,20 @@ def main():
         avg_ms = dom_ms / dom_n
         bytes_per_launch = per_cell * cells
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        if dominant == "rb_finest":
+            work = ((nx + 1) // 2) * ny * nz
+            grid = -(-work // 256) * 256
+        else:
+            grid = -(-cells // 256) * 256
+        tag = f"{args.workload}_{nx}x{ny}x{nz}_{main_prec}"
+        traffic = measured_traffic(tag, dominant, main_prec, grid)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "kernel": dominant,
+                    "traffic": traffic,
+                    "traffic_source": (f"profiles/{tag}_hbm_traffic.json"
+                                       if traffic else None),
+                    "kernel": dominant,
                     "avg_launch_ms": round(avg_ms, 4),
                     "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
@@ -245,8 +293,8 @@ def main():
         "dtype": f"{main_prec} V-cycle storage / f64 Krylov",
         "data": "synthetic (seeded numpy generator)",
         "config": {"workload": f"{args.workload} {nx}x{ny}x{nz}",
-                   "cells": cells, "cycle": "V(2,2) red-black, 24 coarse "
-                   "sweeps, constant null space", "krylov": "PCG, flexible "
+                   "cells": cells, "cycle": f"{args.cycle.upper()}(2,2) "
+                   "red-black, 24 coarse sweeps, constant null space", "krylov": "PCG, flexible "
                    "(Polak-Ribiere) beta", "tolerance": 1e-10},
         "iterations": main["res"].iterations,
         "mcell_solves_per_s": round(cells / (ms * 1e-3) / 1e6, 2),
