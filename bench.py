@@ -154,6 +154,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cycle", default="v", choices=["v", "f"])
+    ap.add_argument("--no-reference-hierarchy", dest="reference_hierarchy",
+                    action="store_false",
+                    help="skip the solves with the reference's face "
+                         "aggregation")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (x10) as "
                          "the known-size reference for HBM counters")
@@ -191,12 +195,18 @@ def main():
 
     results = {}
     plans = {}
-    for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
-        log(f"[bench] plan {prec}")
-        pol = H.default_policy(precision=prec,
+    runs = [(args.precision, 1)]
+    if not args.no_fp64:
+        runs.append(("fp64", 1))
+    if args.reference_hierarchy:
+        runs += [(args.precision, 0)] + ([] if args.no_fp64 else [("fp64", 0)])
+    for prec, scaling in runs:
+        key = (prec, scaling)
+        log(f"[bench] plan {prec} coarse_face_scaling={scaling}")
+        pol = H.default_policy(precision=prec, coarse_face_scaling=scaling,
                                f_cycle=1 if args.cycle == "f" else 0)
         plan = H.plan_from_case(case, pol, dev)
-        plans[prec] = plan
+        plans[key] = plan
         flexible = prec != "fp64"
         x, res, times = gpu_solves(plan, b, args.steps, args.warmup, flexible,
                                    torch)
@@ -205,21 +215,24 @@ def main():
         true_rel = float(torch.linalg.norm(b - ax) / torch.linalg.norm(b))
         solid_max = float(x[torch.from_numpy(act == 0).to(dev)].abs().max()) \
             if (act == 0).any() else 0.0
-        results[prec] = dict(x=x, res=res, times=times, true_rel=true_rel,
-                             solid_max=solid_max)
-        log(f"[bench] {prec}: iterations={res.iterations} "
+        results[key] = dict(x=x, res=res, times=times, true_rel=true_rel,
+                            solid_max=solid_max)
+        log(f"[bench] {prec} scaling={scaling}: iterations={res.iterations} "
             f"ms={['%.2f' % t for t in times]} true_rel={true_rel:.3e} "
             f"solid_max={solid_max}")
         assert res.termination == "converged", res
         assert true_rel <= 1.01e-10, true_rel
         assert solid_max == 0.0
+        if scaling == 0 and key != (args.precision, 1):
+            plan.close()
+            plans.pop(key)
 
     main_prec = args.precision
-    main = results[main_prec]
+    main = results[(main_prec, 1)]
     ms = sum(main["times"]) / len(main["times"])
 
     # dominant kernel: one more profiled solve of the timed configuration
-    plan = plans[main_prec]
+    plan = plans[(main_prec, 1)]
     plan.set_profiling(True)
     x = torch.zeros_like(b)
     plan.solve_pcg(b, x, rtol=1e-10, maximum_iterations=1000, flexible=True)
@@ -253,10 +266,13 @@ def main():
                     "avg_launch_ms": round(avg_ms, 4),
                     "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
+    def mean_ms(r):
+        return sum(r["times"]) / len(r["times"])
+
     extra = {}
-    if "fp64" in results:
-        r64 = results["fp64"]
-        ms64 = sum(r64["times"]) / len(r64["times"])
+    if ("fp64", 1) in results:
+        r64 = results[("fp64", 1)]
+        ms64 = mean_ms(r64)
         fluid = torch.from_numpy(act != 0).to(dev)
 
         def gauge(v):
@@ -271,10 +287,28 @@ def main():
                  "speedup_vs_fp64": round(ms64 / ms, 3),
                  "error_vs_fp64_solution": err}
 
+    reference = None
+    if (main_prec, 0) in results:
+        r32 = results[(main_prec, 0)]
+        reference = {"hierarchy": "reference face aggregation",
+                     "ms_per_solve": round(mean_ms(r32), 3),
+                     "iterations": r32["res"].iterations}
+        if ("fp64", 0) in results:
+            r64 = results[("fp64", 0)]
+            reference.update({"fp64_ms_per_solve": round(mean_ms(r64), 3),
+                              "fp64_iterations": r64["res"].iterations,
+                              "speedup_vs_fp64": round(mean_ms(r64) /
+                                                       mean_ms(r32), 3)})
+
     cpu = None
     if not args.no_cpu_baseline:
-        its = results["fp64"]["res"].iterations if "fp64" in results \
-            else main["res"].iterations
+        # the reference CPU algorithm: reference aggregation, FP64 PCG
+        if ("fp64", 0) in results:
+            its = results[("fp64", 0)]["res"].iterations
+        elif ("fp64", 1) in results:
+            its = results[("fp64", 1)]["res"].iterations
+        else:
+            its = main["res"].iterations
         log("[bench] cpu baseline (oracle, full grid, 1 iteration)")
         cpu = cpu_baseline(case, its)
 
@@ -294,12 +328,15 @@ def main():
         "data": "synthetic (seeded numpy generator)",
         "config": {"workload": f"{args.workload} {nx}x{ny}x{nz}",
                    "cells": cells, "cycle": f"{args.cycle.upper()}(2,2) "
-                   "red-black, 24 coarse sweeps, constant null space", "krylov": "PCG, flexible "
+                   "red-black, 24 coarse sweeps, constant null space",
+                   "hierarchy": "distance-corrected face aggregation "
+                   "(reference aggregation timed in reference_hierarchy)", "krylov": "PCG, flexible "
                    "(Polak-Ribiere) beta", "tolerance": 1e-10},
         "iterations": main["res"].iterations,
         "mcell_solves_per_s": round(cells / (ms * 1e-3) / 1e6, 2),
         "true_relative_residual": main["true_rel"],
         **extra,
+        "reference_hierarchy": reference,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,694 @@
+// Multigrid cycle and Krylov kernels (gfx950).
+//
+// Grid-level kernels map one 256-thread block to a 2-D tile of one z-plane
+// (Tiles in hf_device.hpp) and vector kernels one block to 256 consecutive
+// elements, so every launch streams HBM in dispatch order.  Reductions write
+// one partial per block; the host-side finaliser folds them in fixed order.
+// Block numbers are remapped so that every XCD owns a contiguous z-slab and
+// neighbour planes are served by its own L2.
+#pragma once
+
+#include "hf_device.hpp"
+
+namespace hf {
+
+struct Widths {
+  const double* x;
+  const double* y;
+  const double* z;
+};
+
+__device__ __forceinline__ int cell_index(int nx, int ny, int i, int j, int k) {
+  return i + nx * (j + ny * k);
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels: each block covers one contiguous chunk of elements.
+// ---------------------------------------------------------------------------
+struct Chunk {
+  int n;       // elements
+  int chunk;   // elements per block (multiple of kBlock)
+};
+
+__device__ __forceinline__ void chunk_range(const Chunk& ch, int& lb, int& lo,
+                                            int& hi) {
+  lb = xcd_block32(blockIdx.x, gridDim.x);
+  lo = lb * ch.chunk;
+  hi = min(ch.n, lo + ch.chunk);
+}
+
+__global__ void k_sum(Chunk ch, const double* field, const uint8_t* act,
+                      double* partials) {
+  __shared__ double scratch[16];
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  double v = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock)
+    if (act == nullptr || act[c]) v += field[c];
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) partials[lb] = v;
+}
+
+__global__ void k_count(Chunk ch, const uint8_t* act, double* partials) {
+  __shared__ double scratch[16];
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  double v = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock)
+    v += act == nullptr ? 1.0 : static_cast<double>(act[c] != 0);
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) partials[lb] = v;
+}
+
+__global__ void k_sumsq(Chunk ch, const double* v, double* partials) {
+  __shared__ double scratch[16];
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  double d = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) d += v[c] * v[c];
+  d = block_sum(d, scratch);
+  if (threadIdx.x == 0) partials[lb] = d;
+}
+
+__global__ void k_dot(Chunk ch, const double* a, const double* b,
+                      double* partials) {
+  __shared__ double scratch[16];
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  double d = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) d += a[c] * b[c];
+  d = block_sum(d, scratch);
+  if (threadIdx.x == 0) partials[lb] = d;
+}
+
+__global__ void k_subtract_mean(Chunk ch, double* field, const double* mean,
+                                const uint8_t* act) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  const double m = *mean;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock)
+    field[c] = (act == nullptr || act[c]) ? field[c] - m : 0.0;
+}
+
+// x += alpha p; r += (-alpha) q; partial sum of r.r
+// (add_scaled, src/solver_krylov.cpp:383-399, twice).
+__global__ void k_update_xr(Chunk ch, double* x, const double* p, double* r,
+                            const double* q, const double* alpha,
+                            double* partials) {
+  __shared__ double scratch[16];
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  const double a = *alpha;
+  double d = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
+    x[c] = x[c] + a * p[c];
+    const double rc = r[c] + -a * q[c];
+    r[c] = rc;
+    d += rc * rc;
+  }
+  d = block_sum(d, scratch);
+  if (threadIdx.x == 0) partials[lb] = d;
+}
+
+// p = 1.0*z + beta*p (axpby, src/solver_krylov.cpp:321-339)
+__global__ void k_update_p(Chunk ch, double* p, const double* z,
+                           const double* beta) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  const double be = *beta;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock)
+    p[c] = 1.0 * z[c] + be * p[c];
+}
+
+__global__ void k_copy(Chunk ch, double* dst, const double* src) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) dst[c] = src[c];
+}
+
+// ---------------------------------------------------------------------------
+// Projection helpers (constant null space, src/solver_mg.cpp:1412-1454).
+// Level 0 recomputes its fluid volumes from the axis widths and the activity
+// byte, (w_x*w_y)*w_z exactly as the hierarchy stored them.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_vol_dot_finest(Tiles t, const T* field, const uint8_t* act,
+                                 Widths w, double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  double v = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    if (act[c]) v += (w.x[i] * w.y[j] * w.z[k]) * static_cast<double>(ld(field[c]));
+  }
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) partials[lb] = v;
+}
+
+template <class T>
+__global__ void k_vol_dot(Tiles t, const T* field, const double* vol,
+                          double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  double v = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    v += vol[c] * static_cast<double>(ld(field[c]));
+  }
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) partials[lb] = v;
+}
+
+// f := fluid ? f - mean : 0
+template <class T>
+__global__ void k_project_apply(Tiles t, T* field, const uint8_t* act,
+                                const double* mean) {
+  int i, j, k;
+  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), i, j, k);
+  if (i >= t.nx || j >= t.ny) return;
+  const int c = cell_index(t.nx, t.ny, i, j, k);
+  field[c] = act[c] ? st<T>(static_cast<double>(ld(field[c])) - *mean)
+                    : st<T>(0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid grid-level kernels.
+// ---------------------------------------------------------------------------
+
+// apply_impl entry (src/solver_mg.cpp:4867-4879): b := r on fluid cells and,
+// unless the first smoothing pass starts from zero itself, x := 0.  With a
+// constant null space the block partials of vol*b are produced here.
+template <class T>
+__global__ void k_mg_entry(Tiles t, const double* r, const uint8_t* act,
+                           Widths w, T* b, T* x, double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  double v = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    const bool a = act[c] != 0;
+    const T bv = a ? st<T>(r[c]) : st<T>(0.0);
+    b[c] = bv;
+    if (x != nullptr) x[c] = st<T>(0.0);
+    if (partials != nullptr && a)
+      v += (w.x[i] * w.y[j] * w.z[k]) * static_cast<double>(ld(bv));
+  }
+  if (partials != nullptr) {
+    v = block_sum(v, scratch);
+    if (threadIdx.x == 0) partials[lb] = v;
+  }
+}
+
+// First colour pass of a smoothing that starts from x = 0: A x = 0, so
+// colour-0 cells get x = 0 + 0.72 b / D and colour-1 cells x = 0, reading no
+// neighbours and no faces.  A pending constant projection of b (mean != null)
+// is applied and stored here: b := fluid ? b - mean : 0.
+template <class T>
+__global__ void k_rb_first(Tiles t, Lv<T> L, const double* mean) {
+  using C = typename Compute<T>::type;
+  int i, j, k;
+  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), i, j, k);
+  if (i >= t.nx || j >= t.ny) return;
+  const int c = cell_index(t.nx, t.ny, i, j, k);
+  C bv;
+  if (mean != nullptr) {
+    const T stored = L.act[c] ? st<T>(static_cast<double>(ld(L.b[c])) - *mean)
+                              : st<T>(0.0);
+    L.b[c] = stored;
+    bv = C(ld(stored));
+  } else {
+    bv = C(ld(L.b[c]));
+  }
+  if (((i + j + k) & 1) == 0) {
+    const C omega = C(0.72);
+    const C zero = C(0);
+    L.x[c] = st<T>(zero + omega * (bv - zero) / C(ld(L.D[c])));
+  } else {
+    L.x[c] = st<T>(0.0);
+  }
+}
+
+// One colour pass of the fused red-black update (src/solver_mg.cpp:
+// 3149-3222): x += 0.72 (b - A x)/D on cells with (i+j+k)&1 == color.  One
+// thread per cell of the colour; same-colour cells never neighbour, so the
+// in-place update is order independent.
+template <class T>
+__global__ void k_rb(Tiles t, Lv<T> L, Bc bc, int color) {
+  using C = typename Compute<T>::type;
+  int ic, j, k;
+  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), ic, j, k);
+  if (ic >= t.nx || j >= t.ny) return;
+  const int i = 2 * ic + ((color + j + k) & 1);
+  if (i >= L.nx) return;
+  const int c = cell_index(L.nx, L.ny, i, j, k);
+  const C xc = ld(L.x[c]);
+  const C v = stencil<C>(L, bc, L.x, i, j, k, c, xc);
+  const C omega = C(0.72);
+  L.x[c] = st<T>(xc + omega * (C(ld(L.b[c])) - v) / C(ld(L.D[c])));
+}
+
+// Separated route for levels with an odd periodic extent
+// (src/solver_mg.cpp:3262-3307): the colour update uses the A x computed
+// before the pass.
+template <class T>
+__global__ void k_rb_separated(Tiles t, Lv<T> L, int color) {
+  using C = typename Compute<T>::type;
+  int ic, j, k;
+  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), ic, j, k);
+  if (ic >= t.nx || j >= t.ny) return;
+  const int i = 2 * ic + ((color + j + k) & 1);
+  if (i >= L.nx) return;
+  const int c = cell_index(L.nx, L.ny, i, j, k);
+  const C omega = C(0.72);
+  L.x[c] = st<T>(C(ld(L.x[c])) +
+                 omega * (C(ld(L.b[c])) - C(ld(L.r[c]))) / C(ld(L.D[c])));
+}
+
+// r := A x (defect = 0) or r := b - A x (defect = 1).
+template <class T>
+__global__ void k_level_apply(Tiles t, Lv<T> L, Bc bc, int defect) {
+  using C = typename Compute<T>::type;
+  int i, j, k;
+  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), i, j, k);
+  if (i >= t.nx || j >= t.ny) return;
+  const int c = cell_index(t.nx, t.ny, i, j, k);
+  const C v = stencil<C>(L, bc, L.x, i, j, k, c, C(ld(L.x[c])));
+  L.r[c] = defect ? st<T>(C(ld(L.b[c])) - v) : st<T>(v);
+}
+
+// Restriction (src/solver_mg.cpp:3633-3720): coarse b = sum of children in
+// dz, dy, dx order, coarse x := 0 (skipped when the next smoothing starts
+// from zero by itself).  With a constant null space the block partials of
+// vol*b are produced (chunked launch).
+template <class Tf, class Tc>
+__global__ void k_restrict(Tiles t, Lv<Tf> F, Lv<Tc> Cl, int mask,
+                           int zero_x, double* partials) {
+  using C = typename Compute<Tc>::type;
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  const int mx = mask & 1, my = (mask >> 1) & 1, mz = (mask >> 2) & 1;
+  double pv = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int ci, cj, ck;
+    tile_xyz(t, tile, ci, cj, ck);
+    if (ci >= t.nx || cj >= t.ny) continue;
+    const int xb = mx ? 2 * ci : ci, yb = my ? 2 * cj : cj,
+              zb = mz ? 2 * ck : ck;
+    C sum = C(0);
+    for (int dz = 0; dz < (mz ? 2 : 1); ++dz)
+      for (int dy = 0; dy < (my ? 2 : 1); ++dy)
+        for (int dx = 0; dx < (mx ? 2 : 1); ++dx) {
+          const int fx = xb + dx, fy = yb + dy, fz = zb + dz;
+          if (fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
+          sum += C(ld(F.r[cell_index(F.nx, F.ny, fx, fy, fz)]));
+        }
+    const int c = cell_index(Cl.nx, Cl.ny, ci, cj, ck);
+    const Tc stored = st<Tc>(sum);
+    Cl.b[c] = stored;
+    if (zero_x) Cl.x[c] = st<Tc>(0.0);
+    if (partials != nullptr) pv += Cl.vol[c] * static_cast<double>(ld(stored));
+  }
+  if (partials != nullptr) {
+    pv = block_sum(pv, scratch);
+    if (threadIdx.x == 0) partials[lb] = pv;
+  }
+}
+
+__device__ __forceinline__ void interp(int fine, int coarsened, int cext,
+                                       int periodic, int* l, int* r,
+                                       int* rw4) {
+  // rw4: right weight in quarters (0, 1 or 3)
+  if (!coarsened) {
+    *l = fine;
+    *r = fine;
+    *rw4 = 0;
+    return;
+  }
+  const int parent = fine / 2;
+  if ((fine & 1) == 0) {
+    *l = parent - 1;
+    *r = parent;
+    *rw4 = 3;
+  } else {
+    *l = parent;
+    *r = parent + 1;
+    *rw4 = 1;
+  }
+  if (!periodic && *l < 0) {
+    *l = 0;
+    *r = 0;
+    *rw4 = 0;
+  } else if (!periodic && *r >= cext) {
+    *l = cext - 1;
+    *r = cext - 1;
+    *rw4 = 0;
+  }
+}
+
+__device__ __forceinline__ int wrapi(int v, int extent) {
+  return v < 0 ? v + extent : (v >= extent ? v - extent : v);
+}
+
+// Prolongation (src/solver_mg.cpp:3893-3993): x_f += sum of ((wx*wy)*wz)*x_c
+// over the (az, ay, ax) corners; fine solids := 0.
+template <class Tc, class Tf>
+__device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
+                                             int mask, int per, int i, int j,
+                                             int k, int c) {
+  using C = typename Compute<Tf>::type;
+  if (!F.act[c]) {
+    F.x[c] = st<Tf>(0.0);
+    return;
+  }
+  int xl, xr, yl, yr, zl, zr, xw, yw, zw;
+  interp(i, mask & 1, Cl.nx, per & 1, &xl, &xr, &xw);
+  interp(j, (mask >> 1) & 1, Cl.ny, (per >> 1) & 1, &yl, &yr, &yw);
+  interp(k, (mask >> 2) & 1, Cl.nz, (per >> 2) & 1, &zl, &zr, &zw);
+  C value = C(0);
+  for (int az = 0; az < (zl == zr ? 1 : 2); ++az) {
+    const int gz = wrapi(az == 0 ? zl : zr, Cl.nz);
+    const C wz = az == 0 ? C(1) - C(zw) * C(0.25) : C(zw) * C(0.25);
+    for (int ay = 0; ay < (yl == yr ? 1 : 2); ++ay) {
+      const int gy = wrapi(ay == 0 ? yl : yr, Cl.ny);
+      const C wy = ay == 0 ? C(1) - C(yw) * C(0.25) : C(yw) * C(0.25);
+      for (int ax = 0; ax < (xl == xr ? 1 : 2); ++ax) {
+        const int gx = wrapi(ax == 0 ? xl : xr, Cl.nx);
+        const C wx = ax == 0 ? C(1) - C(xw) * C(0.25) : C(xw) * C(0.25);
+        value += wx * wy * wz * C(ld(Cl.x[cell_index(Cl.nx, Cl.ny, gx, gy, gz)]));
+      }
+    }
+  }
+  F.x[c] = st<Tf>(C(ld(F.x[c])) + value);
+}
+
+template <class Tc, class Tf>
+__global__ void k_prolong(Tiles t, Lv<Tc> Cl, Lv<Tf> F, int mask, int per) {
+  int i, j, k;
+  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), i, j, k);
+  if (i >= t.nx || j >= t.ny) return;
+  prolong_cell(Cl, F, mask, per, i, j, k, cell_index(t.nx, t.ny, i, j, k));
+}
+
+// apply_impl exit (src/solver_mg.cpp:4954-4976, :5082-5118): the candidate's
+// constant projection (mean != null), scale alpha (null = unit) and
+// correction = fluid ? (x - mean) * alpha : 0, fused with the Krylov inner
+// products r.z (and z.q for the flexible beta).
+template <class T>
+__global__ void k_mg_exit(Tiles t, const T* x, const uint8_t* act,
+                          const double* mean, const double* alpha, double* z,
+                          const double* r, const double* q, double* partials,
+                          int stride) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  const double a = alpha == nullptr ? 1.0 : *alpha;
+  const double m = mean == nullptr ? 0.0 : *mean;
+  double rz = 0.0, zq = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    double v = 0.0;
+    if (act[c]) {
+      double xv = static_cast<double>(ld(x[c]));
+      if (mean != nullptr) xv = static_cast<double>(ld(st<T>(xv - m)));
+      v = xv * a;
+    }
+    z[c] = v;
+    if (r != nullptr) rz += r[c] * v;
+    if (q != nullptr) zq += v * q[c];
+  }
+  if (partials != nullptr) {
+    rz = block_sum(rz, scratch);
+    if (threadIdx.x == 0) partials[lb] = rz;
+    if (q != nullptr) {
+      zq = block_sum(zq, scratch);
+      if (threadIdx.x == 0) partials[stride + lb] = zq;
+    }
+  }
+}
+
+// Three dots of the residual-minimising scaling (src/solver_mg.cpp:
+// 4988-5001): r = masked input, az = r - (b - A x).
+template <class T>
+__global__ void k_mg_proj3(Tiles t, const double* rin, const uint8_t* act,
+                           const T* rec, double* partials, int stride) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    const double r = act[c] ? rin[c] : 0.0;
+    const double az = r - static_cast<double>(ld(rec[c]));
+    p0 += r * r;
+    p1 += r * az;
+    p2 += az * az;
+  }
+  p0 = block_sum(p0, scratch);
+  p1 = block_sum(p1, scratch);
+  p2 = block_sum(p2, scratch);
+  if (threadIdx.x == 0) {
+    partials[lb] = p0;
+    partials[stride + lb] = p1;
+    partials[2 * stride + lb] = p2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup tail: every level below the tail threshold (FP64) runs as
+// one kernel that interprets the host-built cycle program, removing ~10
+// launches per coarse level.
+// ---------------------------------------------------------------------------
+enum TailKind : uint8_t { T_SMOOTH = 0, T_RESTRICT = 1, T_PROLONG = 2 };
+struct TailOp {
+  uint8_t kind;
+  uint8_t level;
+  uint8_t reverse;
+  uint8_t retain;
+  uint16_t sweeps;
+  uint16_t pad;
+};
+
+struct TailLevel {
+  Lv<double> L;      // coefficients in global memory; x/b/r live in LDS
+  int coarsen_mask;
+  int fused;
+  double volume;
+  int lds;           // offset (doubles) of this level's x, b, r in LDS
+};
+
+constexpr int kTailLdsBytes = 64 * 1024;
+
+__device__ __forceinline__ void tail_ijk(const Lv<double>& L, int c, int& i,
+                                         int& j, int& k) {
+  const int plane = L.nx * L.ny;
+  k = c / plane;
+  const int r = c - k * plane;
+  j = r / L.nx;
+  i = r - j * L.nx;
+}
+
+// Level view with the work slots redirected into LDS.
+__device__ __forceinline__ Lv<double> tail_view(const TailLevel& T,
+                                                double* lds) {
+  Lv<double> L = T.L;
+  L.x = lds + T.lds;
+  L.b = L.x + L.cells;
+  L.r = L.b + L.cells;
+  return L;
+}
+
+__device__ __forceinline__ void tail_apply(const Lv<double>& L, const Bc& bc,
+                                           int defect) {
+  for (int c = threadIdx.x; c < L.cells; c += blockDim.x) {
+    int i, j, k;
+    tail_ijk(L, c, i, j, k);
+    const double v = stencil<double>(L, bc, L.x, i, j, k, c, L.x[c]);
+    L.r[c] = defect ? L.b[c] - v : v;
+  }
+  __syncthreads();
+}
+
+__device__ void tail_smooth(const Lv<double>& L, int fused, const Bc& bc,
+                            int sweeps, int reverse, int retain) {
+  const double omega = 0.72;
+  if (!fused) tail_apply(L, bc, 0);
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    const int first = reverse ? 1 : 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int color = (first + pass) & 1;
+      for (int c = threadIdx.x; c < L.cells; c += blockDim.x) {
+        int i, j, k;
+        tail_ijk(L, c, i, j, k);
+        if (((i + j + k) & 1) != color) continue;
+        const double xc = L.x[c];
+        const double v =
+            fused ? stencil<double>(L, bc, L.x, i, j, k, c, xc) : L.r[c];
+        L.x[c] = xc + omega * (L.b[c] - v) / L.D[c];
+      }
+      __syncthreads();
+      const bool final_update = sweep + 1 == sweeps && pass == 1;
+      if (!final_update || retain) {
+        if (!fused || (final_update && retain))
+          tail_apply(L, bc, retain && final_update);
+      }
+    }
+  }
+}
+
+__device__ void tail_project(const Lv<double>& L, double volume, double* field,
+                             double* scratch, double* mean) {
+  double v = 0.0;
+  for (int c = threadIdx.x; c < L.cells; c += blockDim.x)
+    v += L.vol[c] * field[c];
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) *mean = v / volume;
+  __syncthreads();
+  const double m = *mean;
+  for (int c = threadIdx.x; c < L.cells; c += blockDim.x)
+    field[c] = L.vol[c] > 0.0 ? field[c] - m : 0.0;
+  __syncthreads();
+}
+
+// The tail's first level receives b (already projected) and x = 0 from the
+// grid-level restriction; its x is stored back for the grid prolongation.
+__global__ void __launch_bounds__(1024)
+k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
+       Bc bc, int periodic_mask, int null_space) {
+  extern __shared__ double lds[];
+  __shared__ double scratch[16];
+  __shared__ double mean;
+  {
+    const TailLevel& T0 = levels[0];
+    const Lv<double> L0 = tail_view(T0, lds);
+    for (int c = threadIdx.x; c < L0.cells; c += blockDim.x) {
+      L0.x[c] = T0.L.x[c];
+      L0.b[c] = T0.L.b[c];
+    }
+    __syncthreads();
+  }
+  for (int o = 0; o < nops; ++o) {
+    const TailOp op = ops[o];
+    const TailLevel& T = levels[op.level - first_level];
+    const Lv<double> L = tail_view(T, lds);
+    if (op.kind == T_SMOOTH) {
+      tail_smooth(L, T.fused, bc, op.sweeps, op.reverse, op.retain);
+    } else if (op.kind == T_RESTRICT) {
+      const TailLevel& Cn = levels[op.level + 1 - first_level];
+      const Lv<double> Cl = tail_view(Cn, lds);
+      const int mask = T.coarsen_mask;
+      const int mx = mask & 1, my = (mask >> 1) & 1, mz = (mask >> 2) & 1;
+      for (int t = threadIdx.x; t < Cl.cells; t += blockDim.x) {
+        int ci, cj, ck;
+        tail_ijk(Cl, t, ci, cj, ck);
+        const int xb = mx ? 2 * ci : ci, yb = my ? 2 * cj : cj,
+                  zb = mz ? 2 * ck : ck;
+        double sum = 0.0;
+        for (int dz = 0; dz < (mz ? 2 : 1); ++dz)
+          for (int dy = 0; dy < (my ? 2 : 1); ++dy)
+            for (int dx = 0; dx < (mx ? 2 : 1); ++dx) {
+              const int fx = xb + dx, fy = yb + dy, fz = zb + dz;
+              if (fx >= L.nx || fy >= L.ny || fz >= L.nz) continue;
+              sum += L.r[cell_index(L.nx, L.ny, fx, fy, fz)];
+            }
+        Cl.b[t] = sum;
+        Cl.x[t] = 0.0;
+      }
+      __syncthreads();
+      if (null_space) tail_project(Cl, Cn.volume, Cl.b, scratch, &mean);
+    } else {
+      const TailLevel& Cn = levels[op.level + 1 - first_level];
+      const Lv<double> Cl = tail_view(Cn, lds);
+      for (int c = threadIdx.x; c < L.cells; c += blockDim.x) {
+        int i, j, k;
+        tail_ijk(L, c, i, j, k);
+        prolong_cell(Cl, L, T.coarsen_mask, periodic_mask, i, j, k, c);
+      }
+      __syncthreads();
+    }
+  }
+  {
+    const TailLevel& T0 = levels[0];
+    const Lv<double> L0 = tail_view(T0, lds);
+    for (int c = threadIdx.x; c < L0.cells; c += blockDim.x)
+      T0.L.x[c] = L0.x[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Krylov stencil kernels (FP64, chunked).  The operator arrays are the masked
+// level-0 coefficients, which equal the exact IBM operator.
+// ---------------------------------------------------------------------------
+
+// q = A p fused with the partial sum of p.q.
+__global__ void k_op_dot(Tiles t, Lv<double> L, Bc bc, const double* p,
+                         double* q, double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  double d = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    const double pc = p[c];
+    const double v = stencil<double>(L, bc, p, i, j, k, c, pc);
+    q[c] = v;
+    d += pc * v;
+  }
+  if (partials != nullptr) {
+    d = block_sum(d, scratch);
+    if (threadIdx.x == 0) partials[lb] = d;
+  }
+}
+
+// r = b - A x (axpby(r, 1, b, -1, Ax), src/solver_krylov.cpp:1082-1091)
+// fused with the partial sum of r.r.
+__global__ void k_true_residual(Tiles t, Lv<double> L, Bc bc, const double* b,
+                                const double* x, double* r, double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int t0 = lb * t.per;
+  const int t1 = min(t.count, t0 + t.per);
+  double d = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int i, j, k;
+    tile_xyz(t, tile, i, j, k);
+    if (i >= t.nx || j >= t.ny) continue;
+    const int c = cell_index(t.nx, t.ny, i, j, k);
+    const double v = stencil<double>(L, bc, x, i, j, k, c, x[c]);
+    const double rc = 1.0 * b[c] + -1.0 * v;
+    r[c] = rc;
+    d += rc * rc;
+  }
+  d = block_sum(d, scratch);
+  if (threadIdx.x == 0) partials[lb] = d;
+}
+
+}  // namespace hf

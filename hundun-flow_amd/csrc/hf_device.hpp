@@ -63,7 +63,7 @@ struct Bc {
 template <class T>
 struct Lv {
   int nx, ny, nz;
-  int64_t cells;
+  int cells;  // every level fits 32-bit indexing (< 2^31 cells and faces)
   const T* D;
   const T* fx;
   const T* fy;
@@ -77,7 +77,7 @@ struct Lv {
 
 template <class C, class T>
 __device__ __forceinline__ C neighbour_or_boundary(int mode, C self,
-                                                   const T* x, int64_t wrapped) {
+                                                   const T* x, int wrapped) {
   if (mode == 0) return C(0);
   if (mode == 1) return self;
   return ld(x[wrapped]);
@@ -89,30 +89,30 @@ __device__ __forceinline__ C neighbour_or_boundary(int mode, C self,
 // to the sequential reference.
 template <class C, class T>
 __device__ __forceinline__ C stencil(const Lv<T>& L, const Bc& bc, const T* x,
-                                     int i, int j, int k, int64_t c, C xc) {
-  const int64_t nx = L.nx;
-  const int64_t plane = nx * L.ny;
-  const int64_t row = static_cast<int64_t>(j) + static_cast<int64_t>(L.ny) * k;
+                                     int i, int j, int k, int c, C xc) {
+  const int nx = L.nx;
+  const int plane = nx * L.ny;
+  const int row = j + L.ny * k;
   const C xl = i > 0 ? ld(x[c - 1])
                      : neighbour_or_boundary<C>(bc.m[0], xc, x, c + (nx - 1));
-  const C xh = i + 1 < L.nx
+  const C xh = i + 1 < nx
                    ? ld(x[c + 1])
                    : neighbour_or_boundary<C>(bc.m[1], xc, x, c - (nx - 1));
   const C yl = j > 0 ? ld(x[c - nx])
                      : neighbour_or_boundary<C>(bc.m[2], xc, x,
-                                                c + (int64_t)(L.ny - 1) * nx);
+                                                c + (L.ny - 1) * nx);
   const C yh = j + 1 < L.ny
                    ? ld(x[c + nx])
                    : neighbour_or_boundary<C>(bc.m[3], xc, x,
-                                              c - (int64_t)(L.ny - 1) * nx);
+                                              c - (L.ny - 1) * nx);
   const C zl = k > 0 ? ld(x[c - plane])
                      : neighbour_or_boundary<C>(bc.m[4], xc, x,
-                                                c + (int64_t)(L.nz - 1) * plane);
+                                                c + (L.nz - 1) * plane);
   const C zh = k + 1 < L.nz
                    ? ld(x[c + plane])
                    : neighbour_or_boundary<C>(bc.m[5], xc, x,
-                                              c - (int64_t)(L.nz - 1) * plane);
-  const int64_t fyo = c + nx * k;
+                                              c - (L.nz - 1) * plane);
+  const int fyo = c + nx * k;
   C v = C(ld(L.D[c])) * xc;
   v -= C(ld(L.fx[c + row])) * xl;
   v -= C(ld(L.fx[c + row + 1])) * xh;
@@ -132,6 +132,40 @@ __device__ __forceinline__ int64_t xcd_block(int64_t block, int64_t blocks) {
   const int64_t local = block / kXcds;
   return xcd < rem ? xcd * (q + 1) + local
                    : rem * (q + 1) + (xcd - rem) * q + local;
+}
+
+__device__ __forceinline__ int xcd_block32(int block, int blocks) {
+  const int q = blocks / kXcds;
+  const int rem = blocks % kXcds;
+  const int xcd = block % kXcds;
+  const int local = block / kXcds;
+  return xcd < rem ? xcd * (q + 1) + local
+                   : rem * (q + 1) + (xcd - rem) * q + local;
+}
+
+// 2-D tiling of one z-plane: a 256-thread block covers 2^txl cells in x by
+// 256 >> txl rows in y, so (i, j, k) come from uniform per-tile arithmetic
+// and lane bit fields instead of per-thread divisions.  Tiles are numbered
+// x-fastest, then y, then z; chunked kernels give each block `per`
+// consecutive tiles.
+struct Tiles {
+  int nx, ny, nz;  // iteration extents (x counts colour cells for RB passes)
+  int txl;         // log2 of the tile width
+  int gx, gy;      // tiles along x and y
+  int count;       // gx * gy * nz
+  int per;         // tiles per block of a chunked launch
+  int blocks;      // blocks of a chunked launch
+};
+
+__device__ __forceinline__ void tile_xyz(const Tiles& t, int tile, int& i,
+                                         int& j, int& k) {
+  const int gxy = t.gx * t.gy;
+  k = tile / gxy;
+  const int r = tile - k * gxy;
+  const int ty = r / t.gx;
+  i = ((r - ty * t.gx) << t.txl) +
+      static_cast<int>(threadIdx.x & ((1u << t.txl) - 1u));
+  j = ty * (kBlock >> t.txl) + static_cast<int>(threadIdx.x >> t.txl);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -283,7 +283,7 @@ __global__ void k_coarse_cells(Agg a, Fine64 f, Coarse64 c) {
 }
 
 __global__ void k_coarse_faces(Agg a, int axis, const double* fine,
-                               double* coarse) {
+                               double* coarse, double scale) {
   int ce[3] = {a.cn[0], a.cn[1], a.cn[2]};
   ce[axis] += 1;
   int fe[3] = {a.fn[0], a.fn[1], a.fn[2]};
@@ -294,9 +294,11 @@ __global__ void k_coarse_faces(Agg a, int axis, const double* fine,
   const int ci = static_cast<int>(t % ce[0]);
   const int cj = static_cast<int>((t / ce[0]) % ce[1]);
   const int ck = static_cast<int>(t / ((int64_t)ce[0] * ce[1]));
-  coarse[t] = aggregate_entry(a, axis, ci, cj, ck, [&](int i, int j, int k) {
-    return fine[fidx(fe, i, j, k)];
-  });
+  const double v = aggregate_entry(a, axis, ci, cj, ck,
+                                   [&](int i, int j, int k) {
+                                     return fine[fidx(fe, i, j, k)];
+                                   });
+  coarse[t] = scale == 1.0 ? v : v * scale;
 }
 
 __global__ void k_coarse_diag(int nx, int ny, int nz, Coarse64 c,
@@ -380,6 +382,22 @@ enum FinalOp : int {
   F_STORE2 = 7     // scalars[a] = sum0, scalars[a+1] = sum1
 };
 
+// First fold of many block partials: block (s, a) sums span s of
+// accumulator a in index order.
+__global__ void k_fold_partials(const double* partials, int64_t blocks,
+                                int64_t stride, double* out,
+                                int64_t out_stride) {
+  __shared__ double scratch[16];
+  const int64_t a = blockIdx.y;
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * 2048;
+  const int64_t hi = lo + 2048 < blocks ? lo + 2048 : blocks;
+  double v = 0.0;
+  for (int64_t b = lo + threadIdx.x; b < hi; b += blockDim.x)
+    v += partials[a * stride + b];
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) out[a * out_stride + blockIdx.x] = v;
+}
+
 __global__ void __launch_bounds__(1024)
 k_finalize(const double* partials, int64_t blocks, int64_t stride, int nacc,
            double* scalars, int op, int slot, double volume, int unit_linear) {
@@ -435,513 +453,11 @@ k_finalize(const double* partials, int64_t blocks, int64_t stride, int nacc,
   }
 }
 
-template <class T>
-__global__ void k_vol_dot(const T* field, const double* vol, int64_t cells,
-                          double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (c < cells) v = vol[c] * static_cast<double>(ld(field[c]));
-  v = block_sum(v, scratch);
-  if (threadIdx.x == 0) partials[blk] = v;
-}
+}  // namespace hf
 
-__global__ void k_sum(const double* field, int64_t cells, double* partials,
-                      const uint8_t* act) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (c < cells && (act == nullptr || act[c])) v = field[c];
-  v = block_sum(v, scratch);
-  if (threadIdx.x == 0) partials[blk] = v;
-}
+#include "hf_cycle.hpp"
 
-__global__ void k_count(const uint8_t* act, int64_t cells, double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (c < cells) v = act == nullptr ? 1.0 : static_cast<double>(act[c] != 0);
-  v = block_sum(v, scratch);
-  if (threadIdx.x == 0) partials[blk] = v;
-}
-
-__global__ void k_subtract_mean(double* field, int64_t cells,
-                                const double* mean, const uint8_t* act) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  field[c] = (act == nullptr || act[c]) ? field[c] - *mean : 0.0;
-}
-
-// f := vol > 0 ? f - mean : 0 (src/solver_mg.cpp:1445-1452)
-template <class T>
-__global__ void k_project_apply(T* field, const double* vol, int64_t cells,
-                                const double* mean) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  field[c] = vol[c] > 0.0
-                 ? st<T>(static_cast<double>(ld(field[c])) - *mean)
-                 : st<T>(0.0);
-}
-
-// ---------------------------------------------------------------------------
-// Multigrid grid-level kernels.
-// ---------------------------------------------------------------------------
-
-// apply_impl entry (src/solver_mg.cpp:4867-4879): x := 0, b := r on fluid.
-template <class T>
-__global__ void k_mg_entry(const double* r, const uint8_t* act, T* b, T* x,
-                           int64_t cells) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  x[c] = st<T>(0.0);
-  b[c] = act[c] ? st<T>(r[c]) : st<T>(0.0);
-}
-
-// One colour pass of the fused red-black update (src/solver_mg.cpp:
-// 3149-3222): x += 0.72 (b - A x)/D on cells with (i+j+k)&1 == color.  One
-// thread per cell of the colour; same-colour cells never neighbour, so the
-// in-place update is order independent.
-template <class T>
-__global__ void k_rb(Lv<T> L, Bc bc, int color, int64_t work) {
-  using C = typename Compute<T>::type;
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= work) return;
-  const int hx = (L.nx + 1) >> 1;
-  const int ti = static_cast<int>(t % hx);
-  const int64_t rest = t / hx;
-  const int j = static_cast<int>(rest % L.ny);
-  const int k = static_cast<int>(rest / L.ny);
-  const int i = 2 * ti + ((color + j + k) & 1);
-  if (i >= L.nx) return;
-  const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
-  const C xc = ld(L.x[c]);
-  const C v = stencil<C>(L, bc, L.x, i, j, k, c, xc);
-  const C omega = C(0.72);
-  L.x[c] = st<T>(xc + omega * (C(ld(L.b[c])) - v) / C(ld(L.D[c])));
-}
-
-// Separated route for levels with an odd periodic extent
-// (src/solver_mg.cpp:3262-3307): the colour update uses the A x computed
-// before the pass.
-template <class T>
-__global__ void k_rb_separated(Lv<T> L, int color, int64_t work) {
-  using C = typename Compute<T>::type;
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= work) return;
-  const int hx = (L.nx + 1) >> 1;
-  const int ti = static_cast<int>(t % hx);
-  const int64_t rest = t / hx;
-  const int j = static_cast<int>(rest % L.ny);
-  const int k = static_cast<int>(rest / L.ny);
-  const int i = 2 * ti + ((color + j + k) & 1);
-  if (i >= L.nx) return;
-  const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
-  const C omega = C(0.72);
-  L.x[c] = st<T>(C(ld(L.x[c])) +
-                 omega * (C(ld(L.b[c])) - C(ld(L.r[c]))) / C(ld(L.D[c])));
-}
-
-// r := A x (defect = 0) or r := b - A x (defect = 1).
-template <class T>
-__global__ void k_level_apply(Lv<T> L, Bc bc, int defect) {
-  using C = typename Compute<T>::type;
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= L.cells) return;
-  const int i = static_cast<int>(c % L.nx);
-  const int j = static_cast<int>((c / L.nx) % L.ny);
-  const int k = static_cast<int>(c / ((int64_t)L.nx * L.ny));
-  const C v = stencil<C>(L, bc, L.x, i, j, k, c, C(ld(L.x[c])));
-  L.r[c] = defect ? st<T>(C(ld(L.b[c])) - v) : st<T>(v);
-}
-
-// Restriction (src/solver_mg.cpp:3633-3720): coarse b = sum of children in
-// dz, dy, dx order, coarse x := 0.  Optional partial sums of vol*b for the
-// constant projection.
-template <class Tf, class Tc>
-__global__ void k_restrict(Lv<Tf> F, Lv<Tc> Cl, int mask, double* partials) {
-  using C = typename Compute<Tc>::type;
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t t = blk * blockDim.x + threadIdx.x;
-  double pv = 0.0;
-  if (t < Cl.cells) {
-    const int ci = static_cast<int>(t % Cl.nx);
-    const int cj = static_cast<int>((t / Cl.nx) % Cl.ny);
-    const int ck = static_cast<int>(t / ((int64_t)Cl.nx * Cl.ny));
-    const int mx = mask & 1, my = (mask >> 1) & 1, mz = (mask >> 2) & 1;
-    const int xb = mx ? 2 * ci : ci, yb = my ? 2 * cj : cj, zb = mz ? 2 * ck : ck;
-    C sum = C(0);
-    for (int dz = 0; dz < (mz ? 2 : 1); ++dz)
-      for (int dy = 0; dy < (my ? 2 : 1); ++dy)
-        for (int dx = 0; dx < (mx ? 2 : 1); ++dx) {
-          const int fx = xb + dx, fy = yb + dy, fz = zb + dz;
-          if (fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
-          sum += C(ld(F.r[fx + (int64_t)F.nx * (fy + (int64_t)F.ny * fz)]));
-        }
-    Cl.b[t] = st<Tc>(sum);
-    Cl.x[t] = st<Tc>(0.0);
-    if (partials != nullptr) pv = Cl.vol[t] * static_cast<double>(ld(Cl.b[t]));
-  }
-  if (partials != nullptr) {
-    pv = block_sum(pv, scratch);
-    if (threadIdx.x == 0) partials[blk] = pv;
-  }
-}
-
-__device__ __forceinline__ void interp(int fine, int coarsened, int cext,
-                                       int periodic, int* l, int* r,
-                                       int* rw4) {
-  // rw4: right weight in quarters (0, 1 or 3)
-  if (!coarsened) {
-    *l = fine;
-    *r = fine;
-    *rw4 = 0;
-    return;
-  }
-  const int parent = fine / 2;
-  if ((fine & 1) == 0) {
-    *l = parent - 1;
-    *r = parent;
-    *rw4 = 3;
-  } else {
-    *l = parent;
-    *r = parent + 1;
-    *rw4 = 1;
-  }
-  if (!periodic && *l < 0) {
-    *l = 0;
-    *r = 0;
-    *rw4 = 0;
-  } else if (!periodic && *r >= cext) {
-    *l = cext - 1;
-    *r = cext - 1;
-    *rw4 = 0;
-  }
-}
-
-__device__ __forceinline__ int wrapi(int v, int extent) {
-  return v < 0 ? v + extent : (v >= extent ? v - extent : v);
-}
-
-// Prolongation (src/solver_mg.cpp:3893-3993): x_f += sum of ((wx*wy)*wz)*x_c
-// over the (az, ay, ax) corners; fine solids := 0.
-template <class Tc, class Tf>
-__device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
-                                             int mask, int per, int64_t c) {
-  using C = typename Compute<Tf>::type;
-  if (!F.act[c]) {
-    F.x[c] = st<Tf>(0.0);
-    return;
-  }
-  const int i = static_cast<int>(c % F.nx);
-  const int j = static_cast<int>((c / F.nx) % F.ny);
-  const int k = static_cast<int>(c / ((int64_t)F.nx * F.ny));
-  int xl, xr, yl, yr, zl, zr, xw, yw, zw;
-  interp(i, mask & 1, Cl.nx, per & 1, &xl, &xr, &xw);
-  interp(j, (mask >> 1) & 1, Cl.ny, (per >> 1) & 1, &yl, &yr, &yw);
-  interp(k, (mask >> 2) & 1, Cl.nz, (per >> 2) & 1, &zl, &zr, &zw);
-  C value = C(0);
-  for (int az = 0; az < (zl == zr ? 1 : 2); ++az) {
-    const int gz = wrapi(az == 0 ? zl : zr, Cl.nz);
-    const C wz = az == 0 ? C(1) - C(zw) * C(0.25) : C(zw) * C(0.25);
-    for (int ay = 0; ay < (yl == yr ? 1 : 2); ++ay) {
-      const int gy = wrapi(ay == 0 ? yl : yr, Cl.ny);
-      const C wy = ay == 0 ? C(1) - C(yw) * C(0.25) : C(yw) * C(0.25);
-      for (int ax = 0; ax < (xl == xr ? 1 : 2); ++ax) {
-        const int gx = wrapi(ax == 0 ? xl : xr, Cl.nx);
-        const C wx = ax == 0 ? C(1) - C(xw) * C(0.25) : C(xw) * C(0.25);
-        value += wx * wy * wz *
-                 C(ld(Cl.x[gx + (int64_t)Cl.nx * (gy + (int64_t)Cl.ny * gz)]));
-      }
-    }
-  }
-  F.x[c] = st<Tf>(C(ld(F.x[c])) + value);
-}
-
-template <class Tc, class Tf>
-__global__ void k_prolong(Lv<Tc> Cl, Lv<Tf> F, int mask, int per) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= F.cells) return;
-  prolong_cell(Cl, F, mask, per, c);
-}
-
-// apply_impl exit (src/solver_mg.cpp:5082-5118): correction = alpha * x on
-// fluid, 0 on solids; fused with the Krylov inner products r.z (and z.q for
-// the flexible beta).
-template <class T>
-__global__ void k_mg_exit(const T* x, const uint8_t* act, const double* alpha,
-                          double* z, const double* r, const double* q,
-                          int64_t cells, double* partials, int64_t stride) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double rz = 0.0, zq = 0.0;
-  if (c < cells) {
-    const double a = alpha == nullptr ? 1.0 : *alpha;
-    const double v = act[c] ? static_cast<double>(ld(x[c])) * a : 0.0;
-    z[c] = v;
-    if (r != nullptr) rz = r[c] * v;
-    if (q != nullptr) zq = v * q[c];
-  }
-  if (partials != nullptr) {
-    rz = block_sum(rz, scratch);
-    if (threadIdx.x == 0) partials[blk] = rz;
-    if (q != nullptr) {
-      zq = block_sum(zq, scratch);
-      if (threadIdx.x == 0) partials[stride + blk] = zq;
-    }
-  }
-}
-
-// Three dots of the residual-minimising scaling (src/solver_mg.cpp:
-// 4988-5001): r = masked input, az = r - (b - A x).
-template <class T>
-__global__ void k_mg_proj3(const double* rin, const uint8_t* act, const T* rec,
-                           int64_t cells, double* partials, int64_t stride) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-  if (c < cells) {
-    const double r = act[c] ? rin[c] : 0.0;
-    const double az = r - static_cast<double>(ld(rec[c]));
-    p0 = r * r;
-    p1 = r * az;
-    p2 = az * az;
-  }
-  p0 = block_sum(p0, scratch);
-  p1 = block_sum(p1, scratch);
-  p2 = block_sum(p2, scratch);
-  if (threadIdx.x == 0) {
-    partials[blk] = p0;
-    partials[stride + blk] = p1;
-    partials[2 * stride + blk] = p2;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Single-workgroup tail: every level below the tail threshold (FP64) runs as
-// one kernel that interprets the host-built cycle program, removing ~10
-// launches per coarse level.
-// ---------------------------------------------------------------------------
-enum TailKind : uint8_t { T_SMOOTH = 0, T_RESTRICT = 1, T_PROLONG = 2 };
-struct TailOp {
-  uint8_t kind;
-  uint8_t level;
-  uint8_t reverse;
-  uint8_t retain;
-  uint16_t sweeps;
-  uint16_t pad;
-};
-
-struct TailLevel {
-  Lv<double> L;
-  int coarsen_mask;
-  int fused;
-  double volume;
-};
-
-__device__ __forceinline__ void tail_apply(const Lv<double>& L, const Bc& bc,
-                                           int defect) {
-  for (int64_t c = threadIdx.x; c < L.cells; c += blockDim.x) {
-    const int i = static_cast<int>(c % L.nx);
-    const int j = static_cast<int>((c / L.nx) % L.ny);
-    const int k = static_cast<int>(c / ((int64_t)L.nx * L.ny));
-    const double v = stencil<double>(L, bc, L.x, i, j, k, c, L.x[c]);
-    L.r[c] = defect ? L.b[c] - v : v;
-  }
-  __syncthreads();
-}
-
-__device__ void tail_smooth(const TailLevel& T, const Bc& bc, int sweeps,
-                            int reverse, int retain) {
-  const Lv<double>& L = T.L;
-  const double omega = 0.72;
-  if (!T.fused) tail_apply(L, bc, 0);
-  for (int sweep = 0; sweep < sweeps; ++sweep) {
-    const int first = reverse ? 1 : 0;
-    for (int pass = 0; pass < 2; ++pass) {
-      const int color = (first + pass) & 1;
-      for (int64_t c = threadIdx.x; c < L.cells; c += blockDim.x) {
-        const int i = static_cast<int>(c % L.nx);
-        const int j = static_cast<int>((c / L.nx) % L.ny);
-        const int k = static_cast<int>(c / ((int64_t)L.nx * L.ny));
-        if (((i + j + k) & 1) != color) continue;
-        const double xc = L.x[c];
-        const double v =
-            T.fused ? stencil<double>(L, bc, L.x, i, j, k, c, xc) : L.r[c];
-        L.x[c] = xc + omega * (L.b[c] - v) / L.D[c];
-      }
-      __syncthreads();
-      const bool final_update = sweep + 1 == sweeps && pass == 1;
-      if (!final_update || retain) {
-        if (!T.fused || (final_update && retain))
-          tail_apply(L, bc, retain && final_update);
-      }
-    }
-  }
-}
-
-__device__ void tail_project(const Lv<double>& L, double volume, double* field,
-                             double* scratch) {
-  double v = 0.0;
-  for (int64_t c = threadIdx.x; c < L.cells; c += blockDim.x)
-    v += L.vol[c] * field[c];
-  __shared__ double mean;
-  v = block_sum(v, scratch);
-  if (threadIdx.x == 0) mean = v / volume;
-  __syncthreads();
-  const double m = mean;
-  for (int64_t c = threadIdx.x; c < L.cells; c += blockDim.x)
-    field[c] = L.vol[c] > 0.0 ? field[c] - m : 0.0;
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(1024)
-k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
-       Bc bc, int periodic_mask, int null_space) {
-  __shared__ double scratch[16];
-  for (int o = 0; o < nops; ++o) {
-    const TailOp op = ops[o];
-    const TailLevel& T = levels[op.level - first_level];
-    if (op.kind == T_SMOOTH) {
-      tail_smooth(T, bc, op.sweeps, op.reverse, op.retain);
-    } else if (op.kind == T_RESTRICT) {
-      const TailLevel& Cn = levels[op.level + 1 - first_level];
-      const Lv<double>& F = T.L;
-      const Lv<double>& Cl = Cn.L;
-      const int mask = T.coarsen_mask;
-      const int mx = mask & 1, my = (mask >> 1) & 1, mz = (mask >> 2) & 1;
-      for (int64_t t = threadIdx.x; t < Cl.cells; t += blockDim.x) {
-        const int ci = static_cast<int>(t % Cl.nx);
-        const int cj = static_cast<int>((t / Cl.nx) % Cl.ny);
-        const int ck = static_cast<int>(t / ((int64_t)Cl.nx * Cl.ny));
-        const int xb = mx ? 2 * ci : ci, yb = my ? 2 * cj : cj,
-                  zb = mz ? 2 * ck : ck;
-        double sum = 0.0;
-        for (int dz = 0; dz < (mz ? 2 : 1); ++dz)
-          for (int dy = 0; dy < (my ? 2 : 1); ++dy)
-            for (int dx = 0; dx < (mx ? 2 : 1); ++dx) {
-              const int fx = xb + dx, fy = yb + dy, fz = zb + dz;
-              if (fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
-              sum += F.r[fx + (int64_t)F.nx * (fy + (int64_t)F.ny * fz)];
-            }
-        Cl.b[t] = sum;
-        Cl.x[t] = 0.0;
-      }
-      __syncthreads();
-      if (null_space) tail_project(Cl, Cn.volume, Cl.b, scratch);
-    } else {
-      const TailLevel& Cn = levels[op.level + 1 - first_level];
-      for (int64_t c = threadIdx.x; c < T.L.cells; c += blockDim.x)
-        prolong_cell(Cn.L, T.L, T.coarsen_mask, periodic_mask, c);
-      __syncthreads();
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Krylov kernels (FP64).  The operator arrays are the masked level-0
-// coefficients, which equal the exact IBM operator.
-// ---------------------------------------------------------------------------
-
-// q = A p fused with the partial sum of p.q.
-__global__ void k_op_dot(Lv<double> L, Bc bc, const double* p, double* q,
-                         double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double d = 0.0;
-  if (c < L.cells) {
-    const int i = static_cast<int>(c % L.nx);
-    const int j = static_cast<int>((c / L.nx) % L.ny);
-    const int k = static_cast<int>(c / ((int64_t)L.nx * L.ny));
-    const double pc = p[c];
-    const double v = stencil<double>(L, bc, p, i, j, k, c, pc);
-    q[c] = v;
-    d = pc * v;
-  }
-  if (partials != nullptr) {
-    d = block_sum(d, scratch);
-    if (threadIdx.x == 0) partials[blk] = d;
-  }
-}
-
-// r = b - A x fused with the partial sum of r.r.
-__global__ void k_true_residual(Lv<double> L, Bc bc, const double* b,
-                                const double* x, double* r, double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double d = 0.0;
-  if (c < L.cells) {
-    const int i = static_cast<int>(c % L.nx);
-    const int j = static_cast<int>((c / L.nx) % L.ny);
-    const int k = static_cast<int>(c / ((int64_t)L.nx * L.ny));
-    const double v = stencil<double>(L, bc, x, i, j, k, c, x[c]);
-    const double rc = 1.0 * b[c] + -1.0 * v;
-    r[c] = rc;
-    d = rc * rc;
-  }
-  d = block_sum(d, scratch);
-  if (threadIdx.x == 0) partials[blk] = d;
-}
-
-// x += alpha p; r += (-alpha) q; partial sum of r.r.
-__global__ void k_update_xr(double* x, const double* p, double* r,
-                            const double* q, const double* alpha,
-                            int64_t cells, double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double d = 0.0;
-  if (c < cells) {
-    const double a = *alpha;
-    x[c] = x[c] + a * p[c];
-    const double rc = r[c] + -a * q[c];
-    r[c] = rc;
-    d = rc * rc;
-  }
-  d = block_sum(d, scratch);
-  if (threadIdx.x == 0) partials[blk] = d;
-}
-
-// p = 1.0*z + beta*p
-__global__ void k_update_p(double* p, const double* z, const double* beta,
-                           int64_t cells) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  p[c] = 1.0 * z[c] + *beta * p[c];
-}
-
-__global__ void k_copy(double* dst, const double* src, int64_t cells) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c < cells) dst[c] = src[c];
-}
-
-__global__ void k_sumsq(const double* v, int64_t cells, double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double d = 0.0;
-  if (c < cells) d = v[c] * v[c];
-  d = block_sum(d, scratch);
-  if (threadIdx.x == 0) partials[blk] = d;
-}
-
-__global__ void k_dot(const double* a, const double* b, int64_t cells,
-                      double* partials) {
-  __shared__ double scratch[16];
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * blockDim.x + threadIdx.x;
-  double d = 0.0;
-  if (c < cells) d = a[c] * b[c];
-  d = block_sum(d, scratch);
-  if (threadIdx.x == 0) partials[blk] = d;
-}
+namespace hf {
 
 // Adjoint: grad_logrho(c) = -sum over the six faces f of c of
 //   (lam_L - lam_R)(p_L - p_R) * d beta_f / d m_c * d m_c / d log rho_c

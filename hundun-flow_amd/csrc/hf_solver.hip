@@ -78,7 +78,50 @@ struct Level {
   void* x = nullptr;
   void* b = nullptr;
   void* r = nullptr;
+  // launch geometry: one tile per block (cells / colour cells) and chunked
+  Tiles tc{};
+  Tiles tcol{};
+  Tiles tr{};
 };
+
+constexpr int kFoldSpan = 2048;   // partials per block of the first fold
+constexpr int kFoldStride = 4096;  // capacity per accumulator of the fold
+
+int ceil_log2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Tile geometry over an nx x ny x nz iteration space.
+Tiles make_tiles(int nx, int ny, int nz, bool chunked) {
+  Tiles t{};
+  t.nx = nx;
+  t.ny = ny;
+  t.nz = nz;
+  t.txl = std::min(6, ceil_log2(std::max(nx, 1)));
+  const int ty = kBlock >> t.txl;
+  t.gx = (nx + (1 << t.txl) - 1) >> t.txl;
+  t.gy = (ny + ty - 1) / ty;
+  t.count = t.gx * t.gy * nz;
+  // one tile per block: streaming in dispatch order keeps HBM pages hot;
+  // reductions fold their per-block partials in a second pass instead
+  (void)chunked;
+  t.per = 1;
+  t.blocks = t.count;
+  return t;
+}
+
+Chunk make_chunk(int n) {
+  Chunk c{};
+  c.n = n;
+  c.chunk = kBlock;  // one element per thread
+  return c;
+}
+
+int chunk_blocks(const Chunk& c) {
+  return std::max(1, (c.n + c.chunk - 1) / c.chunk);
+}
 
 // One step of the host-built cycle program (src/solver_mg.cpp:4192-4338).
 struct CycleOp {
@@ -87,6 +130,7 @@ struct CycleOp {
   int sweeps;
   int reverse;
   int retain;
+  int from_zero;  // smoothing starts from x = 0 (first visit of a level)
 };
 
 // Executable item: one grid-level op, or a contiguous run of ops on tail
@@ -118,7 +162,7 @@ Lv<T> view(const Level& L) {
   v.nx = L.n[0];
   v.ny = L.n[1];
   v.nz = L.n[2];
-  v.cells = L.cells;
+  v.cells = static_cast<int>(L.cells);
   v.D = static_cast<const T*>(L.DT);
   v.fx = static_cast<const T*>(L.fxT);
   v.fy = static_cast<const T*>(L.fyT);
@@ -136,7 +180,7 @@ Lv<double> op_view(const Level& L) {
   v.nx = L.n[0];
   v.ny = L.n[1];
   v.nz = L.n[2];
-  v.cells = L.cells;
+  v.cells = static_cast<int>(L.cells);
   v.D = L.D64;
   v.fx = L.fx64;
   v.fy = L.fy64;
@@ -183,15 +227,18 @@ struct hf_plan {
   std::vector<TailOp> tail_ops_host;
   TailOp* d_tail_ops = nullptr;
   TailLevel* d_tail_levels = nullptr;
+  size_t tail_lds_bytes = 0;
   bool coefficients_ready = false;
   bool has_activity = false;
   // raw activity masks (copies) for the adjoint
   uint8_t* fxa = nullptr;
   uint8_t* fya = nullptr;
   uint8_t* fza = nullptr;
+  Chunk chunk0{};  // 1-D chunking of level-0 vectors
   // reductions
   double* partials = nullptr;
   int64_t partial_stride = 0;
+  double* folded = nullptr;
   double* scalars = nullptr;
   double* host_scalars = nullptr;  // pinned
   int* d_flag = nullptr;
@@ -356,18 +403,20 @@ int64_t tail_threshold() {
 
 // Build the cycle program exactly as mg_cycle recurses
 // (src/solver_mg.cpp:4192-4338).
-void emit_cycle(hf_plan* p, int level, int counter) {
+void emit_cycle(hf_plan* p, int level, int counter, int from_zero) {
   const int last = static_cast<int>(p->levels.size()) - 1;
   if (level == last) {
-    p->program.push_back({T_SMOOTH, level, p->policy.coarse_sweeps, 0, 0});
+    p->program.push_back(
+        {T_SMOOTH, level, p->policy.coarse_sweeps, 0, 0, from_zero});
     return;
   }
-  p->program.push_back({T_SMOOTH, level, p->policy.pre_sweeps, 0, 1});
-  p->program.push_back({T_RESTRICT, level, 0, 0, 0});
-  emit_cycle(p, level + 1, counter);
-  if (counter > 1) emit_cycle(p, level + 1, counter - 1);
-  p->program.push_back({T_PROLONG, level, 0, 0, 0});
-  p->program.push_back({T_SMOOTH, level, p->policy.post_sweeps, 1, 0});
+  p->program.push_back(
+      {T_SMOOTH, level, p->policy.pre_sweeps, 0, 1, from_zero});
+  p->program.push_back({T_RESTRICT, level, 0, 0, 0, 0});
+  emit_cycle(p, level + 1, counter, 1);
+  if (counter > 1) emit_cycle(p, level + 1, counter - 1, 0);
+  p->program.push_back({T_PROLONG, level, 0, 0, 0, 0});
+  p->program.push_back({T_SMOOTH, level, p->policy.post_sweeps, 1, 0, 0});
 }
 
 // An op belongs to the tail when every level it touches is a tail level.
@@ -380,7 +429,7 @@ void build_exec(hf_plan* p) {
   p->program.clear();
   p->exec.clear();
   p->tail_ops_host.clear();
-  emit_cycle(p, 0, p->policy.f_cycle ? 2 : 1);
+  emit_cycle(p, 0, p->policy.f_cycle ? 2 : 1, 1);
   size_t i = 0;
   while (i < p->program.size()) {
     if (!op_on_tail(p, p->program[i])) {
@@ -413,9 +462,22 @@ void build_exec(hf_plan* p) {
 void finalize(hf_plan* p, int64_t blocks, int nacc, int op, int slot,
               double volume = 0.0, int unit_linear = 1) {
   HF_PROF(p, P_FINALIZE);
-  HF_LAUNCH(k_finalize, 1, 1024, 0, p->stream, p->partials, blocks,
-            p->partial_stride, nacc, p->scalars, op, slot, volume,
-            unit_linear);
+  const double* src = p->partials;
+  int64_t count = blocks;
+  int64_t stride = p->partial_stride;
+  if (blocks > kFoldSpan) {
+    // first pass: fixed spans of partials summed in order (deterministic)
+    const int64_t spans = (blocks + kFoldSpan - 1) / kFoldSpan;
+    hipLaunchKernelGGL(k_fold_partials, dim3(static_cast<uint32_t>(spans),
+                                             static_cast<uint32_t>(nacc)),
+                       dim3(kBlock), 0, p->stream, p->partials, blocks,
+                       p->partial_stride, p->folded, kFoldStride);
+    src = p->folded;
+    count = spans;
+    stride = kFoldStride;
+  }
+  HF_LAUNCH(k_finalize, 1, 1024, 0, p->stream, src, count, stride, nacc,
+            p->scalars, op, slot, volume, unit_linear);
   ++p->reductions;
 }
 
@@ -429,33 +491,49 @@ hf_status read_scalars(hf_plan* p) {
 }
 
 // ---- multigrid ------------------------------------------------------------
+Widths widths(const hf_plan* p) {
+  return Widths{p->d_widths[0], p->d_widths[1], p->d_widths[2]};
+}
+
+// Mean of a level field for the constant projection: scalars[S_MEAN0 + l].
 template <class T>
-void project_level(hf_plan* p, int l, T* field) {
+void level_mean(hf_plan* p, int l, const T* field) {
   Level& L = p->levels[l];
-  const int64_t nb = blocks_for(L.cells);
   {
     HF_PROF(p, P_PROJECT);
-    HF_LAUNCH(k_vol_dot<T>, nb, kBlock, 0, p->stream, field, L.vol, L.cells,
-              p->partials);
+    if (l == 0)
+      HF_LAUNCH(k_vol_dot_finest<T>, L.tr.blocks, kBlock, 0, p->stream, L.tr,
+                field, L.act, widths(p), p->partials);
+    else
+      HF_LAUNCH(k_vol_dot<T>, L.tr.blocks, kBlock, 0, p->stream, L.tr, field,
+                static_cast<const double*>(L.vol), p->partials);
   }
-  finalize(p, nb, 1, F_MEAN, S_MEAN0 + l, L.volume);
-  HF_PROF(p, P_PROJECT);
-  HF_LAUNCH(k_project_apply<T>, nb, kBlock, 0, p->stream, field, L.vol,
-            L.cells, p->scalars + S_MEAN0 + l);
+  finalize(p, L.tr.blocks, 1, F_MEAN, S_MEAN0 + l, L.volume);
 }
 
 template <class T>
-void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain) {
+void project_apply(hf_plan* p, int l, T* field) {
+  Level& L = p->levels[l];
+  HF_PROF(p, P_PROJECT);
+  HF_LAUNCH(k_project_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
+            field, static_cast<const uint8_t*>(L.act),
+            static_cast<const double*>(p->scalars + S_MEAN0 + l));
+}
+
+template <class T>
+void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
+                 int from_zero) {
   Level& L = p->levels[l];
   const Lv<T> v = view<T>(L);
-  const int64_t hx = (L.n[0] + 1) / 2;
-  const int64_t work = hx * L.n[1] * L.n[2];
-  const int64_t nbw = blocks_for(work);
-  const int64_t nbc = blocks_for(L.cells);
   const int rb_class = l == 0 ? P_RB_FINEST : P_RB_COARSE;
-  if (!L.fused) {
+  // a pending constant projection of b is folded into the first pass
+  const double* mean = (from_zero && p->policy.null_space_constant)
+                           ? p->scalars + S_MEAN0 + l
+                           : nullptr;
+  if (!L.fused && !from_zero) {
     HF_PROF(p, P_DEFECT);
-    HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc, 0);
+    HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc, v,
+              p->bc, 0);
   }
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     const int first = reverse ? 1 : 0;
@@ -463,18 +541,22 @@ void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain) {
       const int color = (first + pass) & 1;
       {
         HF_PROF(p, rb_class);
-        if (L.fused)
-          HF_LAUNCH(k_rb<T>, nbw, kBlock, 0, p->stream, v, p->bc, color, work);
+        if (from_zero && sweep == 0 && pass == 0)
+          HF_LAUNCH(k_rb_first<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc, v,
+                    mean);
+        else if (L.fused)
+          HF_LAUNCH(k_rb<T>, L.tcol.blocks, kBlock, 0, p->stream, L.tcol, v,
+                    p->bc, color);
         else
-          HF_LAUNCH(k_rb_separated<T>, nbw, kBlock, 0, p->stream, v, color,
-                    work);
+          HF_LAUNCH(k_rb_separated<T>, L.tcol.blocks, kBlock, 0, p->stream,
+                    L.tcol, v, color);
       }
       const bool final_update = sweep + 1 == sweeps && pass == 1;
       if (!final_update || retain) {
         if (!L.fused || (final_update && retain)) {
           HF_PROF(p, P_DEFECT);
-          HF_LAUNCH(k_level_apply<T>, nbc, kBlock, 0, p->stream, v, p->bc,
-                    (retain && final_update) ? 1 : 0);
+          HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
+                    v, p->bc, (retain && final_update) ? 1 : 0);
         }
       }
     }
@@ -486,43 +568,44 @@ void exec_grid_op(hf_plan* p, const CycleOp& op) {
   if (op.kind == T_SMOOTH) {
     dispatch(L.prec, [&](auto t) {
       using T = decltype(t);
-      grid_smooth<T>(p, op.level, op.sweeps, op.reverse, op.retain);
+      grid_smooth<T>(p, op.level, op.sweeps, op.reverse, op.retain,
+                     op.from_zero);
     });
     return;
   }
   Level& Cn = p->levels[op.level + 1];
   if (op.kind == T_RESTRICT) {
-    const int64_t nb = blocks_for(Cn.cells);
     const bool proj = p->policy.null_space_constant != 0;
+    // a grid-level coarse smoothing starts from zero and applies the
+    // projection itself; the tail needs x = 0 and a projected b up front
+    const bool into_tail = Cn.tail;
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       dispatch(Cn.prec, [&](auto tc) {
         using Tc = decltype(tc);
         {
           HF_PROF(p, P_RESTRICT);
-          HF_LAUNCH((k_restrict<Tf, Tc>), nb, kBlock, 0, p->stream,
-                    view<Tf>(L), view<Tc>(Cn), L.coarsen_mask,
+          HF_LAUNCH((k_restrict<Tf, Tc>), Cn.tr.blocks, kBlock, 0, p->stream,
+                    Cn.tr, view<Tf>(L), view<Tc>(Cn), L.coarsen_mask,
+                    into_tail ? 1 : 0,
                     proj ? p->partials : static_cast<double*>(nullptr));
         }
         if (proj) {
-          finalize(p, nb, 1, F_MEAN, S_MEAN0 + op.level + 1, Cn.volume);
-          HF_PROF(p, P_PROJECT);
-          HF_LAUNCH(k_project_apply<Tc>, nb, kBlock, 0, p->stream,
-                    static_cast<Tc*>(Cn.b), Cn.vol, Cn.cells,
-                    p->scalars + S_MEAN0 + op.level + 1);
+          finalize(p, Cn.tr.blocks, 1, F_MEAN, S_MEAN0 + op.level + 1,
+                   Cn.volume);
+          if (into_tail) project_apply<Tc>(p, op.level + 1, static_cast<Tc*>(Cn.b));
         }
       });
     });
     return;
   }
-  const int64_t nb = blocks_for(L.cells);
   dispatch(Cn.prec, [&](auto tc) {
     using Tc = decltype(tc);
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       HF_PROF(p, P_PROLONG);
-      HF_LAUNCH((k_prolong<Tc, Tf>), nb, kBlock, 0, p->stream, view<Tc>(Cn),
-                view<Tf>(L), L.coarsen_mask, p->periodic_mask);
+      HF_LAUNCH((k_prolong<Tc, Tf>), L.tc.blocks, kBlock, 0, p->stream, L.tc,
+                view<Tc>(Cn), view<Tf>(L), L.coarsen_mask, p->periodic_mask);
     });
   });
 }
@@ -534,7 +617,8 @@ void run_cycle(hf_plan* p) {
       continue;
     }
     HF_PROF(p, P_TAIL);
-    HF_LAUNCH(k_tail, 1, 1024, 0, p->stream, p->d_tail_levels, p->tail_first,
+    HF_LAUNCH(k_tail, 1, 1024, p->tail_lds_bytes, p->stream, p->d_tail_levels,
+              p->tail_first,
               p->d_tail_ops + e.tail_begin, e.tail_count, p->bc,
               p->periodic_mask, static_cast<int>(p->policy.null_space_constant));
   }
@@ -544,38 +628,61 @@ void run_cycle(hf_plan* p) {
 hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
                    bool dots) {
   Level& L0 = p->levels[0];
-  const int64_t nb = blocks_for(L0.cells);
+  const bool proj = p->policy.null_space_constant != 0;
+  const bool tail0 = L0.tail;  // whole cycle inside the tail kernel
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
     T* x = static_cast<T*>(L0.x);
     {
       HF_PROF(p, P_MG_ENTRY_EXIT);
-      HF_LAUNCH(k_mg_entry<T>, nb, kBlock, 0, p->stream, r, L0.act, b, x,
-                L0.cells);
+      HF_LAUNCH(k_mg_entry<T>, L0.tr.blocks, kBlock, 0, p->stream, L0.tr, r,
+                static_cast<const uint8_t*>(L0.act), widths(p), b,
+                tail0 ? x : static_cast<T*>(nullptr),
+                proj ? p->partials : static_cast<double*>(nullptr));
     }
-    if (p->policy.null_space_constant) project_level<T>(p, 0, b);
+    if (proj) {
+      finalize(p, L0.tr.blocks, 1, F_MEAN, S_MEAN0, L0.volume);
+      if (tail0) project_apply<T>(p, 0, b);
+    }
   });
   run_cycle(p);
   const double* alpha = nullptr;
+  const double* mean = nullptr;
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* x = static_cast<T*>(L0.x);
-    if (p->policy.null_space_constant) project_level<T>(p, 0, x);
+    if (proj) {
+      level_mean<T>(p, 0, x);
+      mean = p->scalars + S_MEAN0;
+    }
     if (!p->policy.unit_linear) {
-      HF_LAUNCH(k_level_apply<T>, nb, kBlock, 0, p->stream, view<T>(L0), p->bc,
-                1);
-      HF_LAUNCH(k_mg_proj3<T>, nb, kBlock, 0, p->stream, r, L0.act,
-                static_cast<const T*>(L0.r), L0.cells, p->partials,
-                p->partial_stride);
-      finalize(p, nb, 3, F_MGALPHA, 0, 0.0, 0);
+      // residual-minimising scaling needs the projected candidate stored
+      if (proj) {
+        project_apply<T>(p, 0, x);
+        mean = nullptr;
+      }
+      {
+        HF_PROF(p, P_DEFECT);
+        HF_LAUNCH(k_level_apply<T>, L0.tc.blocks, kBlock, 0, p->stream, L0.tc,
+                  view<T>(L0), p->bc, 1);
+      }
+      {
+        HF_PROF(p, P_MG_ENTRY_EXIT);
+        HF_LAUNCH(k_mg_proj3<T>, L0.tr.blocks, kBlock, 0, p->stream, L0.tr, r,
+                  static_cast<const uint8_t*>(L0.act),
+                  static_cast<const T*>(L0.r), p->partials,
+                  static_cast<int>(p->partial_stride));
+      }
+      finalize(p, L0.tr.blocks, 3, F_MGALPHA, 0, 0.0, 0);
       alpha = p->scalars + S_MGALPHA;
     }
     HF_PROF(p, P_MG_ENTRY_EXIT);
-    HF_LAUNCH(k_mg_exit<T>, nb, kBlock, 0, p->stream, x, L0.act, alpha, z,
-              dots ? r : static_cast<const double*>(nullptr), q, L0.cells,
-              dots ? p->partials : static_cast<double*>(nullptr),
-              p->partial_stride);
+    HF_LAUNCH(k_mg_exit<T>, L0.tr.blocks, kBlock, 0, p->stream, L0.tr,
+              static_cast<const T*>(x), static_cast<const uint8_t*>(L0.act),
+              mean, alpha, z, dots ? r : static_cast<const double*>(nullptr),
+              q, dots ? p->partials : static_cast<double*>(nullptr),
+              static_cast<int>(p->partial_stride));
   });
   return ok();
 }
@@ -585,12 +692,15 @@ hf_status upload_tail(hf_plan* p) {
   const int count = L - p->tail_first;
   if (count <= 0) return ok();
   std::vector<TailLevel> host(count);
+  int64_t lds = 0;
   for (int l = p->tail_first; l < L; ++l) {
     TailLevel& t = host[l - p->tail_first];
     t.L = view<double>(p->levels[l]);
     t.coarsen_mask = p->levels[l].coarsen_mask;
     t.fused = p->levels[l].fused;
     t.volume = p->levels[l].volume;
+    t.lds = static_cast<int>(lds);
+    lds += 3 * p->levels[l].cells;
   }
   if (hipMemcpyAsync(p->d_tail_levels, host.data(),
                      count * sizeof(TailLevel), hipMemcpyHostToDevice,
@@ -635,6 +745,7 @@ hf_status hf_policy_default(hf_policy* policy) {
   policy->null_space_constant = 1;
   policy->unit_linear = 1;
   policy->precision = HF_PRECISION_FP64;
+  policy->coarse_face_scaling = 0;
   return ok();
 }
 
@@ -655,7 +766,8 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   if (policy->pre_sweeps == 0 || policy->post_sweeps == 0 ||
       policy->coarse_sweeps == 0 || policy->maximum_levels < 2 ||
       policy->maximum_levels > kMaxLevels || policy->minimum_coarse_extent < 2 ||
-      !(policy->anisotropy_threshold > 1.0) || policy->precision > 2)
+      !(policy->anisotropy_threshold > 1.0) || policy->precision > 2 ||
+      policy->coarse_face_scaling > 1)
     return fail(HF_INVALID_PLAN, kDetailPlan);
   hf_plan* p = new (std::nothrow) hf_plan();
   if (p == nullptr) return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
@@ -692,21 +804,40 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     delete p;
     return fail(HF_INVALID_PLAN, kDetailPlan);
   }
+  // the tail is the longest run of coarsest levels whose x, b, r fit the
+  // tail kernel's LDS and each stay under the cell threshold
   const int64_t threshold = tail_threshold();
   p->tail_first = static_cast<int>(p->levels.size());
+  int64_t lds_doubles = 0;
   for (int l = static_cast<int>(p->levels.size()) - 1; l >= 0; --l) {
-    if (p->levels[l].cells <= threshold)
+    const int64_t need = lds_doubles + 3 * p->levels[l].cells;
+    if (p->levels[l].cells <= threshold &&
+        need * static_cast<int64_t>(sizeof(double)) <= kTailLdsBytes) {
       p->tail_first = l;
-    else
+      lds_doubles = need;
+    } else {
       break;
+    }
   }
   // the coarsest level is always FP64 and handled by the tail kernel
   if (p->tail_first > static_cast<int>(p->levels.size()) - 1)
     p->tail_first = static_cast<int>(p->levels.size()) - 1;
+  int64_t tail_doubles = 0;
   for (size_t l = 0; l < p->levels.size(); ++l) {
     Level& L = p->levels[l];
     L.tail = static_cast<int>(l) >= p->tail_first;
+    if (L.tail) tail_doubles += 3 * L.cells;
     L.prec = L.tail ? HF_PRECISION_FP64 : policy->precision;
+    L.tc = make_tiles(L.n[0], L.n[1], L.n[2], false);
+    L.tcol = make_tiles((L.n[0] + 1) / 2, L.n[1], L.n[2], false);
+    L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
+  }
+  p->chunk0 = make_chunk(static_cast<int>(p->cells));
+  p->tail_lds_bytes = static_cast<size_t>(tail_doubles) * sizeof(double);
+  if (p->tail_lds_bytes > static_cast<size_t>(kTailLdsBytes)) {
+    // the coarsest level alone exceeds the tail kernel's LDS
+    delete p;
+    return fail(HF_INVALID_PLAN, kDetailPlan);
   }
   build_exec(p);
 
@@ -741,9 +872,13 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     guard(dalloc_raw(p, &L.b, es * L.cells));
     guard(dalloc_raw(p, &L.r, es * L.cells));
   }
-  const int64_t max_blocks = blocks_for(p->cells + 2 * (int64_t)p->n[0] * p->n[1]) + 8;
-  p->partial_stride = max_blocks;
-  guard(dalloc(p, &p->partials, 3 * max_blocks));
+  // one partial per tile (level 0 has the most) or per 256-element block
+  int64_t max_blocks = blocks_for(p->cells);
+  for (const Level& L : p->levels)
+    max_blocks = std::max<int64_t>(max_blocks, L.tc.count);
+  p->partial_stride = max_blocks + 8;
+  guard(dalloc(p, &p->partials, 3 * p->partial_stride));
+  guard(dalloc(p, &p->folded, 3 * kFoldStride));
   guard(dalloc(p, &p->scalars, S_COUNT));
   guard(dalloc(p, &p->d_flag, 4));
   guard(dalloc(p, &p->kx, p->cells));
@@ -870,9 +1005,14 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
     Fine64 fine{F.D64, F.fx64, F.fy64, F.fz64, F.vol};
     Coarse64 coarse{C.D64, C.fx64, C.fy64, C.fz64, C.vol, C.act};
     HF_LAUNCH(k_coarse_cells, blocks_for(C.cells), kBlock, 0, p->stream, a, fine, coarse);
-    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfx), kBlock, 0, p->stream, a, 0, F.fx64, C.fx64);
-    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfy), kBlock, 0, p->stream, a, 1, F.fy64, C.fy64);
-    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfz), kBlock, 0, p->stream, a, 2, F.fz64, C.fz64);
+    // distance-corrected aggregation halves faces whose normal was coarsened
+    const bool dc = p->policy.coarse_face_scaling != 0;
+    const double sx = dc && (F.coarsen_mask & 1) ? 0.5 : 1.0;
+    const double sy = dc && (F.coarsen_mask & 2) ? 0.5 : 1.0;
+    const double sz = dc && (F.coarsen_mask & 4) ? 0.5 : 1.0;
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfx), kBlock, 0, p->stream, a, 0, F.fx64, C.fx64, sx);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfy), kBlock, 0, p->stream, a, 1, F.fy64, C.fy64, sy);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfz), kBlock, 0, p->stream, a, 2, F.fz64, C.fz64, sz);
     HF_LAUNCH(k_coarse_diag, blocks_for(C.cells), kBlock, 0, p->stream, C.n[0], C.n[1], C.n[2], coarse, p->d_flag);
   }
   for (Level& L : p->levels)
@@ -891,9 +1031,10 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
   // projection volumes (sum of fluid volumes per level)
   for (size_t l = 0; l < p->levels.size(); ++l) {
     Level& L = p->levels[l];
-    const int64_t nb = blocks_for(L.cells);
-    HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, L.vol, L.cells, p->partials,
-              static_cast<const uint8_t*>(nullptr));
+    const Chunk ch = make_chunk(static_cast<int>(L.cells));
+    const int nb = chunk_blocks(ch);
+    HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, ch, L.vol,
+              static_cast<const uint8_t*>(nullptr), p->partials);
     finalize(p, nb, 1, F_STORE, S_SUM);
     HF_TRY(read_scalars(p));
     L.volume = p->host_scalars[S_SUM];
@@ -912,8 +1053,9 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
 hf_status hf_operator_apply(hf_plan* p, const double* x, double* y) {
   if (p == nullptr || x == nullptr || y == nullptr || !p->coefficients_ready)
     return fail(HF_INVALID_PLAN, kDetailPlan);
-  HF_LAUNCH(k_op_dot, blocks_for(p->cells), kBlock, 0, p->stream,
-            krylov_view(p), p->bc, x, y, static_cast<double*>(nullptr));
+  const Tiles& t = p->levels[0].tr;
+  HF_LAUNCH(k_op_dot, t.blocks, kBlock, 0, p->stream, t, krylov_view(p),
+            p->bc, x, y, static_cast<double*>(nullptr));
   return hip_check(hipGetLastError());
 }
 
@@ -927,8 +1069,10 @@ hf_status hf_mg_apply(hf_plan* p, const double* r, double* z) {
 static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                           const hf_solve_control* ctl, hf_solve_result* res) {
   const int64_t N = p->cells;
-  const int64_t nb = blocks_for(N);
   const Lv<double> A = krylov_view(p);
+  const Tiles& tr = p->levels[0].tr;
+  const Chunk ch = p->chunk0;
+  const int nbv = chunk_blocks(ch);
   double* x = p->kx;
   double* r = p->kr;
   double* z = p->kz;
@@ -942,8 +1086,8 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     return st;
   };
   // ||b||
-  HF_LAUNCH(k_sumsq, nb, kBlock, 0, p->stream, rhs, N, p->partials);
-  finalize(p, nb, 1, F_STORE, S_SUM);
+  HF_LAUNCH(k_sumsq, nbv, kBlock, 0, p->stream, ch, rhs, p->partials);
+  finalize(p, nbv, 1, F_STORE, S_SUM);
   HF_TRY(read_scalars(p));
   const double bnorm = std::sqrt(p->host_scalars[S_SUM]);
   res->rhs_norm = bnorm;
@@ -955,13 +1099,13 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     HF_TRY(hip_check(hipMemsetAsync(solution, 0, N * sizeof(double), p->stream)));
     return finish(ok(), HF_TERM_ZERO_RHS);
   }
-  HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, x, solution, N);
+  HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, x, solution);
   {
     HF_PROF(p, P_TRUE_RESIDUAL);
-    HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x, r,
-              p->partials);
+    HF_LAUNCH(k_true_residual, tr.blocks, kBlock, 0, p->stream, tr, A, p->bc,
+              rhs, x, r, p->partials);
   }
-  finalize(p, nb, 1, F_STORE, S_RNORM2);
+  finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
   res->operator_applies++;
   HF_TRY(read_scalars(p));
   res->initial_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
@@ -970,14 +1114,14 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   if (!std::isfinite(res->initial_true_residual))
     return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_OPERATOR_FAILURE);
   if (res->initial_true_residual <= tol) {
-    HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, solution, x, N);
+    HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, solution, x);
     return finish(ok(), HF_TERM_CONVERGED);
   }
   auto precondition_restart = [&]() -> hf_status {
     HF_TRY(mg_apply(p, r, z, nullptr, true));
     res->preconditioner_applies++;
-    finalize(p, nb, 1, F_RHO, 0);
-    HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, pp, z, N);
+    finalize(p, p->levels[0].tr.blocks, 1, F_RHO, 0);
+    HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, pp, z);
     return ok();
   };
   HF_TRY(precondition_restart());
@@ -990,17 +1134,17 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   while (res->iterations < ctl->maximum_iterations) {
     {
       HF_PROF(p, P_OPERATOR);
-      HF_LAUNCH(k_op_dot, nb, kBlock, 0, p->stream, A, p->bc, pp, q,
+      HF_LAUNCH(k_op_dot, tr.blocks, kBlock, 0, p->stream, tr, A, p->bc, pp, q,
                 p->partials);
     }
     res->operator_applies++;
-    finalize(p, nb, 1, F_ALPHA, 0);
+    finalize(p, tr.blocks, 1, F_ALPHA, 0);
     {
       HF_PROF(p, P_UPDATE_XR);
-      HF_LAUNCH(k_update_xr, nb, kBlock, 0, p->stream, x, pp, r, q,
-                p->scalars + S_ALPHA, N, p->partials);
+      HF_LAUNCH(k_update_xr, nbv, kBlock, 0, p->stream, ch, x, pp, r, q,
+                p->scalars + S_ALPHA, p->partials);
     }
-    finalize(p, nb, 1, F_STORE, S_RNORM2);
+    finalize(p, nbv, 1, F_STORE, S_RNORM2);
     res->iterations++;
     HF_TRY(read_scalars(p));
     const double pap = p->host_scalars[S_PAP];
@@ -1018,10 +1162,10 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     if (verify) {
       {
         HF_PROF(p, P_TRUE_RESIDUAL);
-        HF_LAUNCH(k_true_residual, nb, kBlock, 0, p->stream, A, p->bc, rhs, x,
-                  r, p->partials);
+        HF_LAUNCH(k_true_residual, tr.blocks, kBlock, 0, p->stream, tr, A,
+                  p->bc, rhs, x, r, p->partials);
       }
-      finalize(p, nb, 1, F_STORE, S_RNORM2);
+      finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
       res->operator_applies++;
       HF_TRY(read_scalars(p));
       res->final_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
@@ -1029,7 +1173,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       if (!std::isfinite(res->final_true_residual))
         return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_OPERATOR_FAILURE);
       if (res->final_true_residual <= tol) {
-        HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, solution, x, N);
+        HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, solution, x);
         return finish(ok(), HF_TERM_CONVERGED);
       }
       if (res->iterations == ctl->maximum_iterations)
@@ -1039,10 +1183,11 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     }
     HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true));
     res->preconditioner_applies++;
-    finalize(p, nb, ctl->flexible ? 2 : 1, ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
+    finalize(p, tr.blocks, ctl->flexible ? 2 : 1,
+             ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
     HF_PROF(p, P_UPDATE_P);
-    HF_LAUNCH(k_update_p, nb, kBlock, 0, p->stream, pp, z, p->scalars + S_BETA,
-              N);
+    HF_LAUNCH(k_update_p, nbv, kBlock, 0, p->stream, ch, pp, z,
+              p->scalars + S_BETA);
   }
   return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
 }
@@ -1086,16 +1231,17 @@ hf_status hf_pcg_solve(hf_plan* p, const double* rhs, double* solution,
 hf_status hf_project_mean(hf_plan* p, double* field) {
   if (p == nullptr || field == nullptr || !p->coefficients_ready)
     return fail(HF_INVALID_PLAN, kDetailPlan);
-  const int64_t nb = blocks_for(p->cells);
+  const Chunk ch = p->chunk0;
+  const int nb = chunk_blocks(ch);
   const uint8_t* act = p->levels[0].act;
-  HF_LAUNCH(k_count, nb, kBlock, 0, p->stream, act, p->cells, p->partials);
+  HF_LAUNCH(k_count, nb, kBlock, 0, p->stream, ch, act, p->partials);
   finalize(p, nb, 1, F_STORE, S_SUM2);
   HF_TRY(read_scalars(p));
   const double count = p->host_scalars[S_SUM2];
   if (!(count > 0.0)) return fail(HF_NUMERICAL_FAILURE, kDetailNumeric);
-  HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, field, p->cells, p->partials, act);
+  HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, ch, field, act, p->partials);
   finalize(p, nb, 1, F_MEAN, S_SUM, count);
-  HF_LAUNCH(k_subtract_mean, nb, kBlock, 0, p->stream, field, p->cells,
+  HF_LAUNCH(k_subtract_mean, nb, kBlock, 0, p->stream, ch, field,
             p->scalars + S_SUM, act);
   return hip_check(hipGetLastError());
 }
@@ -1111,19 +1257,21 @@ hf_status hf_adjoint_gradients(hf_plan* p, const double* mobility,
     return fail(HF_INVALID_PLAN, kDetailPlan);
   const int64_t N = p->cells;
   const int64_t nb = blocks_for(N);
+  const Chunk ch = p->chunk0;
+  const int nbv = chunk_blocks(ch);
   if (p->adj_rhs == nullptr) {
     HF_TRY(dalloc(p, &p->adj_rhs, N));
     HF_TRY(dalloc(p, &p->adj_lambda, N));
   }
   // adjoint rhs: P grad_p (solids zero, zero fluid mean)
-  HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, p->adj_rhs, grad_p, N);
+  HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, p->adj_rhs, grad_p);
   HF_TRY(hf_project_mean(p, p->adj_rhs));
   HF_TRY(hip_check(hipMemsetAsync(p->adj_lambda, 0, N * sizeof(double), p->stream)));
   const hf_status st = hf_pcg_solve(p, p->adj_rhs, p->adj_lambda, ctl, res);
   if (st.code != HF_OK) return st;
   HF_TRY(hf_project_mean(p, p->adj_lambda));
   if (grad_b != nullptr)
-    HF_LAUNCH(k_copy, nb, kBlock, 0, p->stream, grad_b, p->adj_lambda, N);
+    HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, grad_b, p->adj_lambda);
   if (grad_logrho != nullptr) {
     HF_LAUNCH(k_grad_logrho, nb, kBlock, 0, p->stream, axis_geom(p), mobility,
               pressure, p->adj_lambda,

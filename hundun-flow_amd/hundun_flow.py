@@ -66,7 +66,8 @@ class hf_policy(ctypes.Structure):
                 ("f_cycle", ctypes.c_uint8),
                 ("null_space_constant", ctypes.c_uint8),
                 ("unit_linear", ctypes.c_uint8),
-                ("precision", ctypes.c_uint8)]
+                ("precision", ctypes.c_uint8),
+                ("coarse_face_scaling", ctypes.c_uint8)]
 
 
 class hf_solve_control(ctypes.Structure):

@@ -102,6 +102,13 @@ typedef struct hf_policy {
   uint8_t null_space_constant;  /* MgNullSpace::constant */
   uint8_t unit_linear;          /* MgCorrectionScaling::unit_linear */
   uint8_t precision;            /* enum hf_precision */
+  /* 0: reference aggregation, coarse face = sum of the fine faces tiling it
+   *    (src/solver_mg.cpp:1706-1962).
+   * 1: distance-corrected aggregation (not in the reference): a coarse face
+   *    whose normal axis was coarsened is additionally scaled by 1/2, the
+   *    ratio of fine to coarse centre distance, which makes the coarse rows a
+   *    finite-volume rediscretisation and the V-cycle grid independent. */
+  uint8_t coarse_face_scaling;
 } hf_policy;
 
 /* LinearSolveControl (v04_linear.hpp:454-466) restricted to PCG. */

@@ -393,6 +393,16 @@ static int build_coarse(or_mg* mg) {
     aggregate(F, 0, F->fx, C->fx, s1, s2);
     aggregate(F, 1, F->fy, C->fy, s1, s2);
     aggregate(F, 2, F->fz, C->fz, s1, s2);
+    if (mg->policy.coarse_face_scaling) {
+      /* Not in the reference: rediscretise the aggregated transmissibility
+       * over the doubled centre distance of a coarsened normal axis. */
+      if (F->coarsen_mask & 1)
+        for (size_t f = 0; f < C->nfx; ++f) C->fx[f] *= 0.5;
+      if (F->coarsen_mask & 2)
+        for (size_t f = 0; f < C->nfy; ++f) C->fy[f] *= 0.5;
+      if (F->coarsen_mask & 4)
+        for (size_t f = 0; f < C->nfz; ++f) C->fz[f] *= 0.5;
+    }
     for (int k = 0; k < C->n[2]; ++k)
       for (int j = 0; j < C->n[1]; ++j)
         for (int i = 0; i < C->n[0]; ++i) {

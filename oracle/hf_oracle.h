@@ -38,6 +38,10 @@ typedef struct or_policy {
   int f_cycle;                  /* 0 = V, 1 = F */
   int null_space_constant;      /* 0 none, 1 constant */
   int unit_linear;              /* 1 unit_linear, 0 residual_minimizing */
+  int coarse_face_scaling;      /* 0 reference aggregation (sum of fine
+                                   faces); 1 distance-corrected: a coarse face
+                                   whose normal axis was coarsened is scaled
+                                   by 1/2 (the coarse centre distance doubles) */
 } or_policy;
 
 typedef struct or_control {

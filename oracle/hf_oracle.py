@@ -22,7 +22,8 @@ class or_policy(ctypes.Structure):
                 ("minimum_coarse_extent", ctypes.c_int),
                 ("f_cycle", ctypes.c_int),
                 ("null_space_constant", ctypes.c_int),
-                ("unit_linear", ctypes.c_int)]
+                ("unit_linear", ctypes.c_int),
+                ("coarse_face_scaling", ctypes.c_int)]
 
 
 class or_control(ctypes.Structure):
@@ -144,7 +145,7 @@ def policy(**kw) -> or_policy:
     p = or_policy(anisotropy_threshold=4.0, pre_sweeps=2, post_sweeps=2,
                   maximum_levels=32, coarse_sweeps=24,
                   minimum_coarse_extent=3, f_cycle=0, null_space_constant=1,
-                  unit_linear=1)
+                  unit_linear=1, coarse_face_scaling=0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -82,11 +82,12 @@ def test_assemble_bitwise(torch_cuda, name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-def test_hierarchy_bitwise(torch_cuda, name):
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_hierarchy_bitwise(torch_cuda, name, scaling):
     torch = torch_cuda
-    _say(f"[hierarchy] case={name}")
+    _say(f"[hierarchy] case={name} coarse_face_scaling={scaling}")
     case = CASES[name]()
-    pol = _policy(name)
+    pol = _policy(name, coarse_face_scaling=scaling)
     plan = _gpu_plan(torch, case, **pol)
     _, mg = O.from_case(case, O.policy(**pol))
     assert plan.level_count() == mg.level_count()
@@ -126,16 +127,19 @@ def test_operator(torch_cuda, name):
 @pytest.mark.parametrize("name", ["mixed", "odd_periodic"])
 @pytest.mark.parametrize("f_cycle", [0, 1])
 @pytest.mark.parametrize("tail", ["default", "0"])
-def test_mg_apply_bitwise(torch_cuda, monkeypatch, name, f_cycle, tail):
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_mg_apply_bitwise(torch_cuda, monkeypatch, name, f_cycle, tail,
+                          scaling):
     """Without a null-space projection every V/F-cycle step is a fixed-order
     FP64 computation, so the GPU cycle equals the oracle bit for bit, both
     through the single-workgroup tail and through grid-level kernels."""
     torch = torch_cuda
-    _say(f"[mg bitwise] case={name} f_cycle={f_cycle} tail={tail}")
+    _say(f"[mg bitwise] case={name} f_cycle={f_cycle} tail={tail} "
+         f"scaling={scaling}")
     if tail != "default":
         monkeypatch.setenv("HF_TAIL_CELLS", tail)
     case = CASES[name]()
-    pol = _policy(name, f_cycle=f_cycle)
+    pol = _policy(name, f_cycle=f_cycle, coarse_face_scaling=scaling)
     plan = _gpu_plan(torch, case, **pol)
     _, mg = O.from_case(case, O.policy(**pol))
     r = np.random.default_rng(5).standard_normal(plan.cells)
@@ -180,11 +184,12 @@ def _gauge(x, act):
 
 @pytest.mark.parametrize("name", ["sphere32", "step", "stretched24", "mixed"])
 @pytest.mark.parametrize("flexible", [False, True])
-def test_pcg_fp64_matches_oracle(torch_cuda, name, flexible):
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_pcg_fp64_matches_oracle(torch_cuda, name, flexible, scaling):
     torch = torch_cuda
-    _say(f"[pcg fp64] case={name} flexible={flexible}")
+    _say(f"[pcg fp64] case={name} flexible={flexible} scaling={scaling}")
     case = CASES[name]()
-    pol = _policy(name)
+    pol = _policy(name, coarse_face_scaling=scaling)
     plan = _gpu_plan(torch, case, **pol)
     (d, bx, by, bz), mg = O.from_case(case, O.policy(**pol))
     b = case["rhs"]
@@ -211,14 +216,19 @@ def test_pcg_fp64_matches_oracle(torch_cuda, name, flexible):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("name", ["sphere32", "step"])
-def test_mixed_precision_reaches_tolerance(torch_cuda, precision, name):
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_mixed_precision_reaches_tolerance(torch_cuda, monkeypatch, precision,
+                                           name, scaling):
     """FP32 / BF16-storage V-cycle inside the FP64 CG: the FP64 true residual
     reaches 1e-10 and the solution agrees with the FP64 oracle to the
-    conditioning bound."""
+    conditioning bound.  A zero tail threshold keeps every level but the
+    coarsest in the storage precision."""
     torch = torch_cuda
-    _say(f"[mixed precision] case={name} precision={precision}")
+    _say(f"[mixed precision] case={name} precision={precision} "
+         f"scaling={scaling}")
+    monkeypatch.setenv("HF_TAIL_CELLS", "0")
     case = CASES[name]()
-    pol = _policy(name, precision=precision)
+    pol = _policy(name, precision=precision, coarse_face_scaling=scaling)
     plan = _gpu_plan(torch, case, **pol)
     (d, bx, by, bz), mg = O.from_case(case, O.policy(**_policy(name)))
     b = case["rhs"]
