@@ -692,3 +692,186 @@ __global__ void k_true_residual(Tiles t, Lv<double> L, Bc bc, const double* b,
 }
 
 }  // namespace hf
+
+namespace hf {
+
+// ---------------------------------------------------------------------------
+// Fused red-black sweep (both colour passes of src/solver_mg.cpp:3149-3222 in
+// one launch).  A block owns a 128 x 8 tile of a z-chunk and marches through
+// its planes with four LDS plane buffers holding x with a two-cell ring.  At
+// step k the first colour is updated on plane k over the tile plus a one-cell
+// ring (recomputing what the neighbouring tiles own, from the same inputs),
+// then the second colour on plane k-1 over the tile, which then has every
+// first-colour neighbour it needs.  x is read from L.x and written to xout
+// (out of place), so tiles never see each other's updates; every update uses
+// the reference arithmetic, so the result equals two colour passes bit for
+// bit.  With `from_zero` the input x is zero (no loads) and a pending
+// constant projection (`mean`) of b is applied on the fly and stored to bout.
+// ---------------------------------------------------------------------------
+constexpr int kSwTX = 128;
+constexpr int kSwTY = 8;
+constexpr int kSwPX = kSwTX + 4;
+constexpr int kSwPY = kSwTY + 4;
+constexpr int kSwPlane = kSwPX * kSwPY;
+constexpr int kSwRingRow = kSwTX / 2 + 1;  // first-colour cells per ring row
+
+template <class T>
+struct SweepArgs {
+  Lv<T> L;
+  Bc bc;
+  T* xout;
+  T* bout;             // projected b (from_zero with mean), else null
+  const double* mean;  // pending projection mean, else null
+  int first;           // colour updated first: 0 forward, 1 reverse
+  int from_zero;
+  int zc;              // planes per block
+  int gx, gy;          // tiles along x and y
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
+  using C = typename Compute<T>::type;
+  __shared__ C buf[4][kSwPlane];
+  const Lv<T>& L = a.L;
+  int blk = xcd_block32(blockIdx.x, gridDim.x);
+  const int tx = blk % a.gx;
+  blk /= a.gx;
+  const int ty = blk % a.gy;
+  const int tz = blk / a.gy;
+  const int x0 = tx * kSwTX, y0 = ty * kSwTY;
+  const int z0 = tz * a.zc;
+  const int z1 = min(L.nz, z0 + a.zc);
+  const bool perx = a.bc.m[0] == 2, pery = a.bc.m[2] == 2,
+             perz = a.bc.m[4] == 2;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const C omega = C(0.72);
+
+  auto plane_of = [&](int kk, int& kg) -> bool {
+    kg = kk;
+    if (kk >= 0 && kk < nz) return true;
+    if (!perz) return false;
+    kg = kk < 0 ? kk + nz : kk - nz;
+    return true;
+  };
+  auto load = [&](int kk) {
+    C* P = buf[kk & 3];
+    int kg;
+    const bool valid = plane_of(kk, kg) && !a.from_zero;
+    for (int p = threadIdx.x; p < kSwPlane; p += kBlock) {
+      const int iy = p / kSwPX;
+      const int ix = p - iy * kSwPX - 2;
+      int gi = x0 + ix, gj = y0 + iy - 2;
+      C v = C(0);
+      // only positions within two cells of the domain are ever read
+      if (valid && gi >= -2 && gi <= nx + 1 && gj >= -2 && gj <= ny + 1) {
+        bool ok = true;
+        if (gi < 0 || gi >= nx) {
+          if (perx) gi = wrapi(gi, nx); else ok = false;
+        }
+        if (gj < 0 || gj >= ny) {
+          if (pery) gj = wrapi(gj, ny); else ok = false;
+        }
+        if (ok) v = C(ld(L.x[cell_index(nx, ny, gi, gj, kg)]));
+      }
+      P[p] = v;
+    }
+  };
+  auto bval = [&](int c) -> C {
+    if (a.mean == nullptr) return C(ld(L.b[c]));
+    return L.act[c] ? C(ld(st<T>(static_cast<double>(ld(L.b[c])) - *a.mean)))
+                    : C(0);
+  };
+  auto mode = [](int m, C self) -> C { return m == 0 ? C(0) : self; };
+  // x <- x + omega (b - A x) / D at one cell held in LDS
+  auto update = [&](const C* Pm, C* P0, const C* Pp, int ix, int iy, int gi,
+                    int gj, int kg) {
+    const int q = (ix + 2) + kSwPX * (iy + 2);
+    const int c = cell_index(nx, ny, gi, gj, kg);
+    const C xc = P0[q];
+    const C xl = (gi > 0 || perx) ? P0[q - 1] : mode(a.bc.m[0], xc);
+    const C xh = (gi + 1 < nx || perx) ? P0[q + 1] : mode(a.bc.m[1], xc);
+    const C yl = (gj > 0 || pery) ? P0[q - kSwPX] : mode(a.bc.m[2], xc);
+    const C yh = (gj + 1 < ny || pery) ? P0[q + kSwPX] : mode(a.bc.m[3], xc);
+    const C zl = (kg > 0 || perz) ? Pm[q] : mode(a.bc.m[4], xc);
+    const C zh = (kg + 1 < nz || perz) ? Pp[q] : mode(a.bc.m[5], xc);
+    const int row = gj + ny * kg;
+    const int fyo = c + nx * kg;
+    const C d = C(ld(L.D[c]));
+    C v = d * xc;
+    v -= C(ld(L.fx[c + row])) * xl;
+    v -= C(ld(L.fx[c + row + 1])) * xh;
+    v -= C(ld(L.fy[fyo])) * yl;
+    v -= C(ld(L.fy[fyo + nx])) * yh;
+    v -= C(ld(L.fz[c])) * zl;
+    v -= C(ld(L.fz[c + nx * ny])) * zh;
+    P0[q] = xc + omega * (bval(c) - v) / d;
+  };
+
+  const int c0 = a.first, c1 = a.first ^ 1;
+  const int kstart = z0 - 1;
+  load(kstart - 1);
+  load(kstart);
+  for (int k = kstart; k <= z1; ++k) {
+    load(k + 1);
+    __syncthreads();
+    int kg;
+    if (plane_of(k, kg)) {
+      // first colour on the tile and its one-cell ring
+      const C* Pm = buf[(k - 1) & 3];
+      C* P0 = buf[k & 3];
+      const C* Pp = buf[(k + 1) & 3];
+      for (int q = threadIdx.x; q < (kSwTY + 2) * kSwRingRow; q += kBlock) {
+        const int r = q / kSwRingRow;
+        const int m = q - r * kSwRingRow;
+        const int iy = r - 1;
+        int gj = y0 + iy;
+        // parity with the unwrapped index equals the wrapped one: the fused
+        // route only runs with even periodic extents
+        const int s = (c0 - (x0 - 1 + gj + kg)) & 1;
+        const int ix = -1 + s + 2 * m;
+        int gi = x0 + ix;
+        // ring cells beyond a periodic face are the wrapped cells
+        if (gi < -1 || gi > nx || gj < -1 || gj > ny) continue;
+        if (gi < 0 || gi >= nx) {
+          if (!perx) continue;
+          gi = wrapi(gi, nx);
+        }
+        if (gj < 0 || gj >= ny) {
+          if (!pery) continue;
+          gj = wrapi(gj, ny);
+        }
+        update(Pm, P0, Pp, ix, iy, gi, gj, kg);
+      }
+    }
+    __syncthreads();
+    const int kb = k - 1;
+    if (kb >= z0 && kb < z1) {
+      // second colour on the tile, one plane behind
+      const C* Pm = buf[(kb - 1) & 3];
+      C* P0 = buf[kb & 3];
+      const C* Pp = buf[(kb + 1) & 3];
+      for (int q = threadIdx.x; q < kSwTY * (kSwTX / 2); q += kBlock) {
+        const int iy = q / (kSwTX / 2);
+        const int m = q - iy * (kSwTX / 2);
+        const int gj = y0 + iy;
+        const int s = (c1 - (x0 + gj + kb)) & 1;
+        const int ix = s + 2 * m;
+        const int gi = x0 + ix;
+        if (gi >= nx || gj >= ny) continue;
+        update(Pm, P0, Pp, ix, iy, gi, gj, kb);
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kBlock) {
+        const int iy = p / kSwTX;
+        const int ix = p - iy * kSwTX;
+        const int gi = x0 + ix, gj = y0 + iy;
+        if (gi >= nx || gj >= ny) continue;
+        const int c = cell_index(nx, ny, gi, gj, kb);
+        a.xout[c] = st<T>(P0[(ix + 2) + kSwPX * (iy + 2)]);
+        if (a.bout != nullptr) a.bout[c] = st<T>(bval(c));
+      }
+    }
+  }
+}
+
+}  // namespace hf

@@ -36,7 +36,9 @@ enum ProfileClass : int {
   P_UPDATE_P = 11,
   P_FINALIZE = 12,
   P_OTHER = 13,
-  kProfileClasses = 14
+  P_SWEEP_FINEST = 14,  // fused two-colour sweep on level 0
+  P_SWEEP_COARSE = 15,  // fused two-colour sweep on grid levels >= 1
+  kProfileClasses = 16
 };
 constexpr uint32_t kDetailPlan = 9101U;
 constexpr uint32_t kDetailAlloc = 9102U;
@@ -74,10 +76,16 @@ struct Level {
   void* fxT = nullptr;
   void* fyT = nullptr;
   void* fzT = nullptr;
-  // workspace slots in storage precision
+  // workspace slots in storage precision; x and b are the current buffers of
+  // a ping-pong pair on levels smoothed by the fused sweep kernel
   void* x = nullptr;
   void* b = nullptr;
   void* r = nullptr;
+  void* x2 = nullptr;
+  void* b2 = nullptr;
+  bool sweep = false;  // fused two-colour sweep kernel
+  int sweep_zc = 0;    // planes per block of the sweep kernel
+  int sweep_gx = 0, sweep_gy = 0, sweep_blocks = 0;
   // launch geometry: one tile per block (cells / colour cells) and chunked
   Tiles tc{};
   Tiles tcol{};
@@ -392,6 +400,14 @@ void plan_levels(hf_plan* p) {
   }
 }
 
+// The fused sweep is exact but not yet faster than two colour passes (each
+// plane step still waits on its coefficient loads); opt in with
+// HF_FUSED_SWEEP=1.
+bool sweep_enabled() {
+  const char* env = std::getenv("HF_FUSED_SWEEP");
+  return env != nullptr && std::atoi(env) != 0;
+}
+
 int64_t tail_threshold() {
   const char* env = std::getenv("HF_TAIL_CELLS");
   if (env != nullptr) {
@@ -521,9 +537,48 @@ void project_apply(hf_plan* p, int l, T* field) {
 }
 
 template <class T>
+void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
+                  int from_zero) {
+  Level& L = p->levels[l];
+  const int rb_class = l == 0 ? P_SWEEP_FINEST : P_SWEEP_COARSE;
+  const double* mean = (from_zero && p->policy.null_space_constant)
+                           ? p->scalars + S_MEAN0 + l
+                           : nullptr;
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    const bool zero = from_zero && sweep == 0;
+    SweepArgs<T> a;
+    a.L = view<T>(L);
+    a.bc = p->bc;
+    a.xout = static_cast<T*>(L.x2);
+    a.bout = (zero && mean != nullptr) ? static_cast<T*>(L.b2) : nullptr;
+    a.mean = zero ? mean : nullptr;
+    a.first = reverse ? 1 : 0;
+    a.from_zero = zero ? 1 : 0;
+    a.zc = L.sweep_zc;
+    a.gx = L.sweep_gx;
+    a.gy = L.sweep_gy;
+    {
+      HF_PROF(p, rb_class);
+      HF_LAUNCH(k_rb_sweep<T>, L.sweep_blocks, kBlock, 0, p->stream, a);
+    }
+    std::swap(L.x, L.x2);
+    if (a.bout != nullptr) std::swap(L.b, L.b2);
+  }
+  if (retain) {
+    HF_PROF(p, P_DEFECT);
+    HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
+              view<T>(L), p->bc, 1);
+  }
+}
+
+template <class T>
 void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
                  int from_zero) {
   Level& L = p->levels[l];
+  if (L.sweep) {
+    sweep_smooth<T>(p, l, sweeps, reverse, retain, from_zero);
+    return;
+  }
   const Lv<T> v = view<T>(L);
   const int rb_class = l == 0 ? P_RB_FINEST : P_RB_COARSE;
   // a pending constant projection of b is folded into the first pass
@@ -831,6 +886,18 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tc = make_tiles(L.n[0], L.n[1], L.n[2], false);
     L.tcol = make_tiles((L.n[0] + 1) / 2, L.n[1], L.n[2], false);
     L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
+    // fused sweeps on grid levels with the fused colour route and rows long
+    // enough for the 128 x 8 tiles to stay mostly busy
+    L.sweep = !L.tail && L.fused && L.n[0] >= 32 && L.n[1] >= 8 &&
+              sweep_enabled();
+    if (L.sweep) {
+      L.sweep_gx = (L.n[0] + kSwTX - 1) / kSwTX;
+      L.sweep_gy = (L.n[1] + kSwTY - 1) / kSwTY;
+      const int per_plane = L.sweep_gx * L.sweep_gy;
+      const int want = std::max(1, 2048 / per_plane);
+      L.sweep_zc = std::min(32, std::max(4, (L.n[2] + want - 1) / want));
+      L.sweep_blocks = per_plane * ((L.n[2] + L.sweep_zc - 1) / L.sweep_zc);
+    }
   }
   p->chunk0 = make_chunk(static_cast<int>(p->cells));
   p->tail_lds_bytes = static_cast<size_t>(tail_doubles) * sizeof(double);
@@ -871,6 +938,10 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     guard(dalloc_raw(p, &L.x, es * L.cells));
     guard(dalloc_raw(p, &L.b, es * L.cells));
     guard(dalloc_raw(p, &L.r, es * L.cells));
+    if (L.sweep) {
+      guard(dalloc_raw(p, &L.x2, es * L.cells));
+      guard(dalloc_raw(p, &L.b2, es * L.cells));
+    }
   }
   // one partial per tile (level 0 has the most) or per 256-element block
   int64_t max_blocks = blocks_for(p->cells);

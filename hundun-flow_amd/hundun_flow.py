@@ -31,7 +31,7 @@ PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 PROFILE_CLASSES = ["rb_finest", "rb_coarse", "defect", "restrict", "prolong",
                    "tail", "operator", "update_xr", "true_residual",
                    "mg_entry_exit", "project", "update_p", "finalize",
-                   "other"]
+                   "other", "sweep_finest", "sweep_coarse"]
 
 ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",

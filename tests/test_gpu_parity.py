@@ -40,7 +40,20 @@ CASES = {
     "step": lambda: hf_cases.flame_step(n=(48, 24, 16), seed=9),
     "mixed": lambda: mixed_case(),
     "odd_periodic": lambda: mixed_case(n=(20, 12, 18)),
+    # fused-sweep levels (x >= 32, y >= 8) with partial 128 x 8 tiles
+    "mixed_wide": lambda: mixed_case(n=(72, 40, 20), seed=12),
+    # periodic x and y exercise the wrapped halo ring of the fused sweep
+    "periodic_xy": lambda: periodic_xy_case(),
 }
+
+
+def periodic_xy_case(n=(64, 48, 20), seed=13):
+    case = hf_cases.lognormal_sphere(n=n, seed=seed, radius=0.2)
+    bc = [2, 2, 2, 2, 0, 0]
+    act = case["cell_active"].reshape(n[2], n[1], n[0])
+    fx, fy, fz = hf_cases.face_activity(act, bc)
+    case.update(bc=bc, x_active=fx, y_active=fy, z_active=fz)
+    return case
 
 
 def _policy(case_name, **kw):
@@ -48,7 +61,8 @@ def _policy(case_name, **kw):
     # threshold above the spacing ratio keeps point red-black + full
     # coarsening on both sides
     thr = 16.0 if case_name == "stretched24" else 4.0
-    null = 0 if case_name in ("mixed", "odd_periodic") else 1
+    null = 0 if case_name in ("mixed", "odd_periodic", "mixed_wide",
+                              "periodic_xy") else 1
     base = dict(anisotropy_threshold=thr, null_space_constant=null)
     base.update(kw)
     return base
@@ -124,20 +138,23 @@ def test_operator(torch_cuda, name):
     plan.close()
 
 
-@pytest.mark.parametrize("name", ["mixed", "odd_periodic"])
+@pytest.mark.parametrize("name", ["mixed", "odd_periodic", "mixed_wide",
+                                  "periodic_xy"])
 @pytest.mark.parametrize("f_cycle", [0, 1])
 @pytest.mark.parametrize("tail", ["default", "0"])
 @pytest.mark.parametrize("scaling", [0, 1])
+@pytest.mark.parametrize("sweep", ["0", "1"])
 def test_mg_apply_bitwise(torch_cuda, monkeypatch, name, f_cycle, tail,
-                          scaling):
+                          scaling, sweep):
     """Without a null-space projection every V/F-cycle step is a fixed-order
     FP64 computation, so the GPU cycle equals the oracle bit for bit, both
     through the single-workgroup tail and through grid-level kernels."""
     torch = torch_cuda
     _say(f"[mg bitwise] case={name} f_cycle={f_cycle} tail={tail} "
-         f"scaling={scaling}")
+         f"scaling={scaling} sweep={sweep}")
     if tail != "default":
         monkeypatch.setenv("HF_TAIL_CELLS", tail)
+    monkeypatch.setenv("HF_FUSED_SWEEP", sweep)
     case = CASES[name]()
     pol = _policy(name, f_cycle=f_cycle, coarse_face_scaling=scaling)
     plan = _gpu_plan(torch, case, **pol)
@@ -152,13 +169,17 @@ def test_mg_apply_bitwise(torch_cuda, monkeypatch, name, f_cycle, tail,
 @pytest.mark.parametrize("name", ["sphere32", "step", "stretched24"])
 @pytest.mark.parametrize("unit_linear", [1, 0])
 @pytest.mark.parametrize("tail", ["default", "0"])
-def test_mg_apply_projected(torch_cuda, monkeypatch, name, unit_linear, tail):
+@pytest.mark.parametrize("sweep", ["1", "0"])
+def test_mg_apply_projected(torch_cuda, monkeypatch, name, unit_linear, tail,
+                            sweep):
     """With the constant null space the projection means come from a
     parallel reduction: agreement to 1e-12 relative."""
     torch = torch_cuda
-    _say(f"[mg projected] case={name} unit_linear={unit_linear} tail={tail}")
+    _say(f"[mg projected] case={name} unit_linear={unit_linear} tail={tail} "
+         f"sweep={sweep}")
     if tail != "default":
         monkeypatch.setenv("HF_TAIL_CELLS", tail)
+    monkeypatch.setenv("HF_FUSED_SWEEP", sweep)
     case = CASES[name]()
     pol = _policy(name, unit_linear=unit_linear)
     plan = _gpu_plan(torch, case, **pol)
