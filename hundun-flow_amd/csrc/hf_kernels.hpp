@@ -368,6 +368,7 @@ enum ScalarSlot : int {
   S_MGALPHA = 51,
   S_SUM = 52,
   S_SUM2 = 53,
+  S_LINE_FAIL = 54,  // nonzero after a line relaxation pivot vanished
   S_COUNT = 64
 };
 

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "hf_kernels.hpp"
+#include "hf_line.hpp"
 #include "hundun_flow.h"
 
 namespace hf {
@@ -38,7 +39,8 @@ enum ProfileClass : int {
   P_OTHER = 13,
   P_SWEEP_FINEST = 14,  // fused two-colour sweep on level 0
   P_SWEEP_COARSE = 15,  // fused two-colour sweep on grid levels >= 1
-  kProfileClasses = 16
+  P_LINE = 16,          // line relaxation (any level)
+  kProfileClasses = 17
 };
 constexpr uint32_t kDetailPlan = 9101U;
 constexpr uint32_t kDetailAlloc = 9102U;
@@ -47,6 +49,7 @@ constexpr uint32_t kDetailNumeric = 9104U;
 constexpr uint32_t kDetailCoefficients = 9105U;
 constexpr uint32_t kDetailBreakdown = 9106U;
 constexpr uint32_t kDetailNonFinite = 9107U;
+constexpr uint32_t kDetailLine = 9108U;  // line relaxation pivot vanished
 
 hf_status ok() { return {HF_OK, 0U}; }
 hf_status fail(int code, uint32_t detail) { return {code, detail}; }
@@ -83,6 +86,11 @@ struct Level {
   void* r = nullptr;
   void* x2 = nullptr;
   void* b2 = nullptr;
+  // line relaxation scratch: Thomas c' and d' (compute precision) and the
+  // periodic ghost planes of the widest line axis
+  void* lcw = nullptr;
+  void* lcd = nullptr;
+  void* lghost = nullptr;
   bool sweep = false;  // fused two-colour sweep kernel
   int sweep_zc = 0;    // planes per block of the sweep kernel
   int sweep_gx = 0, sweep_gy = 0, sweep_blocks = 0;
@@ -497,12 +505,22 @@ void finalize(hf_plan* p, int64_t blocks, int nacc, int op, int slot,
   ++p->reductions;
 }
 
+// Copies the scalar block to the host.  A line relaxation failure since the
+// last read is reported (and cleared) as a numerical failure, the deferred
+// status of src/solver_mg.cpp:3522-3524.
 hf_status read_scalars(hf_plan* p) {
   if (hipMemcpyAsync(p->host_scalars, p->scalars, S_COUNT * sizeof(double),
                      hipMemcpyDeviceToHost, p->stream) != hipSuccess)
     return fail(HF_INVALID_PLAN, kDetailHip);
   if (hipStreamSynchronize(p->stream) != hipSuccess)
     return fail(HF_INVALID_PLAN, kDetailHip);
+  if (p->host_scalars[S_LINE_FAIL] != 0.0) {
+    p->host_scalars[S_LINE_FAIL] = 0.0;
+    if (hipMemcpy(p->scalars + S_LINE_FAIL, p->host_scalars + S_LINE_FAIL,
+                  sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(HF_INVALID_PLAN, kDetailHip);
+    return fail(HF_NUMERICAL_FAILURE, kDetailLine);
+  }
   return ok();
 }
 
@@ -572,8 +590,88 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
 }
 
 template <class T>
+void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
+                  int from_zero);
+
+// Line geometry of axis ax on level L: lines (a, b), E cells each.
+template <class T>
+LineArgs<T> line_args(hf_plan* p, Level& L, int ax) {
+  using C = typename Compute<T>::type;
+  LineArgs<T> g{};
+  g.L = view<T>(L);
+  g.bc = p->bc;
+  g.ax = ax;
+  g.A = ax == 0 ? L.n[1] : L.n[0];
+  g.B = ax == 2 ? L.n[1] : L.n[2];
+  g.E = L.n[ax];
+  g.cw = static_cast<C*>(L.lcw);
+  g.cd = static_cast<C*>(L.lcd);
+  const int a_axis = ax == 0 ? 1 : 0;
+  const int b_axis = ax == 2 ? 1 : 2;
+  T* ghost = static_cast<T*>(L.lghost);
+  g.ghost_a = (p->periodic_mask >> a_axis) & 1 ? ghost : nullptr;
+  g.ghost_b = (p->periodic_mask >> b_axis) & 1
+                  ? ghost + (g.ghost_a != nullptr ? 2 * g.B * g.E : 0)
+                  : nullptr;
+  g.flag = p->scalars + S_LINE_FAIL;
+  return g;
+}
+
+// smooth() on a line level (src/solver_mg.cpp:3540-3590): each line axis in
+// x, y, z order (reverse: z, y, x), all sweeps of one axis before the next.
+template <class T>
+void line_smooth(hf_plan* p, int l, int sweeps, int reverse) {
+  Level& L = p->levels[l];
+  for (int i = 0; i < 3; ++i) {
+    const int ax = reverse ? 2 - i : i;
+    if ((L.line_mask & (1 << ax)) == 0) continue;
+    if (L.n[ax] > kLineMaximumExtent) {
+      point_smooth<T>(p, l, sweeps, reverse, 0, 0);
+      continue;
+    }
+    const LineArgs<T> g = line_args<T>(p, L, ax);
+    HF_PROF(p, P_LINE);
+    if (p->policy.line_order == 0) {
+      HF_LAUNCH(k_line_lex<T>, 1, 1024, 0, p->stream, g, sweeps, reverse);
+      continue;
+    }
+    const int ghosts = (g.ghost_a != nullptr ? 2 * g.B * g.E : 0) +
+                       (g.ghost_b != nullptr ? 2 * g.A * g.E : 0);
+    const int lines = (g.A + 1) / 2 * g.B;
+    for (int sweep = 0; sweep < sweeps; ++sweep) {
+      for (int pass = 0; pass < 2; ++pass) {
+        const int colour = ((reverse ? 1 : 0) + pass) & 1;
+        if (ghosts > 0)
+          HF_LAUNCH(k_line_ghost<T>, blocks_for(ghosts), kBlock, 0, p->stream,
+                    g);
+        HF_LAUNCH(k_line_zebra<T>, blocks_for(lines), kBlock, 0, p->stream, g,
+                  colour);
+      }
+    }
+  }
+}
+
+template <class T>
 void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
                  int from_zero) {
+  Level& L = p->levels[l];
+  if (L.line_mask != 0) {
+    // x starts from an explicit zero on line levels; the retained defect is
+    // recomputed after line smoothing (compute_residual, :4256-4262)
+    line_smooth<T>(p, l, sweeps, reverse);
+    if (retain) {
+      HF_PROF(p, P_DEFECT);
+      HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
+                view<T>(L), p->bc, 1);
+    }
+    return;
+  }
+  point_smooth<T>(p, l, sweeps, reverse, retain, from_zero);
+}
+
+template <class T>
+void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
+                  int from_zero) {
   Level& L = p->levels[l];
   if (L.sweep) {
     sweep_smooth<T>(p, l, sweeps, reverse, retain, from_zero);
@@ -631,9 +729,10 @@ void exec_grid_op(hf_plan* p, const CycleOp& op) {
   Level& Cn = p->levels[op.level + 1];
   if (op.kind == T_RESTRICT) {
     const bool proj = p->policy.null_space_constant != 0;
-    // a grid-level coarse smoothing starts from zero and applies the
-    // projection itself; the tail needs x = 0 and a projected b up front
-    const bool into_tail = Cn.tail;
+    // a point-smoothed grid level starts from zero and applies the
+    // projection in its first pass; the tail and line levels need x = 0 and
+    // a projected b up front
+    const bool into_tail = Cn.tail || Cn.line_mask != 0;
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       dispatch(Cn.prec, [&](auto tc) {
@@ -684,7 +783,9 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
                    bool dots) {
   Level& L0 = p->levels[0];
   const bool proj = p->policy.null_space_constant != 0;
-  const bool tail0 = L0.tail;  // whole cycle inside the tail kernel
+  // whole cycle inside the tail kernel, or a line level 0: x = 0 and the
+  // projected b are stored before the cycle
+  const bool tail0 = L0.tail || L0.line_mask != 0;
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
@@ -801,6 +902,7 @@ hf_status hf_policy_default(hf_policy* policy) {
   policy->unit_linear = 1;
   policy->precision = HF_PRECISION_FP64;
   policy->coarse_face_scaling = 0;
+  policy->line_order = 0;
   return ok();
 }
 
@@ -822,7 +924,7 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       policy->coarse_sweeps == 0 || policy->maximum_levels < 2 ||
       policy->maximum_levels > kMaxLevels || policy->minimum_coarse_extent < 2 ||
       !(policy->anisotropy_threshold > 1.0) || policy->precision > 2 ||
-      policy->coarse_face_scaling > 1)
+      policy->coarse_face_scaling > 1 || policy->line_order > 1)
     return fail(HF_INVALID_PLAN, kDetailPlan);
   hf_plan* p = new (std::nothrow) hf_plan();
   if (p == nullptr) return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
@@ -854,17 +956,20 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
                      (grid->boundary[2] == HF_BC_PERIODIC ? 2 : 0) |
                      (grid->boundary[4] == HF_BC_PERIODIC ? 4 : 0);
   plan_levels(p);
-  if (p->levels.size() < 2 || p->levels[0].line_mask != 0) {
-    // line relaxation levels are not provided by this build
+  if (p->levels.size() < 2) {
     delete p;
     return fail(HF_INVALID_PLAN, kDetailPlan);
   }
+  const bool line_plan = p->levels[0].line_mask != 0;
   // the tail is the longest run of coarsest levels whose x, b, r fit the
   // tail kernel's LDS and each stay under the cell threshold
   const int64_t threshold = tail_threshold();
   p->tail_first = static_cast<int>(p->levels.size());
   int64_t lds_doubles = 0;
-  for (int l = static_cast<int>(p->levels.size()) - 1; l >= 0; --l) {
+  // line levels (every level of a line plan) are smoothed by the line
+  // kernels, never in the tail
+  for (int l = static_cast<int>(p->levels.size()) - 1; l >= 0 && !line_plan;
+       --l) {
     const int64_t need = lds_doubles + 3 * p->levels[l].cells;
     if (p->levels[l].cells <= threshold &&
         need * static_cast<int64_t>(sizeof(double)) <= kTailLdsBytes) {
@@ -874,22 +979,24 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       break;
     }
   }
-  // the coarsest level is always FP64 and handled by the tail kernel
-  if (p->tail_first > static_cast<int>(p->levels.size()) - 1)
+  // the coarsest level is always FP64 and, on point plans, handled by the
+  // tail kernel
+  if (!line_plan && p->tail_first > static_cast<int>(p->levels.size()) - 1)
     p->tail_first = static_cast<int>(p->levels.size()) - 1;
   int64_t tail_doubles = 0;
   for (size_t l = 0; l < p->levels.size(); ++l) {
     Level& L = p->levels[l];
     L.tail = static_cast<int>(l) >= p->tail_first;
     if (L.tail) tail_doubles += 3 * L.cells;
-    L.prec = L.tail ? HF_PRECISION_FP64 : policy->precision;
+    const bool coarsest = l + 1 == p->levels.size();
+    L.prec = (L.tail || coarsest) ? HF_PRECISION_FP64 : policy->precision;
     L.tc = make_tiles(L.n[0], L.n[1], L.n[2], false);
     L.tcol = make_tiles((L.n[0] + 1) / 2, L.n[1], L.n[2], false);
     L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
     // fused sweeps on grid levels with the fused colour route and rows long
     // enough for the 128 x 8 tiles to stay mostly busy
-    L.sweep = !L.tail && L.fused && L.n[0] >= 32 && L.n[1] >= 8 &&
-              sweep_enabled();
+    L.sweep = !L.tail && L.line_mask == 0 && L.fused && L.n[0] >= 32 &&
+              L.n[1] >= 8 && sweep_enabled();
     if (L.sweep) {
       L.sweep_gx = (L.n[0] + kSwTX - 1) / kSwTX;
       L.sweep_gy = (L.n[1] + kSwTY - 1) / kSwTY;
@@ -942,6 +1049,19 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       guard(dalloc_raw(p, &L.x2, es * L.cells));
       guard(dalloc_raw(p, &L.b2, es * L.cells));
     }
+    if (L.line_mask != 0) {
+      const size_t cs = L.prec == HF_PRECISION_FP64 ? 8 : 4;
+      guard(dalloc_raw(p, &L.lcw, cs * L.cells));
+      guard(dalloc_raw(p, &L.lcd, cs * L.cells));
+      int64_t ghosts = 1;
+      for (int ax = 0; ax < 3; ++ax) {
+        if ((L.line_mask & (1 << ax)) == 0) continue;
+        const int64_t A = ax == 0 ? L.n[1] : L.n[0];
+        const int64_t B = ax == 2 ? L.n[1] : L.n[2];
+        ghosts = std::max<int64_t>(ghosts, 2 * (A + B) * L.n[ax]);
+      }
+      guard(dalloc_raw(p, &L.lghost, es * ghosts));
+    }
   }
   // one partial per tile (level 0 has the most) or per 256-element block
   int64_t max_blocks = blocks_for(p->cells);
@@ -974,6 +1094,9 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
                          p->stream) != hipSuccess)
         s = fail(HF_INVALID_PLAN, kDetailHip);
     }
+    if (s.code == HF_OK)
+      s = hip_check(hipMemsetAsync(p->scalars, 0, S_COUNT * sizeof(double),
+                                   p->stream));
     if (s.code == HF_OK) s = hip_check(hipStreamSynchronize(p->stream));
   }
   if (s.code != HF_OK) {
@@ -1134,7 +1257,10 @@ hf_status hf_mg_apply(hf_plan* p, const double* r, double* z) {
   if (p == nullptr || r == nullptr || z == nullptr || !p->coefficients_ready)
     return fail(HF_INVALID_PLAN, kDetailPlan);
   HF_TRY(mg_apply(p, r, z, nullptr, false));
-  return hip_check(hipGetLastError());
+  HF_TRY(hip_check(hipGetLastError()));
+  // line relaxation reports a vanished pivot through the scalar block
+  if (p->levels[0].line_mask != 0) return read_scalars(p);
+  return ok();
 }
 
 static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
@@ -1282,7 +1408,13 @@ hf_status hf_pcg_solve(hf_plan* p, const double* rhs, double* solution,
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   (void)hipEventRecord(e0, p->stream);
-  const hf_status st = pcg_impl(p, rhs, solution, ctl, res);
+  hf_status st = pcg_impl(p, rhs, solution, ctl, res);
+  if (st.code != HF_OK && res->termination == HF_TERM_CONVERGED) {
+    // a failure raised outside the Krylov gates (device error, line pivot)
+    res->status = st;
+    res->termination = st.detail == kDetailLine ? HF_TERM_PRECONDITIONER_FAILURE
+                                                : HF_TERM_INVALID_PLAN;
+  }
   (void)hipEventRecord(e1, p->stream);
   (void)hipEventSynchronize(e1);
   profile_collect(p);

@@ -31,7 +31,7 @@ PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 PROFILE_CLASSES = ["rb_finest", "rb_coarse", "defect", "restrict", "prolong",
                    "tail", "operator", "update_xr", "true_residual",
                    "mg_entry_exit", "project", "update_p", "finalize",
-                   "other", "sweep_finest", "sweep_coarse"]
+                   "other", "sweep_finest", "sweep_coarse", "line"]
 
 ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",
@@ -67,7 +67,8 @@ class hf_policy(ctypes.Structure):
                 ("null_space_constant", ctypes.c_uint8),
                 ("unit_linear", ctypes.c_uint8),
                 ("precision", ctypes.c_uint8),
-                ("coarse_face_scaling", ctypes.c_uint8)]
+                ("coarse_face_scaling", ctypes.c_uint8),
+                ("line_order", ctypes.c_uint8)]
 
 
 class hf_solve_control(ctypes.Structure):

@@ -109,6 +109,14 @@ typedef struct hf_policy {
    *    ratio of fine to coarse centre distance, which makes the coarse rows a
    *    finite-volume rediscretisation and the V-cycle grid independent. */
   uint8_t coarse_face_scaling;
+  /* Line relaxation order on anisotropic levels (src/solver_mg.cpp:
+   * 3352-3536):
+   * 0: lexicographic over lines, the reference's order, bit for bit (one
+   *    workgroup walks the line hyperplanes: exact but latency bound);
+   * 1: zebra (not in the reference): lines with (a+b) even, then odd
+   *    (reverse sweeps odd, even), each colour one parallel pass; periodic
+   *    ghosts are refreshed per colour pass. */
+  uint8_t line_order;
 } hf_policy;
 
 /* LinearSolveControl (v04_linear.hpp:454-466) restricted to PCG. */

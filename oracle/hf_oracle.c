@@ -43,6 +43,7 @@ struct or_mg {
   int levels;
   or_level level[OR_MAX_LEVELS];
   uint8_t* cell_active; /* copy, level-0 cells (NULL = all active) */
+  int line_failed;      /* a Thomas pivot vanished during the last apply */
 };
 
 static size_t cidx(const int n[3], int i, int j, int k) {
@@ -460,8 +461,7 @@ or_mg* or_mg_create(const int n[3], const double* wx, const double* wy,
     if (mask == 0 || mg->levels == policy->maximum_levels) break;
     memcpy(g, next, sizeof g);
   }
-  if (mg->levels < 2 || line_mask != 0) {
-    /* Line relaxation levels are not restated by this oracle. */
+  if (mg->levels < 2) {
     free(mg);
     return NULL;
   }
@@ -615,9 +615,160 @@ static void point_smooth(or_mg* mg, int level, int sweeps, int reverse,
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* Line relaxation, src/solver_mg.cpp:3352-3536 (line_smooth_axis) and  */
+/* :3540-3590 (line_smooth, smooth).                                    */
+/* ------------------------------------------------------------------ */
+/* MgHierarchyPolicy::line_relaxation_maximum_extent default
+ * (include/hundun/v04_linear.hpp:123): longer lines use the point smoother. */
+#define OR_LINE_MAXIMUM_EXTENT 4096
+
+/* Cell m of line (a, b) along axis ax: x-lines run over (y=a, z=b), y-lines
+ * over (x=a, z=b), z-lines over (x=a, y=b). */
+static void line_cell(int ax, int a, int b, int m, int c[3]) {
+  if (ax == 0) {
+    c[0] = m; c[1] = a; c[2] = b;
+  } else if (ax == 1) {
+    c[0] = a; c[1] = m; c[2] = b;
+  } else {
+    c[0] = a; c[1] = b; c[2] = m;
+  }
+}
+
+static double face_low(const or_level* L, int axis, const int c[3]) {
+  if (axis == 0) return L->fx[xfidx(L->n, c[0], c[1], c[2])];
+  if (axis == 1) return L->fy[yfidx(L->n, c[0], c[1], c[2])];
+  return L->fz[zfidx(L->n, c[0], c[1], c[2])];
+}
+
+static double face_high(const or_level* L, int axis, const int c[3]) {
+  if (axis == 0) return L->fx[xfidx(L->n, c[0] + 1, c[1], c[2])];
+  if (axis == 1) return L->fy[yfidx(L->n, c[0], c[1] + 1, c[2])];
+  return L->fz[zfidx(L->n, c[0], c[1], c[2] + 1)];
+}
+
+/* level_neighbor (src/solver_mg.cpp:2229-2265) on one rank: in-range reads
+ * the current iterate, a periodic wrap reads the ghost copy made when the
+ * sweep (zebra: the colour pass) began, Neumann clamps to the cell itself,
+ * Dirichlet is 0. */
+static double line_neighbour(const or_mg* mg, const or_level* L,
+                             const double* ghost, const int c[3], int axis,
+                             int dir) {
+  int s[3] = {c[0], c[1], c[2]};
+  s[axis] += dir;
+  if (s[axis] >= 0 && s[axis] < L->n[axis])
+    return L->x[cidx(L->n, s[0], s[1], s[2])];
+  const int kind = mg->bc[2 * axis + (dir > 0 ? 1 : 0)];
+  if (kind == OR_DIRICHLET) return 0.0;
+  if (kind == OR_NEUMANN) return L->x[cidx(L->n, c[0], c[1], c[2])];
+  s[axis] = dir < 0 ? L->n[axis] - 1 : 0;
+  return ghost[cidx(L->n, s[0], s[1], s[2])];
+}
+
+/* One Thomas solve (solve_one, src/solver_mg.cpp:3390-3502): c' in the
+ * temporary slot, d' in the residual slot, then back substitution into x.
+ * Returns 0 (x of the line untouched) when a pivot is not above DBL_MIN. */
+static int solve_line(const or_mg* mg, or_level* L, const double* ghost,
+                      int ax, int a, int b) {
+  const int E = L->n[ax];
+  int c[3], p[3];
+  for (int m = 0; m < E; ++m) {
+    line_cell(ax, a, b, m, c);
+    const size_t ci = cidx(L->n, c[0], c[1], c[2]);
+    const double lo = face_low(L, ax, c);
+    const double hi = face_high(L, ax, c);
+    const double sub = m == 0 ? 0.0 : -lo;
+    const double dia = L->D[ci];
+    const double sup = m + 1 == E ? 0.0 : -hi;
+    double d = L->b[ci];
+    for (int t = 0; t < 3; ++t) {
+      if (t == ax) continue;
+      d += face_low(L, t, c) * line_neighbour(mg, L, ghost, c, t, -1);
+      d += face_high(L, t, c) * line_neighbour(mg, L, ghost, c, t, 1);
+    }
+    if (m == 0) d += lo * line_neighbour(mg, L, ghost, c, ax, -1);
+    if (m + 1 == E) d += hi * line_neighbour(mg, L, ghost, c, ax, 1);
+    if (m > 0) {
+      line_cell(ax, a, b, m - 1, p);
+      const size_t pi = cidx(L->n, p[0], p[1], p[2]);
+      const double den = dia - sub * L->t[pi];
+      if (!(fabs(den) > DBL_MIN)) return 0;
+      L->t[ci] = sup / den;
+      L->r[ci] = (d - sub * L->r[pi]) / den;
+    } else {
+      if (!(fabs(dia) > DBL_MIN)) return 0;
+      L->t[ci] = sup / dia;
+      L->r[ci] = d / dia;
+    }
+  }
+  for (int m = E; m-- > 0;) {
+    line_cell(ax, a, b, m, c);
+    const size_t ci = cidx(L->n, c[0], c[1], c[2]);
+    if (m + 1 == E) {
+      L->x[ci] = L->r[ci];
+    } else {
+      line_cell(ax, a, b, m + 1, p);
+      L->x[ci] = L->r[ci] - L->t[ci] * L->x[cidx(L->n, p[0], p[1], p[2])];
+    }
+  }
+  return 1;
+}
+
+
+static void line_smooth_axis(or_mg* mg, int level, int ax, int sweeps,
+                             int reverse) {
+  or_level* L = &mg->level[level];
+  const int E = L->n[ax];
+  if (E < 1 || E > OR_LINE_MAXIMUM_EXTENT) {
+    point_smooth(mg, level, sweeps, reverse, 0);
+    return;
+  }
+  const int A = ax == 0 ? L->n[1] : L->n[0];
+  const int B = ax == 2 ? L->n[1] : L->n[2];
+  double* ghost = (double*)malloc(L->cells * sizeof(double));
+  int ok = 1;
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    if (mg->policy.line_order == 0) {
+      memcpy(ghost, L->x, L->cells * sizeof(double));
+      if (reverse) {
+        for (int b = B; b-- > 0;)
+          for (int a = A; a-- > 0;) ok = solve_line(mg, L, ghost, ax, a, b) && ok;
+      } else {
+        for (int b = 0; b < B; ++b)
+          for (int a = 0; a < A; ++a) ok = solve_line(mg, L, ghost, ax, a, b) && ok;
+      }
+    } else {
+      for (int pass = 0; pass < 2; ++pass) {
+        const int colour = ((reverse ? 1 : 0) + pass) & 1;
+        memcpy(ghost, L->x, L->cells * sizeof(double));
+        for (int b = 0; b < B; ++b)
+          for (int a = 0; a < A; ++a)
+            if (((a + b) & 1) == colour)
+              ok = solve_line(mg, L, ghost, ax, a, b) && ok;
+      }
+    }
+  }
+  free(ghost);
+  if (!ok) mg->line_failed = 1;
+}
+
+/* smooth(), src/solver_mg.cpp:3557-3590: point levels use red-black, line
+ * levels relax each line axis in x, y, z order (reverse: z, y, x). */
+static void smooth(or_mg* mg, int level, int sweeps, int reverse, int retain) {
+  const int mask = mg->level[level].line_mask;
+  if (mask == 0) {
+    point_smooth(mg, level, sweeps, reverse, retain);
+    return;
+  }
+  for (int i = 0; i < 3; ++i) {
+    const int ax = reverse ? 2 - i : i;
+    if (mask & (1 << ax)) line_smooth_axis(mg, level, ax, sweeps, reverse);
+  }
+}
+
 int or_mg_smooth(or_mg* mg, int level, int sweeps, int reverse, int retain) {
   if (level < 0 || level >= mg->levels) return ST_INVALID_PLAN;
-  point_smooth(mg, level, sweeps, reverse, retain);
+  smooth(mg, level, sweeps, reverse, retain);
   return ST_OK;
 }
 
@@ -745,10 +896,14 @@ int or_mg_prolongate(or_mg* mg, int fine) {
 /* src/solver_mg.cpp:4192-4338 */
 static int mg_cycle(or_mg* mg, int level, int counter) {
   if (level + 1 == mg->levels) {
-    point_smooth(mg, level, mg->policy.coarse_sweeps, 0, 0);
+    smooth(mg, level, mg->policy.coarse_sweeps, 0, 0);
     return ST_OK;
   }
-  point_smooth(mg, level, mg->policy.pre_sweeps, 0, 1);
+  smooth(mg, level, mg->policy.pre_sweeps, 0, 1);
+  /* line levels recompute the defect (compute_residual, :4256-4262) */
+  if (mg->level[level].line_mask != 0)
+    level_defect(mg, &mg->level[level], mg->level[level].x,
+                 mg->level[level].b, mg->level[level].r);
   int st = or_mg_restrict(mg, level);
   if (st != ST_OK) return st;
   st = mg_cycle(mg, level + 1, counter);
@@ -758,7 +913,7 @@ static int mg_cycle(or_mg* mg, int level, int counter) {
     if (st != ST_OK) return st;
   }
   or_mg_prolongate(mg, level);
-  point_smooth(mg, level, mg->policy.post_sweeps, 1, 0);
+  smooth(mg, level, mg->policy.post_sweeps, 1, 0);
   return ST_OK;
 }
 
@@ -775,8 +930,10 @@ int or_mg_apply(or_mg* mg, const double* residual, double* correction) {
     st = project_constant(L, L->b);
     if (st != ST_OK) return st;
   }
+  mg->line_failed = 0;
   st = mg_cycle(mg, 0, mg->policy.f_cycle ? 2 : 1);
   if (st != ST_OK) return st;
+  if (mg->line_failed) return ST_NUMERICAL;
   if (mg->policy.null_space_constant) {
     st = project_constant(L, L->x);
     if (st != ST_OK) return st;

@@ -42,6 +42,10 @@ typedef struct or_policy {
                                    faces); 1 distance-corrected: a coarse face
                                    whose normal axis was coarsened is scaled
                                    by 1/2 (the coarse centre distance doubles) */
+  int line_order;               /* line relaxation order: 0 lexicographic over
+                                   lines (the reference); 1 zebra: lines with
+                                   (a+b) even, then odd (reverse: odd, even),
+                                   periodic ghosts refreshed per colour pass */
 } or_policy;
 
 typedef struct or_control {

@@ -23,7 +23,8 @@ class or_policy(ctypes.Structure):
                 ("f_cycle", ctypes.c_int),
                 ("null_space_constant", ctypes.c_int),
                 ("unit_linear", ctypes.c_int),
-                ("coarse_face_scaling", ctypes.c_int)]
+                ("coarse_face_scaling", ctypes.c_int),
+                ("line_order", ctypes.c_int)]
 
 
 class or_control(ctypes.Structure):
@@ -145,7 +146,7 @@ def policy(**kw) -> or_policy:
     p = or_policy(anisotropy_threshold=4.0, pre_sweeps=2, post_sweeps=2,
                   maximum_levels=32, coarse_sweeps=24,
                   minimum_coarse_extent=3, f_cycle=0, null_space_constant=1,
-                  unit_linear=1, coarse_face_scaling=0)
+                  unit_linear=1, coarse_face_scaling=0, line_order=0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -212,6 +213,11 @@ class MG:
                   nx * ny * nz, nx * ny * nz]
         ptr = lib().or_mg_level_array(self.h, level, which)
         return np.ctypeslib.as_array(ptr, shape=(counts[which],)).copy()
+
+    def smooth(self, level, sweeps, reverse=0, retain=0):
+        st = lib().or_mg_smooth(self.h, level, sweeps, reverse, retain)
+        if st != 0:
+            raise RuntimeError(f"or_mg_smooth status {st}")
 
     def slot(self, level, which):
         nx, ny, nz = self.level_info(level)["shape"]

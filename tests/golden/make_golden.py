@@ -38,6 +38,29 @@ def with_bc(case, bc):
     return case
 
 
+def line_case(n, seed, stretch_axis, bc, ratio=8.0, depth=1.0):
+    """Lognormal rho with a sphere mask on a grid tanh-stretched along one
+    axis (line relaxation under the default anisotropy threshold); `depth`
+    scales the z extent (a deep z makes x and y both strong)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    faces = []
+    for a in range(3):
+        f = (hf_cases.tanh_faces(n[a], ratio) if a == stretch_axis
+             else hf_cases.uniform_faces(n[a]))
+        if a == 2:
+            f = hf_cases.real4(f * depth)
+        faces.append(f)
+    nx, ny, nz = n
+    xi = rng.standard_normal((nz, ny, nx))
+    rho = np.exp(hf_cases.box_filter(0.3 * xi, (False, False, False), 2))
+    X, Y, Z = hf_cases._grid(faces)
+    zc = 0.5 * depth
+    solid = ((X - 0.45) ** 2 + (Y - 0.55) ** 2 + ((Z - zc) / depth) ** 2
+             < 0.2 ** 2)
+    rhs = rng.standard_normal((nz, ny, nx))
+    return hf_cases._pack(n, faces, bc, rho, ~solid, rhs)
+
+
 def cases():
     mixed = [0, 0, 1, 1, 2, 2]
     yield "mixed_sphere", with_bc(
@@ -55,6 +78,19 @@ def cases():
     yield "neumann_all", with_bc(
         hf_cases.lognormal_sphere(n=(12, 10, 14), seed=26, radius=0.3),
         [1, 1, 1, 1, 1, 1]), dict(coarse_sweeps=6)
+    # line relaxation (default anisotropy threshold 4)
+    yield "line_y_cylinder", hf_cases.flame_cylinder(
+        n=(12, 16, 8), seed=27), dict()
+    yield "line_y_periodic", with_bc(hf_cases.flame_cylinder(
+        n=(11, 16, 8), seed=28), [2, 2, 2, 2, 1, 1]), dict(coarse_sweeps=8)
+    yield "line_x_dirichlet", line_case(
+        (16, 10, 9), 29, 0, [0, 0, 1, 1, 2, 2]), dict(null_space_constant=0)
+    yield "line_z_fcycle", line_case(
+        (10, 9, 16), 30, 2, [1, 1, 0, 1, 1, 1]), dict(
+            null_space_constant=0, f_cycle=1, unit_linear=0)
+    yield "line_xy_deep", line_case(
+        (12, 10, 6), 31, -1, [1, 1, 2, 2, 1, 1], depth=6.0), dict(
+            coarse_sweeps=6)
 
 
 def write_case(path, case, pol, r, b, rtol, maxit, interval):

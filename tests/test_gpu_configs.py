@@ -1,0 +1,192 @@
+"""Full-size workloads on the GPU, checked by size-independent properties.
+
+At these sizes the CPU oracle is too slow for a full solve, so each GPU
+solution is verified with the oracle's own FP64 IBM operator (an independent
+code path): the true relative residual ||b - A p|| / ||b|| must meet the
+solve tolerance, solid cells must be exactly zero, and the mixed-precision
+solutions must agree with the FP64 solution to the conditioning bound.  The
+64^3 case is also compared with a full oracle solve.
+"""
+import functools
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import hf_cases
+import hf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@functools.lru_cache(maxsize=2)
+def cached_case(name, shape, seed):
+    return hf_cases.make(name, n=shape, seed=seed)
+
+
+def _solve(case, **policy):
+    import torch
+
+    import hundun_flow as H
+    pol = H.default_policy(**policy)
+    plan = H.plan_from_case(case, pol)
+    b = torch.from_numpy(case["rhs"]).cuda()
+    flexible = policy.get("precision", "fp64") != "fp64" or \
+        policy.get("coarse_face_scaling", 0) == 1
+    t0 = time.perf_counter()
+    x, res = plan.solve_pcg(b, rtol=1e-10, maximum_iterations=1000,
+                            flexible=flexible)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    xh = x.cpu().numpy()
+    plan.close()
+    return xh, res, ms
+
+
+def _oracle_residual(case, x):
+    d, bx, by, bz = O.assemble(case["n"], case["faces"], case["bc"],
+                               case["mobility"])
+    ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz,
+                          case["cell_active"], x)
+    return np.linalg.norm(case["rhs"] - ax) / np.linalg.norm(case["rhs"])
+
+
+def _gauge(x, act):
+    fluid = act != 0
+    y = x.copy()
+    y[fluid] -= y[fluid].mean()
+    y[~fluid] = 0.0
+    return y
+
+
+def test_lognormal_sphere_64_matches_oracle_solve():
+    """64^3 lognormal rho, sphere mask, Neumann walls, periodic z: the GPU
+    FP64 solve with the reference hierarchy takes the oracle's iterations and
+    reaches the same solution."""
+    case = hf_cases.lognormal_sphere(n=(64, 64, 64), seed=1)
+    x, res, ms = _solve(case)
+    (d, bx, by, bz), mg = O.from_case(case, O.policy())
+    xo, ores = O.pcg(case["n"], case["bc"], d, bx, by, bz,
+                     case["cell_active"], mg, case["rhs"], rtol=1e-10)
+    act = case["cell_active"]
+    rel = _oracle_residual(case, x)
+    diff = np.linalg.norm(_gauge(x, act) - _gauge(xo, act)) / \
+        np.linalg.norm(_gauge(xo, act))
+    _say(f"[64^3] gpu it={res.iterations} ({ms:.1f} ms) oracle "
+         f"it={ores.iterations} rel={rel:.2e} diff={diff:.2e}")
+    assert abs(res.iterations - ores.iterations) <= 2
+    assert rel <= 1.01e-10
+    assert diff <= 1e-6
+    assert np.all(x[act == 0] == 0.0)
+
+
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_flame_cylinder_256_stretched(scaling):
+    """256^3, tanh-stretched y (spacing ratio 8), flame sheet 1:7, cylinder
+    mask.  Point red-black with full coarsening on both sides (anisotropy
+    threshold 16 keeps the reference planner off line relaxation)."""
+    case = cached_case("flame_cylinder", (256, 256, 256), 2)
+    x, res, ms = _solve(case, anisotropy_threshold=16.0,
+                        coarse_face_scaling=scaling)
+    rel = _oracle_residual(case, x)
+    _say(f"[256^3 stretched] scaling={scaling} it={res.iterations} "
+         f"{ms:.1f} ms rel={rel:.2e}")
+    assert res.termination == "converged"
+    assert rel <= 1.01e-10
+    assert np.all(x[case["cell_active"] == 0] == 0.0)
+
+
+def test_flame_step_512_precisions():
+    """512x256x256 flame sheet + 5% noise, backward-facing step: FP64, FP32
+    and BF16-storage V-cycles all reach 1e-10 with the FP64 operator; the
+    mixed-precision solutions agree with the FP64 one."""
+    case = hf_cases.flame_step(n=(512, 256, 256), seed=3)
+    act = case["cell_active"]
+    sols = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        x, res, ms = _solve(case, precision=prec, coarse_face_scaling=1)
+        rel = _oracle_residual(case, x)
+        sols[prec] = _gauge(x, act)
+        err = (np.linalg.norm(sols[prec] - sols["fp64"]) /
+               np.linalg.norm(sols["fp64"]))
+        _say(f"[512x256x256] {prec}: it={res.iterations} {ms:.1f} ms "
+             f"rel={rel:.2e} err_vs_fp64={err:.2e}")
+        assert res.termination == "converged"
+        assert rel <= 1.01e-10
+        assert np.all(x[act == 0] == 0.0)
+        assert err <= 1e-6
+
+
+def test_flame_probe_256_adjoint_64_directions():
+    """Differentiation workload at 256^3: gradients of the probe-box pressure
+    sum w.r.t. log rho along 64 seeded directions against central finite
+    differences (h = 1e-6) of tight FP64 solves, and w.r.t. b against the
+    linear response."""
+    import torch
+
+    import hundun_flow as H
+    case = hf_cases.flame_probe(n=(256, 256, 256), seed=4)
+    dev = torch.device("cuda")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    log_rho = t(np.log(case["rho"]))
+    act = case["cell_active"] != 0
+    actf = t(act.astype(np.float64))
+    masks = [t(case[k]) for k in ("cell_active", "x_active", "y_active",
+                                  "z_active")]
+    probe = t(case["probe"])
+    b0 = t(case["rhs"])
+    pol = H.default_policy(coarse_face_scaling=1)
+    ctl = dict(rtol=1e-13, maximum_iterations=2000, flexible=True)
+
+    def forward(lr, b):
+        plan = H.Plan(case["n"], case["faces"], case["bc"], pol)
+        mob = torch.exp(-lr)
+        diag, bx, by, bz = plan.assemble(mob)
+        plan.set_coefficients(diag, bx, by, bz, *masks)
+        plan.mobility = mob
+        rhs = b * actf
+        plan.project_mean(rhs)
+        x, res = plan.solve_pcg(rhs, **ctl)
+        plan.project_mean(x)
+        return plan, x, res
+
+    plan, x, res = forward(log_rho, b0)
+    gb, gr, ares = plan.adjoint(plan.mobility, x, probe, **ctl)
+    gr = gr.cpu().numpy()
+    gb = gb.cpu().numpy()
+    plan.close()
+    rng = np.random.default_rng(64)
+    h = 1e-6
+    worst = 0.0
+    t0 = time.perf_counter()
+    for i in range(64):
+        d = rng.standard_normal(len(act)) * act
+        dt = t(d)
+        vals = []
+        for sgn in (1.0, -1.0):
+            pl, xs, _ = forward(log_rho + sgn * h * dt, b0)
+            vals.append(float(xs @ probe))
+            pl.close()
+        fd = (vals[0] - vals[1]) / (2 * h)
+        adj = float(gr @ d)
+        worst = max(worst, abs(adj - fd) / abs(fd))
+    d = rng.standard_normal(len(act)) * act
+    dt = t(d)
+    pl, xp, _ = forward(log_rho, b0 + dt)
+    pl.close()
+    pl, xm, _ = forward(log_rho, b0 - dt)
+    pl.close()
+    fd_b = float((xp - xm) @ probe) / 2.0
+    adj_b = float(gb @ d)
+    _say(f"[256^3 adjoint] forward it={res.iterations} adjoint "
+         f"it={ares.iterations} worst FD rel={worst:.2e} "
+         f"grad_b rel={abs(adj_b - fd_b) / abs(fd_b):.2e} "
+         f"({time.perf_counter() - t0:.1f} s for 130 solves)")
+    assert worst <= 1e-6
+    assert abs(adj_b - fd_b) <= 1e-8 * abs(fd_b)

@@ -1,0 +1,210 @@
+"""GPU line relaxation (anisotropic levels) through the C ABI.
+
+* Against the reference itself: the golden fixtures written by the compiled
+  reference (tests/golden/ref_line_*.npz) hold the level plan, every level's
+  diagonal and z = M r.  The GPU preconditioner with the lexicographic order
+  reproduces z bit for bit without a null space, and to 1e-12 relative with
+  one (the projection means come from a parallel reduction).
+* Against the oracle: the zebra order (not in the reference) bit for bit in
+  FP64, and PCG solves in both orders.
+* FP32 / BF16 storage with zebra lines reach 1e-10 with the FP64 operator.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import hf_cases
+import hf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LINE_FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_line_*.npz")))
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _load(path):
+    with np.load(path, allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _fixture_policy(g, **kw):
+    p = g["policy"]
+    pol = dict(anisotropy_threshold=float(p[0]), pre_sweeps=int(p[1]),
+               post_sweeps=int(p[2]), maximum_levels=int(p[3]),
+               coarse_sweeps=int(p[4]), minimum_coarse_extent=int(p[5]),
+               f_cycle=int(p[6]), null_space_constant=int(p[7]),
+               unit_linear=int(p[8]))
+    pol.update(kw)
+    return pol
+
+
+def _fixture_plan(g, **kw):
+    import hundun_flow as H
+    n = tuple(int(v) for v in g["n"])
+    faces = [g["faces_x"], g["faces_y"], g["faces_z"]]
+    plan = H.Plan(n, faces, [int(v) for v in g["bc"]],
+                  H.default_policy(**_fixture_policy(g, **kw)))
+    coeffs = [_t(g[k]) for k in ("diag", "bx", "by", "bz")]
+    masks = [_t(g[k]) for k in ("cell_active", "x_active", "y_active",
+                                "z_active")]
+    plan.set_coefficients(*coeffs, *masks)
+    plan.keep = (coeffs, masks)
+    return plan
+
+
+def _ids(paths):
+    return [os.path.basename(p)[4:-4] for p in paths]
+
+
+@pytest.mark.parametrize("path", LINE_FIXTURES, ids=_ids(LINE_FIXTURES))
+def test_line_preconditioner_matches_reference(path):
+    g = _load(path)
+    name = os.path.basename(path)
+    _say(f"[line vs reference] {name}")
+    plan = _fixture_plan(g)
+    plan_rows = g["plan"]
+    assert plan.level_count() == len(plan_rows)
+    for level, row in enumerate(plan_rows):
+        info = plan.level_info(level)
+        assert info["shape"] == tuple(int(v) for v in row[:3])
+        assert info["coarsen_mask"] == int(row[3])
+        assert info["line_mask"] == int(row[4])
+        assert info["on_tail"] == 0
+        assert np.array_equal(plan.level_array(level, 0).cpu().numpy(),
+                              g[f"diag_{level}"])
+    z = plan.mg_apply(_t(g["r"])).cpu().numpy()
+    want = g["z"]
+    if int(g["policy"][7]) == 0:
+        assert np.array_equal(z, want), np.max(np.abs(z - want))
+    else:
+        rel = np.linalg.norm(z - want) / np.linalg.norm(want)
+        _say(f"  rel={rel:.3e}")
+        assert rel <= 1e-12
+    plan.close()
+
+
+def line_cases():
+    """Small anisotropic cases with periodic transverse axes and odd
+    extents (lines relaxed along y unless stated)."""
+    def rebc(case, bc):
+        n = case["n"]
+        act = case["cell_active"].reshape(n[2], n[1], n[0])
+        fx, fy, fz = hf_cases.face_activity(act, bc)
+        case.update(bc=bc, x_active=fx, y_active=fy, z_active=fz)
+        return case
+    return {
+        "cyl_dirichlet": lambda: rebc(hf_cases.flame_cylinder(
+            n=(20, 24, 12), seed=51), [0, 0, 1, 1, 2, 2]),
+        "cyl_periodic_x_odd": lambda: rebc(hf_cases.flame_cylinder(
+            n=(15, 16, 9), seed=52), [2, 2, 1, 1, 0, 0]),
+        "cyl_periodic_z_odd": lambda: rebc(hf_cases.flame_cylinder(
+            n=(14, 16, 9), seed=54), [0, 0, 1, 1, 2, 2]),
+        "cyl_periodic_y": lambda: rebc(hf_cases.flame_cylinder(
+            n=(12, 16, 10), seed=56), [0, 0, 2, 2, 1, 1]),
+        "cyl_wide": lambda: rebc(hf_cases.flame_cylinder(
+            n=(40, 32, 24), seed=53), [0, 0, 1, 1, 2, 2]),
+    }
+
+
+CASES = line_cases()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("f_cycle", [0, 1])
+def test_line_mg_bitwise_vs_oracle(name, order, f_cycle):
+    """No null space: the FP64 cycle with line levels equals the oracle bit
+    for bit in both line orders."""
+    _say(f"[line mg bitwise] case={name} order={order} f_cycle={f_cycle}")
+    import hundun_flow as H
+    case = CASES[name]()
+    pol = dict(null_space_constant=0, line_order=order, f_cycle=f_cycle)
+    plan = H.plan_from_case(case, H.default_policy(**pol))
+    _, mg = O.from_case(case, O.policy(**pol))
+    assert plan.level_info(0)["line_mask"] == 2
+    r = np.random.default_rng(9).standard_normal(plan.cells)
+    z = plan.mg_apply(_t(r)).cpu().numpy()
+    zo = mg.apply(r)
+    assert np.array_equal(z, zo), np.max(np.abs(z - zo))
+    plan.close()
+
+
+def _gauge(x, act):
+    fluid = act != 0
+    y = x.copy()
+    y[fluid] -= y[fluid].mean()
+    y[~fluid] = 0.0
+    return y
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("order", [0, 1])
+def test_line_pcg_fp64_vs_oracle(name, order):
+    _say(f"[line pcg] case={name} order={order}")
+    import hundun_flow as H
+    case = CASES[name]()
+    pol = dict(null_space_constant=0, line_order=order)
+    plan = H.plan_from_case(case, H.default_policy(**pol))
+    (d, bx, by, bz), mg = O.from_case(case, O.policy(**pol))
+    act = case["cell_active"]
+    b = np.where(act != 0, np.random.default_rng(4).standard_normal(
+        plan.cells), 0.0)
+    xg, res = plan.solve_pcg(_t(b), rtol=1e-12)
+    xo, ores = O.pcg(case["n"], case["bc"], d, bx, by, bz, act, mg, b,
+                     rtol=1e-12)
+    xg = xg.cpu().numpy()
+    Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
+    true_rel = np.linalg.norm(b - Ax) / np.linalg.norm(b)
+    diff = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    _say(f"  it gpu={res.iterations} oracle={ores.iterations} "
+         f"true_rel={true_rel:.2e} diff={diff:.2e}")
+    assert res.termination == "converged"
+    assert res.iterations == ores.iterations
+    assert true_rel <= 1.01e-12
+    assert diff <= 1e-10
+    assert np.all(xg[act == 0] == 0.0)
+    plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("scaling", [0, 1])
+def test_line_mixed_precision(precision, scaling):
+    """Zebra lines with reduced-precision storage inside the FP64 CG, with
+    the constant null space (Neumann/periodic walls)."""
+    _say(f"[line mixed] precision={precision} scaling={scaling}")
+    import hundun_flow as H
+    case = hf_cases.flame_cylinder(n=(48, 64, 32), seed=55)
+    pol = dict(line_order=1, precision=precision, coarse_face_scaling=scaling)
+    plan = H.plan_from_case(case, H.default_policy(**pol))
+    (d, bx, by, bz), mg = O.from_case(case, O.policy(line_order=1))
+    b = case["rhs"]
+    xg, res = plan.solve_pcg(_t(b), rtol=1e-10, flexible=True,
+                             maximum_iterations=400)
+    xo, ores = O.pcg(case["n"], case["bc"], d, bx, by, bz,
+                     case["cell_active"], mg, b, rtol=1e-12)
+    xg = xg.cpu().numpy()
+    act = case["cell_active"]
+    Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
+    true_rel = np.linalg.norm(b - Ax) / np.linalg.norm(b)
+    err = (np.linalg.norm(_gauge(xg, act) - _gauge(xo, act))
+           / np.linalg.norm(_gauge(xo, act)))
+    _say(f"  it={res.iterations} (fp64 oracle {ores.iterations}) "
+         f"true_rel={true_rel:.2e} err={err:.2e}")
+    assert res.termination == "converged"
+    assert true_rel <= 1.01e-10
+    assert err <= 1e-6
+    assert np.all(xg[act == 0] == 0.0)
+    plan.close()

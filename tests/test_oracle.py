@@ -219,3 +219,57 @@ def test_coarse_operator_preserves_constants():
         fluid = vol > 0
         assert np.allclose(D[fluid], faces[fluid], rtol=1e-13)
         assert np.all(D[~fluid] == 1.0) and np.all(faces[~fluid] == 0.0)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("reverse", [0, 1])
+def test_line_smoother_is_block_gauss_seidel(order, reverse):
+    """One line sweep of the oracle equals block Gauss-Seidel over the y-lines
+    of the dense level-0 matrix (each block solved by LAPACK), visited in
+    lexicographic order (b outer, a inner; reverse: both descending) or in
+    zebra colour passes ((a+b) even first; reverse: odd first)."""
+    case = hf_cases.flame_cylinder(n=(6, 10, 5), seed=61)
+    n = case["n"]
+    bc = [0, 0, 1, 1, 0, 0]
+    act = case["cell_active"].reshape(n[2], n[1], n[0])
+    fx, fy, fz = hf_cases.face_activity(act, bc)
+    case.update(bc=bc, x_active=fx, y_active=fy, z_active=fz)
+    _, mg = O.from_case(case, O.policy(line_order=order,
+                                       null_space_constant=0))
+    assert mg.level_info(0)["line_mask"] == 2
+    N = n[0] * n[1] * n[2]
+    A = np.zeros((N, N))
+    for c in range(N):
+        e = np.zeros(N)
+        e[c] = 1.0
+        A[:, c] = mg.masked_apply(e)
+    rng = np.random.default_rng(62)
+    x0 = rng.standard_normal(N)
+    b = rng.standard_normal(N)
+    mg.slot(0, 5)[:] = x0
+    mg.slot(0, 6)[:] = b
+    mg.smooth(0, 1, reverse, 0)
+    got = mg.slot(0, 5).copy()
+    nx, ny, nz = n
+    lines = [(a, bb) for bb in range(nz) for a in range(nx)]
+    if order == 0:
+        seq = lines[::-1] if reverse else lines
+    else:
+        first = 1 if reverse else 0
+        seq = ([l for l in lines if (l[0] + l[1]) % 2 == first] +
+               [l for l in lines if (l[0] + l[1]) % 2 != first])
+    x = x0.copy()
+    for a, bb in seq:
+        idx = np.array([a + nx * (j + ny * bb) for j in range(ny)])
+        rest = np.setdiff1d(np.arange(N), idx)
+        rhs = b[idx] - A[np.ix_(idx, rest)] @ x[rest]
+        x[idx] = np.linalg.solve(A[np.ix_(idx, idx)], rhs)
+    assert np.allclose(got, x, rtol=1e-12, atol=1e-12 * np.abs(x).max())
+    # the two orders really differ
+    if order == 0 and reverse == 0:
+        _, mz = O.from_case(case, O.policy(line_order=1,
+                                           null_space_constant=0))
+        mz.slot(0, 5)[:] = x0
+        mz.slot(0, 6)[:] = b
+        mz.smooth(0, 1, 0, 0)
+        assert not np.allclose(mz.slot(0, 5), got, rtol=1e-6)
