@@ -42,6 +42,10 @@ def class_bytes_per_cell(cls: str, s: int):
         # half-sweep: read x (all), b and D (active half), 3 face arrays,
         # write x (active half)
         "rb_finest": 5.5 * s,
+        # first half-sweep from x = 0: read b, D; write b (projected), x
+        "rb_first": 4 * s,
+        # retained defect r = b - A x: x, b, D, 3 faces, r
+        "defect": 7 * s,
         # Krylov q = A p + dot: p, D, 3 faces, q (FP64)
         "operator": 6 * 8,
         # x += a p, r -= a q: read x p r q, write x r
@@ -55,8 +59,8 @@ def class_bytes_per_cell(cls: str, s: int):
 
 # kernel class -> symbol of its kernel for the rocprof traffic lookup
 CLASS_KERNELS = {
-    "rb_finest": "void hf::k_rb<{t}>(hf::Lv<{t}>, hf::Bc, int, long)",
-    "operator": "hf::k_op_dot(hf::Lv<double>, hf::Bc, double const*, double*, double*)",
+    "rb_finest": "void hf::k_rb<{t}>(hf::Tiles, hf::Lv<{t}>, hf::Bc, int)",
+    "operator": "hf::k_op_dot(hf::Tiles, hf::Lv<double>, hf::Bc, double const*, double*, double*)",
 }
 
 
@@ -84,6 +88,8 @@ def make_case(name, nx, ny, nz):
         return hf_cases.flame_cylinder(n=(nx, ny, nz))
     if name == "lognormal_sphere":
         return hf_cases.lognormal_sphere(n=(nx, ny, nz))
+    if name == "random_spheres":
+        return hf_cases.random_spheres(n=(nx, ny, nz))
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -143,6 +149,95 @@ def cpu_baseline(case, iterations):
     }
 
 
+def large_case(args, torch, H):
+    """Forward + adjoint on the combined large case: FP32-storage V-cycle in
+    FP64 CG to 1e-10, then dL/dlog(rho) for L = sum of p over the fluid
+    cells of a 64^3 probe box, with the hierarchy rebuilt for the adjoint
+    (the forward plan is freed first).  Peak device memory comes from
+    hipMemGetInfo sampled after every phase."""
+    import numpy as np
+    nx, ny, nz = (int(v) for v in args.shape.split(","))
+    log(f"[large] generating {args.workload} {nx}x{ny}x{nz}")
+    case = make_case(args.workload, nx, ny, nz)
+    dev = torch.device("cuda", 0)
+    free0, total = torch.cuda.mem_get_info(dev)
+    low = [free0]
+
+    def sample():
+        torch.cuda.synchronize()
+        low[0] = min(low[0], torch.cuda.mem_get_info(dev)[0])
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    b = t(case["rhs"])
+    masks = [t(case[k]) for k in ("cell_active", "x_active", "y_active",
+                                  "z_active")]
+    log_rho = t(np.log(case["rho"]))
+    del case["rho"], case["mobility"]
+    pol = H.default_policy(precision=args.precision, coarse_face_scaling=1)
+    ctl = dict(rtol=1e-10, maximum_iterations=1000, flexible=True)
+
+    def build(lr):
+        plan = H.Plan(case["n"], case["faces"], case["bc"], pol)
+        mob = torch.exp(-lr)
+        diag, bx, by, bz = plan.assemble(mob)
+        plan.set_coefficients(diag, bx, by, bz, *masks)
+        sample()
+        return plan, mob, (diag, bx, by, bz)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan, mob, coeffs = build(log_rho)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, res = plan.solve_pcg(b, x, **ctl)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3
+    sample()
+    true_rel = float(torch.linalg.norm(b - plan.operator_apply(x)) /
+                     torch.linalg.norm(b))
+    log(f"[large] forward it={res.iterations} {fwd_ms:.1f} ms "
+        f"true_rel={true_rel:.2e}")
+    plan.project_mean(x)
+    plan.close()
+    del coeffs, mob
+    # loss: sum of p over a 64^3 probe box at the domain centre
+    probe = torch.zeros(nz, ny, nx, dtype=torch.float64, device=dev)
+    probe[nz // 2 - 32:nz // 2 + 32, ny // 2 - 32:ny // 2 + 32,
+          nx // 2 - 32:nx // 2 + 32] = 1.0
+    probe = probe.reshape(-1) * masks[0].to(torch.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan, mob, coeffs = build(log_rho)
+    gb, gr, ares = plan.adjoint(mob, x, probe, want_b=False, **ctl)
+    torch.cuda.synchronize()
+    adj_ms = (time.perf_counter() - t0) * 1e3
+    sample()
+    log(f"[large] adjoint it={ares.iterations} {adj_ms:.1f} ms "
+        f"(hierarchy rebuilt)")
+    plan.close()
+    peak = total - low[0]
+    cells = nx * ny * nz
+    print(json.dumps({
+        "metric": "combined large case: forward + adjoint, peak HBM",
+        "value": round(fwd_ms + adj_ms, 1), "unit": "ms",
+        "n_gpus": 1, "higher_is_better": False,
+        "dtype": f"{args.precision} V-cycle storage / f64 Krylov",
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": f"{args.workload} {nx}x{ny}x{nz}",
+                   "cells": cells, "tolerance": 1e-10,
+                   "hierarchy": "distance-corrected face aggregation"},
+        "setup_ms": round(setup_ms, 1),
+        "forward_ms": round(fwd_ms, 1), "forward_iterations": res.iterations,
+        "forward_true_relative_residual": true_rel,
+        "adjoint_ms_including_rebuild": round(adj_ms, 1),
+        "adjoint_iterations": ares.iterations,
+        "peak_device_bytes": int(peak),
+        "peak_device_gib": round(peak / 2**30, 2),
+        "device_total_gib": round(total / 2**30, 1),
+        "grad_logrho_norm": float(torch.linalg.norm(gr)),
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +253,10 @@ def main():
                     action="store_false",
                     help="skip the solves with the reference's face "
                          "aggregation")
+    ap.add_argument("--large", action="store_true",
+                    help="forward + adjoint on the combined large case "
+                         "(random_spheres 1024x512x512 unless --workload/"
+                         "--shape say otherwise) and peak device memory")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (x10) as "
                          "the known-size reference for HBM counters")
@@ -181,6 +280,14 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": 8 * n, "launches": 10}))
+        return
+
+    if args.large:
+        if args.workload == "flame_step":
+            args.workload = "random_spheres"
+        if args.shape == "512,256,256":
+            args.shape = "1024,512,512"
+        large_case(args, torch, H)
         return
 
     nx, ny, nz = (int(v) for v in args.shape.split(","))
@@ -207,8 +314,10 @@ def main():
                                f_cycle=1 if args.cycle == "f" else 0)
         plan = H.plan_from_case(case, pol, dev)
         plans[key] = plan
-        flexible = prec != "fp64"
-        x, res, times = gpu_solves(plan, b, args.steps, args.warmup, flexible,
+        # one Krylov method for every run: the flexible (Polak-Ribiere) beta
+        # is needed by reduced-precision and residual-minimising cycles and
+        # costs the FP64 cycle nothing
+        x, res, times = gpu_solves(plan, b, args.steps, args.warmup, True,
                                    torch)
         # independent FP64 check of the published solution
         ax = plan.operator_apply(x)

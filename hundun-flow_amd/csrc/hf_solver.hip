@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -40,7 +41,8 @@ enum ProfileClass : int {
   P_SWEEP_FINEST = 14,  // fused two-colour sweep on level 0
   P_SWEEP_COARSE = 15,  // fused two-colour sweep on grid levels >= 1
   P_LINE = 16,          // line relaxation (any level)
-  kProfileClasses = 17
+  P_RB_FIRST = 17,      // first colour pass from x = 0 (any grid level)
+  kProfileClasses = 18
 };
 constexpr uint32_t kDetailPlan = 9101U;
 constexpr uint32_t kDetailAlloc = 9102U;
@@ -324,7 +326,11 @@ template <class T>
 hf_status dalloc(hf_plan* p, T** out, size_t count) {
   void* ptr = nullptr;
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  if (hipMalloc(&ptr, bytes) != hipSuccess) {
+  const hipError_t e = hipMalloc(&ptr, bytes);
+  if (e != hipSuccess) {
+    // the status carries no HIP code; name it once on stderr
+    std::fprintf(stderr, "hundun_flow: hipMalloc(%zu bytes) failed: %s\n",
+                 bytes, hipGetErrorString(e));
     *out = nullptr;
     return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
   }
@@ -692,9 +698,10 @@ void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     const int first = reverse ? 1 : 0;
     for (int pass = 0; pass < 2; ++pass) {
       const int color = (first + pass) & 1;
+      const bool zero_pass = from_zero && sweep == 0 && pass == 0;
       {
-        HF_PROF(p, rb_class);
-        if (from_zero && sweep == 0 && pass == 0)
+        HF_PROF(p, zero_pass ? P_RB_FIRST : rb_class);
+        if (zero_pass)
           HF_LAUNCH(k_rb_first<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc, v,
                     mean);
         else if (L.fused)
@@ -1082,8 +1089,11 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   guard(dalloc(p, &p->d_tail_ops, std::max<size_t>(p->tail_ops_host.size(), 1)));
   if (s.code == HF_OK &&
       hipHostMalloc(reinterpret_cast<void**>(&p->host_scalars),
-                    S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess)
+                    S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    std::fprintf(stderr, "hundun_flow: hipHostMalloc(%zu bytes) failed\n",
+                 S_COUNT * sizeof(double));
     s = fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  }
   if (s.code == HF_OK) {
     for (int a = 0; a < 3; ++a) {
       if (hipMemcpyAsync(p->d_faces[a], p->faces[a].data(),

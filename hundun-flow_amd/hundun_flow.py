@@ -31,7 +31,8 @@ PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 PROFILE_CLASSES = ["rb_finest", "rb_coarse", "defect", "restrict", "prolong",
                    "tail", "operator", "update_xr", "true_residual",
                    "mg_entry_exit", "project", "update_p", "finalize",
-                   "other", "sweep_finest", "sweep_coarse", "line"]
+                   "other", "sweep_finest", "sweep_coarse", "line",
+                   "rb_first"]
 
 ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",
@@ -121,6 +122,11 @@ def load_library(path: str = LIBRARY_PATH):
         raise RuntimeError(
             f"libhundun_flow.so not found at {path}: build it first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
+    # torch ships its own HIP and HSA runtimes.  Loading torch first lets this
+    # library bind to those by soname, so the process holds one HIP runtime;
+    # loaded the other way round, two runtimes compete for the device and the
+    # second one finds none.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     D = ctypes.c_void_p  # device pointers

@@ -21,6 +21,8 @@ INVALID_PLAN = 2
 def shim():
     if not os.path.exists(SHIM):
         pytest.skip("tests/cpp/libhf_linear_check.so not built (build())")
+    import hundun_flow
+    hundun_flow.load_library()  # one HIP runtime: torch's, loaded first
     lib = ctypes.CDLL(SHIM)
     P = ctypes.c_void_p
     lib.hf_linear_check.argtypes = [P] * 20
