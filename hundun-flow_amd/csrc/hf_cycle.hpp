@@ -728,6 +728,19 @@ struct SweepArgs {
   int gx, gy;          // tiles along x and y
 };
 
+// Coefficients of one cell update, fetched ahead of the LDS phase.
+template <class C>
+struct SwCoef {
+  C d, xl, xh, yl, yh, zl, zh, b;
+};
+
+// Slots per thread of the three per-plane work lists.
+constexpr int kSwLoadSlots = (kSwPlane + kBlock - 1) / kBlock;
+constexpr int kSwRingCells = (kSwTY + 2) * kSwRingRow;
+constexpr int kSwRingSlots = (kSwRingCells + kBlock - 1) / kBlock;
+constexpr int kSwTileCells = kSwTY * (kSwTX / 2);
+constexpr int kSwTileSlots = (kSwTileCells + kBlock - 1) / kBlock;
+
 template <class T>
 __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
   using C = typename Compute<T>::type;
@@ -745,6 +758,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
              perz = a.bc.m[4] == 2;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const C omega = C(0.72);
+  const double mean = a.mean != nullptr ? *a.mean : 0.0;
 
   auto plane_of = [&](int kk, int& kg) -> bool {
     kg = kk;
@@ -753,40 +767,67 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     kg = kk < 0 ? kk + nz : kk - nz;
     return true;
   };
-  auto load = [&](int kk) {
-    C* P = buf[kk & 3];
+  // Every global load of one plane step is issued before any of its values
+  // is used, so a step waits on memory once rather than once per cell.
+  auto fetch_plane = [&](int kk, C (&v)[kSwLoadSlots]) {
     int kg;
     const bool valid = plane_of(kk, kg) && !a.from_zero;
-    for (int p = threadIdx.x; p < kSwPlane; p += kBlock) {
+#pragma unroll
+    for (int s = 0; s < kSwLoadSlots; ++s) {
+      const int p = threadIdx.x + s * kBlock;
       const int iy = p / kSwPX;
       const int ix = p - iy * kSwPX - 2;
       int gi = x0 + ix, gj = y0 + iy - 2;
-      C v = C(0);
       // only positions within two cells of the domain are ever read
-      if (valid && gi >= -2 && gi <= nx + 1 && gj >= -2 && gj <= ny + 1) {
-        bool ok = true;
-        if (gi < 0 || gi >= nx) {
-          if (perx) gi = wrapi(gi, nx); else ok = false;
-        }
-        if (gj < 0 || gj >= ny) {
-          if (pery) gj = wrapi(gj, ny); else ok = false;
-        }
-        if (ok) v = C(ld(L.x[cell_index(nx, ny, gi, gj, kg)]));
+      bool ok = valid && p < kSwPlane && gi >= -2 && gi <= nx + 1 &&
+                gj >= -2 && gj <= ny + 1;
+      if (gi < 0 || gi >= nx) {
+        if (perx) gi = wrapi(gi, nx); else ok = false;
       }
-      P[p] = v;
+      if (gj < 0 || gj >= ny) {
+        if (pery) gj = wrapi(gj, ny); else ok = false;
+      }
+      const int c = ok ? cell_index(nx, ny, gi, gj, kg) : 0;
+      const C x = C(ld(L.x[c]));
+      v[s] = ok ? x : C(0);
+    }
+  };
+  auto store_plane = [&](int kk, const C (&v)[kSwLoadSlots]) {
+    C* P = buf[kk & 3];
+#pragma unroll
+    for (int s = 0; s < kSwLoadSlots; ++s) {
+      const int p = threadIdx.x + s * kBlock;
+      if (p < kSwPlane) P[p] = v[s];
     }
   };
   auto bval = [&](int c) -> C {
     if (a.mean == nullptr) return C(ld(L.b[c]));
-    return L.act[c] ? C(ld(st<T>(static_cast<double>(ld(L.b[c])) - *a.mean)))
-                    : C(0);
+    const T projected = st<T>(static_cast<double>(ld(L.b[c])) - mean);
+    return L.act[c] ? C(ld(projected)) : C(0);
+  };
+  // coefficients of the update at (gi, gj, kg); c = 0 stands in for an
+  // inactive slot so every load address is in bounds
+  auto fetch_coef = [&](bool ok, int gi, int gj, int kg) -> SwCoef<C> {
+    const int c = ok ? cell_index(nx, ny, gi, gj, kg) : 0;
+    const int row = ok ? gj + ny * kg : 0;
+    const int fyo = ok ? c + nx * kg : 0;
+    SwCoef<C> f;
+    f.d = C(ld(L.D[c]));
+    f.xl = C(ld(L.fx[c + row]));
+    f.xh = C(ld(L.fx[c + row + 1]));
+    f.yl = C(ld(L.fy[fyo]));
+    f.yh = C(ld(L.fy[fyo + nx]));
+    f.zl = C(ld(L.fz[c]));
+    f.zh = C(ld(L.fz[c + nx * ny]));
+    f.b = bval(c);
+    return f;
   };
   auto mode = [](int m, C self) -> C { return m == 0 ? C(0) : self; };
-  // x <- x + omega (b - A x) / D at one cell held in LDS
-  auto update = [&](const C* Pm, C* P0, const C* Pp, int ix, int iy, int gi,
-                    int gj, int kg) {
+  // x <- x + omega (b - A x) / D at one cell held in LDS, in the reference
+  // evaluation order
+  auto update = [&](const SwCoef<C>& f, const C* Pm, C* P0, const C* Pp,
+                    int ix, int iy, int gi, int gj, int kg) {
     const int q = (ix + 2) + kSwPX * (iy + 2);
-    const int c = cell_index(nx, ny, gi, gj, kg);
     const C xc = P0[q];
     const C xl = (gi > 0 || perx) ? P0[q - 1] : mode(a.bc.m[0], xc);
     const C xh = (gi + 1 < nx || perx) ? P0[q + 1] : mode(a.bc.m[1], xc);
@@ -794,72 +835,107 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     const C yh = (gj + 1 < ny || pery) ? P0[q + kSwPX] : mode(a.bc.m[3], xc);
     const C zl = (kg > 0 || perz) ? Pm[q] : mode(a.bc.m[4], xc);
     const C zh = (kg + 1 < nz || perz) ? Pp[q] : mode(a.bc.m[5], xc);
-    const int row = gj + ny * kg;
-    const int fyo = c + nx * kg;
-    const C d = C(ld(L.D[c]));
-    C v = d * xc;
-    v -= C(ld(L.fx[c + row])) * xl;
-    v -= C(ld(L.fx[c + row + 1])) * xh;
-    v -= C(ld(L.fy[fyo])) * yl;
-    v -= C(ld(L.fy[fyo + nx])) * yh;
-    v -= C(ld(L.fz[c])) * zl;
-    v -= C(ld(L.fz[c + nx * ny])) * zh;
-    P0[q] = xc + omega * (bval(c) - v) / d;
+    C v = f.d * xc;
+    v -= f.xl * xl;
+    v -= f.xh * xh;
+    v -= f.yl * yl;
+    v -= f.yh * yh;
+    v -= f.zl * zl;
+    v -= f.zh * zh;
+    P0[q] = xc + omega * (f.b - v) / f.d;
   };
 
   const int c0 = a.first, c1 = a.first ^ 1;
   const int kstart = z0 - 1;
-  load(kstart - 1);
-  load(kstart);
+  {
+    C v[kSwLoadSlots];
+    fetch_plane(kstart - 1, v);
+    store_plane(kstart - 1, v);
+    fetch_plane(kstart, v);
+    store_plane(kstart, v);
+  }
   for (int k = kstart; k <= z1; ++k) {
-    load(k + 1);
-    __syncthreads();
+    // ---- issue: x of plane k+1, coefficients of both colour phases
+    C vx[kSwLoadSlots];
+    fetch_plane(k + 1, vx);
     int kg;
-    if (plane_of(k, kg)) {
+    const bool first_plane = plane_of(k, kg);
+    SwCoef<C> f0[kSwRingSlots];
+    int ix0[kSwRingSlots], iy0[kSwRingSlots], gi0[kSwRingSlots],
+        gj0[kSwRingSlots];
+    bool ok0[kSwRingSlots];
+#pragma unroll
+    for (int s = 0; s < kSwRingSlots; ++s) {
       // first colour on the tile and its one-cell ring
+      const int q = threadIdx.x + s * kBlock;
+      const int r = q / kSwRingRow;
+      const int m = q - r * kSwRingRow;
+      const int iy = r - 1;
+      int gj = y0 + iy;
+      // parity with the unwrapped index equals the wrapped one: the fused
+      // route only runs with even periodic extents
+      const int par = (c0 - (x0 - 1 + gj + kg)) & 1;
+      const int ix = -1 + par + 2 * m;
+      int gi = x0 + ix;
+      // ring cells beyond a periodic face are the wrapped cells
+      bool ok = first_plane && q < kSwRingCells && gi >= -1 && gi <= nx &&
+                gj >= -1 && gj <= ny;
+      if (gi < 0 || gi >= nx) {
+        if (perx) gi = wrapi(gi, nx); else ok = false;
+      }
+      if (gj < 0 || gj >= ny) {
+        if (pery) gj = wrapi(gj, ny); else ok = false;
+      }
+      ok0[s] = ok;
+      ix0[s] = ix;
+      iy0[s] = iy;
+      gi0[s] = gi;
+      gj0[s] = gj;
+      f0[s] = fetch_coef(ok, gi, gj, kg);
+    }
+    const int kb = k - 1;
+    const bool second_plane = kb >= z0 && kb < z1;
+    SwCoef<C> f1[kSwTileSlots];
+    int ix1[kSwTileSlots], iy1[kSwTileSlots];
+    bool ok1[kSwTileSlots];
+#pragma unroll
+    for (int s = 0; s < kSwTileSlots; ++s) {
+      // second colour on the tile, one plane behind
+      const int q = threadIdx.x + s * kBlock;
+      const int iy = q / (kSwTX / 2);
+      const int m = q - iy * (kSwTX / 2);
+      const int gj = y0 + iy;
+      const int par = (c1 - (x0 + gj + kb)) & 1;
+      const int ix = par + 2 * m;
+      const int gi = x0 + ix;
+      const bool ok = second_plane && q < kSwTileCells && gi < nx && gj < ny;
+      ok1[s] = ok;
+      ix1[s] = ix;
+      iy1[s] = iy;
+      f1[s] = fetch_coef(ok, gi, gj, kb);
+    }
+    // ---- x of plane k+1 into LDS
+    store_plane(k + 1, vx);
+    __syncthreads();
+    if (first_plane) {
       const C* Pm = buf[(k - 1) & 3];
       C* P0 = buf[k & 3];
       const C* Pp = buf[(k + 1) & 3];
-      for (int q = threadIdx.x; q < (kSwTY + 2) * kSwRingRow; q += kBlock) {
-        const int r = q / kSwRingRow;
-        const int m = q - r * kSwRingRow;
-        const int iy = r - 1;
-        int gj = y0 + iy;
-        // parity with the unwrapped index equals the wrapped one: the fused
-        // route only runs with even periodic extents
-        const int s = (c0 - (x0 - 1 + gj + kg)) & 1;
-        const int ix = -1 + s + 2 * m;
-        int gi = x0 + ix;
-        // ring cells beyond a periodic face are the wrapped cells
-        if (gi < -1 || gi > nx || gj < -1 || gj > ny) continue;
-        if (gi < 0 || gi >= nx) {
-          if (!perx) continue;
-          gi = wrapi(gi, nx);
-        }
-        if (gj < 0 || gj >= ny) {
-          if (!pery) continue;
-          gj = wrapi(gj, ny);
-        }
-        update(Pm, P0, Pp, ix, iy, gi, gj, kg);
-      }
+#pragma unroll
+      for (int s = 0; s < kSwRingSlots; ++s)
+        if (ok0[s])
+          update(f0[s], Pm, P0, Pp, ix0[s], iy0[s], gi0[s], gj0[s], kg);
     }
     __syncthreads();
-    const int kb = k - 1;
-    if (kb >= z0 && kb < z1) {
-      // second colour on the tile, one plane behind
+    if (second_plane) {
       const C* Pm = buf[(kb - 1) & 3];
       C* P0 = buf[kb & 3];
       const C* Pp = buf[(kb + 1) & 3];
-      for (int q = threadIdx.x; q < kSwTY * (kSwTX / 2); q += kBlock) {
-        const int iy = q / (kSwTX / 2);
-        const int m = q - iy * (kSwTX / 2);
-        const int gj = y0 + iy;
-        const int s = (c1 - (x0 + gj + kb)) & 1;
-        const int ix = s + 2 * m;
-        const int gi = x0 + ix;
-        if (gi >= nx || gj >= ny) continue;
-        update(Pm, P0, Pp, ix, iy, gi, gj, kb);
-      }
+#pragma unroll
+      for (int s = 0; s < kSwTileSlots; ++s)
+        if (ok1[s])
+          update(f1[s], Pm, P0, Pp, ix1[s], iy1[s], x0 + ix1[s], y0 + iy1[s],
+                 kb);
       __syncthreads();
       for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kBlock) {
         const int iy = p / kSwTX;

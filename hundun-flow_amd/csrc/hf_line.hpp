@@ -33,7 +33,7 @@ template <class T>
 struct LineArgs {
   using C = typename Compute<T>::type;
   Lv<T> L;
-  Bc bc;
+  int modes;      // boundary mode of face f (Bc encoding) in bits 2f, 2f+1
   int ax;         // line axis
   int A, B, E;    // lines per a, lines per b, cells per line
   C* cw;          // Thomas c' per cell (the reference's temporary slot)
@@ -74,15 +74,16 @@ __device__ __forceinline__ typename Compute<T>::type line_neighbour(
     int axis, int dir) {
   using C = typename Compute<T>::type;
   const Lv<T>& L = g.L;
-  const int n[3] = {L.nx, L.ny, L.nz};
+  const int extent = axis == 0 ? L.nx : (axis == 1 ? L.ny : L.nz);
   const int pos = axis == 0 ? i : (axis == 1 ? j : k);
   const int stride = axis == 0 ? 1 : (axis == 1 ? L.nx : L.nx * L.ny);
   const int next = pos + dir;
-  if (next >= 0 && next < n[axis]) return ld(L.x[c + dir * stride]);
-  const int mode = g.bc.m[2 * axis + (dir > 0 ? 1 : 0)];
+  if (next >= 0 && next < extent) return ld(L.x[c + dir * stride]);
+  const int face = 2 * axis + (dir > 0 ? 1 : 0);
+  const int mode = (g.modes >> (2 * face)) & 3;
   if (mode == 0) return C(0);
   if (mode == 1) return ld(L.x[c]);
-  if (axis == g.ax) return ld(L.x[c - dir * (n[axis] - 1) * stride]);
+  if (axis == g.ax) return ld(L.x[c - dir * (extent - 1) * stride]);
   const int a_axis = g.ax == 0 ? 1 : 0;
   // wrap from side 0 reads the far plane (side 1) and vice versa
   const int side = dir < 0 ? 1 : 0;
@@ -94,7 +95,7 @@ __device__ __forceinline__ typename Compute<T>::type line_neighbour(
 // leaving x of the line untouched, when a pivot is not above the smallest
 // normal number.
 template <class T>
-__device__ bool line_solve(const LineArgs<T>& g, int a, int b) {
+__device__ __forceinline__ bool line_solve(const LineArgs<T>& g, int a, int b) {
   using C = typename Compute<T>::type;
   const Lv<T>& L = g.L;
   const int ax = g.ax;

@@ -605,7 +605,8 @@ LineArgs<T> line_args(hf_plan* p, Level& L, int ax) {
   using C = typename Compute<T>::type;
   LineArgs<T> g{};
   g.L = view<T>(L);
-  g.bc = p->bc;
+  g.modes = 0;
+  for (int f = 0; f < 6; ++f) g.modes |= (p->bc.m[f] & 3) << (2 * f);
   g.ax = ax;
   g.A = ax == 0 ? L.n[1] : L.n[0];
   g.B = ax == 2 ? L.n[1] : L.n[2];
