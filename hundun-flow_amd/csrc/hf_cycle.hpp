@@ -381,21 +381,40 @@ __device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
   interp(i, mask & 1, Cl.nx, per & 1, &xl, &xr, &xw);
   interp(j, (mask >> 1) & 1, Cl.ny, (per >> 1) & 1, &yl, &yr, &yw);
   interp(k, (mask >> 2) & 1, Cl.nz, (per >> 2) & 1, &zl, &zr, &zw);
+  // all eight corner values are loaded before any is used (a corner that
+  // does not exist on an axis repeats the first one and is skipped below)
+  const int gx[2] = {wrapi(xl, Cl.nx), wrapi(xr, Cl.nx)};
+  const int gy[2] = {wrapi(yl, Cl.ny), wrapi(yr, Cl.ny)};
+  const int gz[2] = {wrapi(zl, Cl.nz), wrapi(zr, Cl.nz)};
+  C corner[2][2][2];
+#pragma unroll
+  for (int az = 0; az < 2; ++az)
+#pragma unroll
+    for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+      for (int ax = 0; ax < 2; ++ax)
+        corner[az][ay][ax] = C(ld(
+            Cl.x[cell_index(Cl.nx, Cl.ny, gx[ax], gy[ay], gz[az])]));
+  const C xf = C(ld(F.x[c]));
+  const int nz2 = zl == zr ? 1 : 2, ny2 = yl == yr ? 1 : 2,
+            nx2 = xl == xr ? 1 : 2;
+  // the reference order: az, ay, ax; value += ((wx*wy)*wz)*x_c
   C value = C(0);
-  for (int az = 0; az < (zl == zr ? 1 : 2); ++az) {
-    const int gz = wrapi(az == 0 ? zl : zr, Cl.nz);
+#pragma unroll
+  for (int az = 0; az < 2; ++az) {
     const C wz = az == 0 ? C(1) - C(zw) * C(0.25) : C(zw) * C(0.25);
-    for (int ay = 0; ay < (yl == yr ? 1 : 2); ++ay) {
-      const int gy = wrapi(ay == 0 ? yl : yr, Cl.ny);
+#pragma unroll
+    for (int ay = 0; ay < 2; ++ay) {
       const C wy = ay == 0 ? C(1) - C(yw) * C(0.25) : C(yw) * C(0.25);
-      for (int ax = 0; ax < (xl == xr ? 1 : 2); ++ax) {
-        const int gx = wrapi(ax == 0 ? xl : xr, Cl.nx);
+#pragma unroll
+      for (int ax = 0; ax < 2; ++ax) {
         const C wx = ax == 0 ? C(1) - C(xw) * C(0.25) : C(xw) * C(0.25);
-        value += wx * wy * wz * C(ld(Cl.x[cell_index(Cl.nx, Cl.ny, gx, gy, gz)]));
+        if (az < nz2 && ay < ny2 && ax < nx2)
+          value += wx * wy * wz * corner[az][ay][ax];
       }
     }
   }
-  F.x[c] = st<Tf>(C(ld(F.x[c])) + value);
+  F.x[c] = st<Tf>(xf + value);
 }
 
 template <class Tc, class Tf>
