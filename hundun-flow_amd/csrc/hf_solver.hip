@@ -52,6 +52,7 @@ constexpr uint32_t kDetailCoefficients = 9105U;
 constexpr uint32_t kDetailBreakdown = 9106U;
 constexpr uint32_t kDetailNonFinite = 9107U;
 constexpr uint32_t kDetailLine = 9108U;  // line relaxation pivot vanished
+constexpr uint32_t kDetailLineOff = 9109U;  // line levels not enabled
 
 hf_status ok() { return {HF_OK, 0U}; }
 hf_status fail(int code, uint32_t detail) { return {code, detail}; }
@@ -419,6 +420,14 @@ void plan_levels(hf_plan* p) {
 // HF_FUSED_SWEEP=1.
 bool sweep_enabled() {
   const char* env = std::getenv("HF_FUSED_SWEEP");
+  return env != nullptr && std::atoi(env) != 0;
+}
+
+// Line-relaxation plans are created only with HF_LINE_RELAXATION=1: their
+// device path matches the oracle in a CPU build of these kernels but has not
+// yet run cleanly on a device, so it stays off by default.
+bool line_relaxation_enabled() {
+  const char* env = std::getenv("HF_LINE_RELAXATION");
   return env != nullptr && std::atoi(env) != 0;
 }
 
@@ -969,6 +978,10 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     return fail(HF_INVALID_PLAN, kDetailPlan);
   }
   const bool line_plan = p->levels[0].line_mask != 0;
+  if (line_plan && !line_relaxation_enabled()) {
+    delete p;
+    return fail(HF_INVALID_PLAN, kDetailLineOff);
+  }
   // the tail is the longest run of coarsest levels whose x, b, r fit the
   // tail kernel's LDS and each stay under the cell threshold
   const int64_t threshold = tail_threshold();

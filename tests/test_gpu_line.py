@@ -19,7 +19,15 @@ import pytest
 import hf_cases
 import hf_oracle as O
 
-pytestmark = pytest.mark.gpu
+# The line-relaxation device path is switched off (hf_plan_create rejects
+# line plans unless HF_LINE_RELAXATION=1): the one device run of this file
+# ended in a GPU fault whose cause has not been found -- every case here runs
+# clean and matches the oracle bit for bit in a CPU build of the same kernels
+# with guard pages around every buffer.  Until a cause is shown these tests
+# stay off the device.
+pytestmark = [pytest.mark.gpu, pytest.mark.skip(
+    reason="line-relaxation device path switched off after an unexplained "
+           "GPU fault; see the module docstring")]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LINE_FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_line_*.npz")))
@@ -87,7 +95,9 @@ def test_line_preconditioner_matches_reference(path):
                               g[f"diag_{level}"])
     z = plan.mg_apply(_t(g["r"])).cpu().numpy()
     want = g["z"]
-    if int(g["policy"][7]) == 0:
+    # bit for bit unless a parallel reduction enters z: the projection means
+    # (constant null space) or the residual-minimising scale (unit_linear 0)
+    if int(g["policy"][7]) == 0 and int(g["policy"][8]) == 1:
         assert np.array_equal(z, want), np.max(np.abs(z - want))
     else:
         rel = np.linalg.norm(z - want) / np.linalg.norm(want)
