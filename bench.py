@@ -44,6 +44,9 @@ def class_bytes_per_cell(cls: str, s: int):
         "rb_finest": 5.5 * s,
         # first half-sweep from x = 0: read b, D; write b (projected), x
         "rb_first": 4 * s,
+        # fused two-colour sweep: read x, b, D, 3 face arrays, write x (the
+        # first sweep from zero reads no x and writes the projected b)
+        "sweep_finest": 7 * s,
         # retained defect r = b - A x: x, b, D, 3 faces, r
         "defect": 7 * s,
         # Krylov q = A p + dot: p, D, 3 faces, q (FP64)
@@ -60,6 +63,7 @@ def class_bytes_per_cell(cls: str, s: int):
 # kernel class -> symbol of its kernel for the rocprof traffic lookup
 CLASS_KERNELS = {
     "rb_finest": "void hf::k_rb<{t}>(hf::Tiles, hf::Lv<{t}>, hf::Bc, int)",
+    "sweep_finest": "void hf::k_rb_sweep<{t}>(hf::SweepArgs<{t}>)",
     "operator": "hf::k_op_dot(hf::Tiles, hf::Lv<double>, hf::Bc, double const*, double*, double*)",
 }
 
@@ -302,22 +306,25 @@ def main():
 
     results = {}
     plans = {}
-    runs = [(args.precision, 1)]
+    # (storage precision, coarse-face scaling, flexible beta).  The
+    # reduced-precision cycle runs under the flexible (Polak-Ribiere) beta;
+    # FP64 runs twice: the plain solve of the reference algorithm (FP64
+    # cycle, beta = rho'/rho) and the same flexible Krylov as the timed run.
+    runs = [(args.precision, 1, True)]
     if not args.no_fp64:
-        runs.append(("fp64", 1))
+        runs += [("fp64", 1, False), ("fp64", 1, True)]
     if args.reference_hierarchy:
-        runs += [(args.precision, 0)] + ([] if args.no_fp64 else [("fp64", 0)])
-    for prec, scaling in runs:
-        key = (prec, scaling)
-        log(f"[bench] plan {prec} coarse_face_scaling={scaling}")
+        runs += [(args.precision, 0, True)] + (
+            [] if args.no_fp64 else [("fp64", 0, False)])
+    for prec, scaling, flexible in runs:
+        key = (prec, scaling, flexible)
+        log(f"[bench] plan {prec} coarse_face_scaling={scaling} "
+            f"flexible={flexible}")
         pol = H.default_policy(precision=prec, coarse_face_scaling=scaling,
                                f_cycle=1 if args.cycle == "f" else 0)
         plan = H.plan_from_case(case, pol, dev)
         plans[key] = plan
-        # one Krylov method for every run: the flexible (Polak-Ribiere) beta
-        # is needed by reduced-precision and residual-minimising cycles and
-        # costs the FP64 cycle nothing
-        x, res, times = gpu_solves(plan, b, args.steps, args.warmup, True,
+        x, res, times = gpu_solves(plan, b, args.steps, args.warmup, flexible,
                                    torch)
         # independent FP64 check of the published solution
         ax = plan.operator_apply(x)
@@ -326,22 +333,23 @@ def main():
             if (act == 0).any() else 0.0
         results[key] = dict(x=x, res=res, times=times, true_rel=true_rel,
                             solid_max=solid_max)
-        log(f"[bench] {prec} scaling={scaling}: iterations={res.iterations} "
+        log(f"[bench] {prec} scaling={scaling} flexible={flexible}: "
+            f"iterations={res.iterations} "
             f"ms={['%.2f' % t for t in times]} true_rel={true_rel:.3e} "
             f"solid_max={solid_max}")
         assert res.termination == "converged", res
         assert true_rel <= 1.01e-10, true_rel
         assert solid_max == 0.0
-        if scaling == 0 and key != (args.precision, 1):
+        if key != (args.precision, 1, True):
             plan.close()
             plans.pop(key)
 
     main_prec = args.precision
-    main = results[(main_prec, 1)]
+    main = results[(main_prec, 1, True)]
     ms = sum(main["times"]) / len(main["times"])
 
     # dominant kernel: one more profiled solve of the timed configuration
-    plan = plans[(main_prec, 1)]
+    plan = plans[(main_prec, 1, True)]
     plan.set_profiling(True)
     x = torch.zeros_like(b)
     plan.solve_pcg(b, x, rtol=1e-10, maximum_iterations=1000, flexible=True)
@@ -361,6 +369,13 @@ def main():
         if dominant == "rb_finest":
             work = ((nx + 1) // 2) * ny * nz
             grid = -(-work // 256) * 256
+        elif dominant == "sweep_finest":
+            # k_rb_sweep: 128 x 8 tiles, planes per block as the library
+            # chooses them
+            per_plane = -(-nx // 128) * -(-ny // 8)
+            want = max(1, 2048 // per_plane)
+            zc = min(32, max(4, -(-nz // want)))
+            grid = per_plane * -(-nz // zc) * 256
         else:
             grid = -(-cells // 256) * 256
         tag = f"{args.workload}_{nx}x{ny}x{nz}_{main_prec}"
@@ -379,8 +394,8 @@ def main():
         return sum(r["times"]) / len(r["times"])
 
     extra = {}
-    if ("fp64", 1) in results:
-        r64 = results[("fp64", 1)]
+    if ("fp64", 1, False) in results:
+        r64 = results[("fp64", 1, False)]
         ms64 = mean_ms(r64)
         fluid = torch.from_numpy(act != 0).to(dev)
 
@@ -391,19 +406,24 @@ def main():
             return v
         err = float(torch.linalg.norm(gauge(main["x"]) - gauge(r64["x"]))
                     / torch.linalg.norm(gauge(r64["x"])))
+        r64f = results[("fp64", 1, True)]
         extra = {"fp64_ms_per_solve": round(ms64, 3),
                  "fp64_iterations": r64["res"].iterations,
                  "speedup_vs_fp64": round(ms64 / ms, 3),
+                 "fp64_krylov": "PCG, beta = rho'/rho (the reference solve)",
+                 "fp64_flexible_ms_per_solve": round(mean_ms(r64f), 3),
+                 "fp64_flexible_iterations": r64f["res"].iterations,
+                 "speedup_vs_fp64_flexible": round(mean_ms(r64f) / ms, 3),
                  "error_vs_fp64_solution": err}
 
     reference = None
-    if (main_prec, 0) in results:
-        r32 = results[(main_prec, 0)]
+    if (main_prec, 0, True) in results:
+        r32 = results[(main_prec, 0, True)]
         reference = {"hierarchy": "reference face aggregation",
                      "ms_per_solve": round(mean_ms(r32), 3),
                      "iterations": r32["res"].iterations}
-        if ("fp64", 0) in results:
-            r64 = results[("fp64", 0)]
+        if ("fp64", 0, False) in results:
+            r64 = results[("fp64", 0, False)]
             reference.update({"fp64_ms_per_solve": round(mean_ms(r64), 3),
                               "fp64_iterations": r64["res"].iterations,
                               "speedup_vs_fp64": round(mean_ms(r64) /
@@ -412,10 +432,10 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         # the reference CPU algorithm: reference aggregation, FP64 PCG
-        if ("fp64", 0) in results:
-            its = results[("fp64", 0)]["res"].iterations
-        elif ("fp64", 1) in results:
-            its = results[("fp64", 1)]["res"].iterations
+        if ("fp64", 0, False) in results:
+            its = results[("fp64", 0, False)]["res"].iterations
+        elif ("fp64", 1, False) in results:
+            its = results[("fp64", 1, False)]["res"].iterations
         else:
             its = main["res"].iterations
         log("[bench] cpu baseline (oracle, full grid, 1 iteration)")

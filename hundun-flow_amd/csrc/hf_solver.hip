@@ -415,12 +415,13 @@ void plan_levels(hf_plan* p) {
   }
 }
 
-// The fused sweep is exact but not yet faster than two colour passes (each
-// plane step still waits on its coefficient loads); opt in with
-// HF_FUSED_SWEEP=1.
+// The fused two-colour sweep (one pass over the level instead of two) runs on
+// the finest level; HF_FUSED_SWEEP=0 selects two colour passes there too.
+// Coarser levels have too few tiles per plane to hide its plane-step
+// barriers and keep the two-pass route.
 bool sweep_enabled() {
   const char* env = std::getenv("HF_FUSED_SWEEP");
-  return env != nullptr && std::atoi(env) != 0;
+  return env == nullptr || std::atoi(env) != 0;
 }
 
 // Line-relaxation plans are created only with HF_LINE_RELAXATION=1: their
@@ -1016,8 +1017,8 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
     // fused sweeps on grid levels with the fused colour route and rows long
     // enough for the 128 x 8 tiles to stay mostly busy
-    L.sweep = !L.tail && L.line_mask == 0 && L.fused && L.n[0] >= 32 &&
-              L.n[1] >= 8 && sweep_enabled();
+    L.sweep = l == 0 && !L.tail && L.line_mask == 0 && L.fused &&
+              L.n[0] >= 32 && L.n[1] >= 8 && sweep_enabled();
     if (L.sweep) {
       L.sweep_gx = (L.n[0] + kSwTX - 1) / kSwTX;
       L.sweep_gy = (L.n[1] + kSwTY - 1) / kSwTY;

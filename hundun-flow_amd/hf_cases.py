@@ -245,7 +245,9 @@ def flame_probe(n=(256, 256, 256), seed: int = 4) -> dict:
     nx, ny, nz = n
     faces = case["faces"]
     X, Y, Z = _grid(faces)
-    half = [16 / nx, 16 / ny, 16 / nz]
+    # 32 cells across on large grids, never more than a quarter of the
+    # domain (a box covering every fluid cell would sum a zero-mean field)
+    half = [min(16 / nx, 0.125), min(16 / ny, 0.125), min(16 / nz, 0.125)]
     probe = ((np.abs(X - 0.75) < half[0]) & (np.abs(Y - 0.5) < half[1])
              & (np.abs(Z - 0.5) < half[2]))
     case["probe"] = np.ascontiguousarray(

@@ -126,8 +126,14 @@ def test_flame_step_512_precisions():
 def test_flame_probe_256_adjoint_64_directions():
     """Differentiation workload at 256^3: gradients of the probe-box pressure
     sum w.r.t. log rho along 64 seeded directions against central finite
-    differences (h = 1e-6) of tight FP64 solves, and w.r.t. b against the
-    linear response."""
+    differences (h = 1e-6), and w.r.t. b against the linear response.
+
+    Each central difference L(rho e^{+h d}) - L(rho e^{-h d}) is evaluated
+    against the base solution p0: A+- (p+- - p0) = b - A+- p0 has an O(h)
+    right-hand side, so the relative tolerance bounds the error of each
+    correction, and p+ - p- = (p+ - p0) - (p- - p0) holds exactly whatever
+    the error of p0 -- the difference carries no 1/h amplification of the
+    solve tolerance."""
     import torch
 
     import hundun_flow as H
@@ -142,21 +148,25 @@ def test_flame_probe_256_adjoint_64_directions():
     probe = t(case["probe"])
     b0 = t(case["rhs"])
     pol = H.default_policy(coarse_face_scaling=1)
-    ctl = dict(rtol=1e-13, maximum_iterations=2000, flexible=True)
+    ctl = dict(rtol=1e-12, maximum_iterations=2000, flexible=True)
 
-    def forward(lr, b):
+    def build(lr):
         plan = H.Plan(case["n"], case["faces"], case["bc"], pol)
         mob = torch.exp(-lr)
         diag, bx, by, bz = plan.assemble(mob)
         plan.set_coefficients(diag, bx, by, bz, *masks)
         plan.mobility = mob
-        rhs = b * actf
-        plan.project_mean(rhs)
-        x, res = plan.solve_pcg(rhs, **ctl)
-        plan.project_mean(x)
-        return plan, x, res
+        return plan
 
-    plan, x, res = forward(log_rho, b0)
+    def solve(plan, rhs, **kw):
+        rhs = rhs * actf
+        plan.project_mean(rhs)
+        x, res = plan.solve_pcg(rhs, **kw)
+        plan.project_mean(x)
+        return x, res
+
+    plan = build(log_rho)
+    x, res = solve(plan, b0, **ctl)
     gb, gr, ares = plan.adjoint(plan.mobility, x, probe, **ctl)
     gr = gr.cpu().numpy()
     gb = gb.cpu().numpy()
@@ -165,22 +175,28 @@ def test_flame_probe_256_adjoint_64_directions():
     h = 1e-6
     worst = 0.0
     t0 = time.perf_counter()
+    rhs0 = b0 * actf
+    pbase = build(log_rho)
+    pbase.project_mean(rhs0)
+    pbase.close()
     for i in range(64):
         d = rng.standard_normal(len(act)) * act
         dt = t(d)
-        vals = []
+        corr = []
         for sgn in (1.0, -1.0):
-            pl, xs, _ = forward(log_rho + sgn * h * dt, b0)
-            vals.append(float(xs @ probe))
+            pl = build(log_rho + sgn * h * dt)
+            delta, _ = solve(pl, rhs0 - pl.operator_apply(x), rtol=1e-10,
+                             maximum_iterations=2000, flexible=True)
             pl.close()
-        fd = (vals[0] - vals[1]) / (2 * h)
+            corr.append(delta)
+        fd = float((corr[0] - corr[1]) @ probe) / (2 * h)
         adj = float(gr @ d)
         worst = max(worst, abs(adj - fd) / abs(fd))
     d = rng.standard_normal(len(act)) * act
     dt = t(d)
-    pl, xp, _ = forward(log_rho, b0 + dt)
-    pl.close()
-    pl, xm, _ = forward(log_rho, b0 - dt)
+    pl = build(log_rho)
+    xp, _ = solve(pl, b0 + dt, **ctl)
+    xm, _ = solve(pl, b0 - dt, **ctl)
     pl.close()
     fd_b = float((xp - xm) @ probe) / 2.0
     adj_b = float(gb @ d)
