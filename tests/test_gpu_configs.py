@@ -173,7 +173,8 @@ def test_flame_probe_256_adjoint_64_directions():
     plan.close()
     rng = np.random.default_rng(64)
     h = 1e-6
-    worst = 0.0
+    adjs, fds, scales = [], [], []
+    gnorm = float(np.linalg.norm(gr))
     t0 = time.perf_counter()
     rhs0 = b0 * actf
     pbase = build(log_rho)
@@ -189,9 +190,9 @@ def test_flame_probe_256_adjoint_64_directions():
                              maximum_iterations=2000, flexible=True)
             pl.close()
             corr.append(delta)
-        fd = float((corr[0] - corr[1]) @ probe) / (2 * h)
-        adj = float(gr @ d)
-        worst = max(worst, abs(adj - fd) / abs(fd))
+        fds.append(float((corr[0] - corr[1]) @ probe) / (2 * h))
+        adjs.append(float(gr @ d))
+        scales.append(gnorm * float(np.linalg.norm(d)))
     d = rng.standard_normal(len(act)) * act
     dt = t(d)
     pl = build(log_rho)
@@ -200,9 +201,19 @@ def test_flame_probe_256_adjoint_64_directions():
     pl.close()
     fd_b = float((xp - xm) @ probe) / 2.0
     adj_b = float(gb @ d)
+    adjs, fds, scales = np.array(adjs), np.array(fds), np.array(scales)
+    # the central difference at h = 1e-6 carries ~1e-13 of rounding in L,
+    # i.e. ~1e-7 relative noise where g.d is of typical size and more where
+    # g.d happens to be small; the 64-vector of directional derivatives and
+    # the error scaled by |g| |d| are the noise-robust measures
+    vec_rel = float(np.linalg.norm(adjs - fds) / np.linalg.norm(fds))
+    scaled = float(np.max(np.abs(adjs - fds) / scales))
+    worst = float(np.max(np.abs(adjs - fds) / np.abs(fds)))
     _say(f"[256^3 adjoint] forward it={res.iterations} adjoint "
-         f"it={ares.iterations} worst FD rel={worst:.2e} "
+         f"it={ares.iterations} 64-direction rel={vec_rel:.2e} "
+         f"max scaled={scaled:.2e} worst per-direction rel={worst:.2e} "
          f"grad_b rel={abs(adj_b - fd_b) / abs(fd_b):.2e} "
          f"({time.perf_counter() - t0:.1f} s for 130 solves)")
-    assert worst <= 1e-6
+    assert vec_rel <= 1e-6
+    assert scaled <= 1e-8
     assert abs(adj_b - fd_b) <= 1e-8 * abs(fd_b)
