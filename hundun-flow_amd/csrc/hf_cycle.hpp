@@ -753,12 +753,11 @@ struct SwCoef {
   C d, xl, xh, yl, yh, zl, zh, b;
 };
 
-// Slots per thread of the three per-plane work lists.
+// Work of one block per plane: the x plane with its two-cell halo (7 slots
+// per thread), the 4 columns a thread owns, and one first-colour ring cell.
 constexpr int kSwLoadSlots = (kSwPlane + kBlock - 1) / kBlock;
-constexpr int kSwRingCells = (kSwTY + 2) * kSwRingRow;
-constexpr int kSwRingSlots = (kSwRingCells + kBlock - 1) / kBlock;
-constexpr int kSwTileCells = kSwTY * (kSwTX / 2);
-constexpr int kSwTileSlots = (kSwTileCells + kBlock - 1) / kBlock;
+constexpr int kSwRingCells = 2 * (kSwTX + 2) / 2 + 2 * kSwTY / 2;  // 138
+constexpr int kSwCols = 4;
 
 template <class T>
 __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
@@ -778,6 +777,8 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const C omega = C(0.72);
   const double mean = a.mean != nullptr ? *a.mean : 0.0;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
 
   auto plane_of = [&](int kk, int& kg) -> bool {
     kg = kk;
@@ -786,8 +787,6 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     kg = kk < 0 ? kk + nz : kk - nz;
     return true;
   };
-  // Every global load of one plane step is issued before any of its values
-  // is used, so a step waits on memory once rather than once per cell.
   auto fetch_plane = [&](int kk, C (&v)[kSwLoadSlots]) {
     int kg;
     const bool valid = plane_of(kk, kg) && !a.from_zero;
@@ -843,9 +842,9 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
   };
   auto mode = [](int m, C self) -> C { return m == 0 ? C(0) : self; };
   // x <- x + omega (b - A x) / D at one cell held in LDS, in the reference
-  // evaluation order
+  // evaluation order; returns the new value
   auto update = [&](const SwCoef<C>& f, const C* Pm, C* P0, const C* Pp,
-                    int ix, int iy, int gi, int gj, int kg) {
+                    int ix, int iy, int gi, int gj, int kg) -> C {
     const int q = (ix + 2) + kSwPX * (iy + 2);
     const C xc = P0[q];
     const C xl = (gi > 0 || perx) ? P0[q - 1] : mode(a.bc.m[0], xc);
@@ -861,10 +860,42 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     v -= f.yh * yh;
     v -= f.zl * zl;
     v -= f.zh * zh;
-    P0[q] = xc + omega * (f.b - v) / f.d;
+    const C xn = xc + omega * (f.b - v) / f.d;
+    P0[q] = xn;
+    return xn;
   };
 
+  // Own columns: (ix, iy) = (lane + 64 dx, wave + 4 dy).  kind 1: a cell of
+  // the domain (updated in its colour phase and stored); kind 2: the wrapped
+  // cell just past a periodic far face of a partial tile (first colour only,
+  // recomputed like a ring cell, never stored); kind 0: nothing.
+  int kind[kSwCols], cix[kSwCols], ciy[kSwCols], cgi[kSwCols], cgj[kSwCols];
+#pragma unroll
+  for (int q = 0; q < kSwCols; ++q) {
+    const int ix = lane + kWave * (q & 1);
+    const int iy = wave + 4 * (q >> 1);
+    int gi = x0 + ix, gj = y0 + iy;
+    int kd = 0;
+    if (gi < nx && gj < ny) {
+      kd = 1;
+    } else if (gi <= nx && gj <= ny && (gi < nx || perx) &&
+               (gj < ny || pery)) {
+      kd = 2;
+      if (gi == nx) gi = 0;
+      if (gj == ny) gj = 0;
+    }
+    kind[q] = kd;
+    cix[q] = ix;
+    ciy[q] = iy;
+    cgi[q] = gi;
+    cgj[q] = gj;
+  }
+  // every own column has the parity of (x0 + lane + y0 + wave): 64 and 4
+  // are even (parity of the unwrapped index equals the wrapped one: the
+  // fused route only runs with even periodic extents)
+  const int own_parity = (x0 + lane + y0 + wave) & 1;
   const int c0 = a.first, c1 = a.first ^ 1;
+  (void)c1;
   const int kstart = z0 - 1;
   {
     C v[kSwLoadSlots];
@@ -873,99 +904,96 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     fetch_plane(kstart, v);
     store_plane(kstart, v);
   }
+  SwCoef<C> prev[kSwCols];
+#pragma unroll
+  for (int q = 0; q < kSwCols; ++q) prev[q] = SwCoef<C>{};
   for (int k = kstart; k <= z1; ++k) {
-    // ---- issue: x of plane k+1, coefficients of both colour phases
+    // ---- issue every global load of the step
     C vx[kSwLoadSlots];
     fetch_plane(k + 1, vx);
     int kg;
-    const bool first_plane = plane_of(k, kg);
-    SwCoef<C> f0[kSwRingSlots];
-    int ix0[kSwRingSlots], iy0[kSwRingSlots], gi0[kSwRingSlots],
-        gj0[kSwRingSlots];
-    bool ok0[kSwRingSlots];
+    const bool plane_ok = plane_of(k, kg);
+    SwCoef<C> cur[kSwCols];
 #pragma unroll
-    for (int s = 0; s < kSwRingSlots; ++s) {
-      // first colour on the tile and its one-cell ring
-      const int q = threadIdx.x + s * kBlock;
-      const int r = q / kSwRingRow;
-      const int m = q - r * kSwRingRow;
-      const int iy = r - 1;
-      int gj = y0 + iy;
-      // parity with the unwrapped index equals the wrapped one: the fused
-      // route only runs with even periodic extents
-      const int par = (c0 - (x0 - 1 + gj + kg)) & 1;
-      const int ix = -1 + par + 2 * m;
-      int gi = x0 + ix;
-      // ring cells beyond a periodic face are the wrapped cells
-      bool ok = first_plane && q < kSwRingCells && gi >= -1 && gi <= nx &&
-                gj >= -1 && gj <= ny;
-      if (gi < 0 || gi >= nx) {
-        if (perx) gi = wrapi(gi, nx); else ok = false;
+    for (int q = 0; q < kSwCols; ++q)
+      cur[q] = fetch_coef(plane_ok && kind[q] != 0, cgi[q], cgj[q], kg);
+    // first-colour ring cell of this thread: rows iy = -1 and 8 (65 cells
+    // each), columns ix = -1 and 128 (4 cells each)
+    int rix = 0, riy = 0, rgi = 0, rgj = 0;
+    bool rok = false;
+    {
+      const int s = threadIdx.x;
+      if (s < 2 * kSwRingRow) {
+        riy = s < kSwRingRow ? -1 : kSwTY;
+        const int m = s < kSwRingRow ? s : s - kSwRingRow;
+        const int par = (c0 - (x0 - 1 + y0 + riy + k)) & 1;
+        rix = -1 + par + 2 * m;
+      } else if (s < kSwRingCells) {
+        const int t = s - 2 * kSwRingRow;
+        rix = t < kSwTY / 2 ? -1 : kSwTX;
+        const int m = t < kSwTY / 2 ? t : t - kSwTY / 2;
+        const int par = (c0 - (x0 + rix + y0 + k)) & 1;
+        riy = par + 2 * m;
       }
-      if (gj < 0 || gj >= ny) {
-        if (pery) gj = wrapi(gj, ny); else ok = false;
+      rgi = x0 + rix;
+      rgj = y0 + riy;
+      rok = plane_ok && s < kSwRingCells && rgi >= -1 && rgi <= nx &&
+            rgj >= -1 && rgj <= ny;
+      // a ring position past the domain's far face is the wrapped cell when
+      // the axis is periodic; on a partial tile the far face lies inside
+      // the tile and its wrapped cells are own columns of kind 2
+      if (rgi < 0 || rgi >= nx) {
+        if (perx) rgi = wrapi(rgi, nx); else rok = false;
       }
-      ok0[s] = ok;
-      ix0[s] = ix;
-      iy0[s] = iy;
-      gi0[s] = gi;
-      gj0[s] = gj;
-      f0[s] = fetch_coef(ok, gi, gj, kg);
+      if (rgj < 0 || rgj >= ny) {
+        if (pery) rgj = wrapi(rgj, ny); else rok = false;
+      }
     }
-    const int kb = k - 1;
-    const bool second_plane = kb >= z0 && kb < z1;
-    SwCoef<C> f1[kSwTileSlots];
-    int ix1[kSwTileSlots], iy1[kSwTileSlots];
-    bool ok1[kSwTileSlots];
-#pragma unroll
-    for (int s = 0; s < kSwTileSlots; ++s) {
-      // second colour on the tile, one plane behind
-      const int q = threadIdx.x + s * kBlock;
-      const int iy = q / (kSwTX / 2);
-      const int m = q - iy * (kSwTX / 2);
-      const int gj = y0 + iy;
-      const int par = (c1 - (x0 + gj + kb)) & 1;
-      const int ix = par + 2 * m;
-      const int gi = x0 + ix;
-      const bool ok = second_plane && q < kSwTileCells && gi < nx && gj < ny;
-      ok1[s] = ok;
-      ix1[s] = ix;
-      iy1[s] = iy;
-      f1[s] = fetch_coef(ok, gi, gj, kb);
-    }
-    // ---- x of plane k+1 into LDS
+    const SwCoef<C> rc = fetch_coef(rok, rgi, rgj, kg);
+    // the owner's columns are first colour on plane k and second colour on
+    // plane k - 1 (or the other way round, then idle this step)
+    const bool first_here = ((own_parity + k) & 1) == c0;
+    __syncthreads();  // plane k-3's buffer is free (read by the last step)
     store_plane(k + 1, vx);
     __syncthreads();
-    if (first_plane) {
+    // ---- first colour on plane k: own columns and the ring
+    if (plane_ok) {
       const C* Pm = buf[(k - 1) & 3];
       C* P0 = buf[k & 3];
       const C* Pp = buf[(k + 1) & 3];
+      const bool store = k >= z0 && k < z1;
 #pragma unroll
-      for (int s = 0; s < kSwRingSlots; ++s)
-        if (ok0[s])
-          update(f0[s], Pm, P0, Pp, ix0[s], iy0[s], gi0[s], gj0[s], kg);
+      for (int q = 0; q < kSwCols; ++q) {
+        if (kind[q] == 0) continue;
+        if (first_here) {
+          const C xn = update(cur[q], Pm, P0, Pp, cix[q], ciy[q], cgi[q],
+                              cgj[q], kg);
+          if (store && kind[q] == 1)
+            a.xout[cell_index(nx, ny, cgi[q], cgj[q], kg)] = st<T>(xn);
+        }
+        if (store && kind[q] == 1 && a.bout != nullptr)
+          a.bout[cell_index(nx, ny, cgi[q], cgj[q], kg)] = st<T>(cur[q].b);
+      }
+      if (rok) update(rc, Pm, P0, Pp, rix, riy, rgi, rgj, kg);
     }
     __syncthreads();
-    if (second_plane) {
+    // ---- second colour on plane k - 1 with the coefficients kept from the
+    // last step
+    const int kb = k - 1;
+    if (first_here && kb >= z0 && kb < z1) {
       const C* Pm = buf[(kb - 1) & 3];
       C* P0 = buf[kb & 3];
       const C* Pp = buf[(kb + 1) & 3];
 #pragma unroll
-      for (int s = 0; s < kSwTileSlots; ++s)
-        if (ok1[s])
-          update(f1[s], Pm, P0, Pp, ix1[s], iy1[s], x0 + ix1[s], y0 + iy1[s],
-                 kb);
-      __syncthreads();
-      for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kBlock) {
-        const int iy = p / kSwTX;
-        const int ix = p - iy * kSwTX;
-        const int gi = x0 + ix, gj = y0 + iy;
-        if (gi >= nx || gj >= ny) continue;
-        const int c = cell_index(nx, ny, gi, gj, kb);
-        a.xout[c] = st<T>(P0[(ix + 2) + kSwPX * (iy + 2)]);
-        if (a.bout != nullptr) a.bout[c] = st<T>(bval(c));
+      for (int q = 0; q < kSwCols; ++q) {
+        if (kind[q] != 1) continue;
+        const C xn = update(prev[q], Pm, P0, Pp, cix[q], ciy[q], cgi[q],
+                            cgj[q], kb);
+        a.xout[cell_index(nx, ny, cgi[q], cgj[q], kb)] = st<T>(xn);
       }
     }
+#pragma unroll
+    for (int q = 0; q < kSwCols; ++q) prev[q] = cur[q];
   }
 }
 
