@@ -865,15 +865,17 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     return xn;
   };
 
-  // Own columns: (ix, iy) = (lane + 64 dx, wave + 4 dy).  kind 1: a cell of
-  // the domain (updated in its colour phase and stored); kind 2: the wrapped
+  // Own columns: (ix, iy) = (lane + 64 dx, wave or 7 - wave): rows wave
+  // and 7 - wave have opposite parity, so every plane step gives each thread
+  // two first-colour and two second-colour updates.  kind 1: a cell of the
+  // domain (updated in its colour phase and stored); kind 2: the wrapped
   // cell just past a periodic far face of a partial tile (first colour only,
   // recomputed like a ring cell, never stored); kind 0: nothing.
   int kind[kSwCols], cix[kSwCols], ciy[kSwCols], cgi[kSwCols], cgj[kSwCols];
 #pragma unroll
   for (int q = 0; q < kSwCols; ++q) {
     const int ix = lane + kWave * (q & 1);
-    const int iy = wave + 4 * (q >> 1);
+    const int iy = (q >> 1) == 0 ? wave : kSwTY - 1 - wave;
     int gi = x0 + ix, gj = y0 + iy;
     int kd = 0;
     if (gi < nx && gj < ny) {
@@ -890,10 +892,11 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     cgi[q] = gi;
     cgj[q] = gj;
   }
-  // every own column has the parity of (x0 + lane + y0 + wave): 64 and 4
-  // are even (parity of the unwrapped index equals the wrapped one: the
+  // column parity (of the unwrapped index, equal to the wrapped one: the
   // fused route only runs with even periodic extents)
-  const int own_parity = (x0 + lane + y0 + wave) & 1;
+  int cpar[kSwCols];
+#pragma unroll
+  for (int q = 0; q < kSwCols; ++q) cpar[q] = (x0 + cix[q] + y0 + ciy[q]) & 1;
   const int c0 = a.first, c1 = a.first ^ 1;
   (void)c1;
   const int kstart = z0 - 1;
@@ -950,9 +953,11 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
       }
     }
     const SwCoef<C> rc = fetch_coef(rok, rgi, rgj, kg);
-    // the owner's columns are first colour on plane k and second colour on
-    // plane k - 1 (or the other way round, then idle this step)
-    const bool first_here = ((own_parity + k) & 1) == c0;
+    // a column that is first colour on plane k is second colour on plane
+    // k - 1; the others wait for the next step
+    bool first_here[kSwCols];
+#pragma unroll
+    for (int q = 0; q < kSwCols; ++q) first_here[q] = ((cpar[q] + k) & 1) == c0;
     __syncthreads();  // plane k-3's buffer is free (read by the last step)
     store_plane(k + 1, vx);
     __syncthreads();
@@ -965,7 +970,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < kSwCols; ++q) {
         if (kind[q] == 0) continue;
-        if (first_here) {
+        if (first_here[q]) {
           const C xn = update(cur[q], Pm, P0, Pp, cix[q], ciy[q], cgi[q],
                               cgj[q], kg);
           if (store && kind[q] == 1)
@@ -980,13 +985,13 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     // ---- second colour on plane k - 1 with the coefficients kept from the
     // last step
     const int kb = k - 1;
-    if (first_here && kb >= z0 && kb < z1) {
+    if (kb >= z0 && kb < z1) {
       const C* Pm = buf[(kb - 1) & 3];
       C* P0 = buf[kb & 3];
       const C* Pp = buf[(kb + 1) & 3];
 #pragma unroll
       for (int q = 0; q < kSwCols; ++q) {
-        if (kind[q] != 1) continue;
+        if (kind[q] != 1 || !first_here[q]) continue;
         const C xn = update(prev[q], Pm, P0, Pp, cix[q], ciy[q], cgi[q],
                             cgj[q], kb);
         a.xout[cell_index(nx, ny, cgi[q], cgj[q], kb)] = st<T>(xn);
