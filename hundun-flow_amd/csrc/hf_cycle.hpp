@@ -757,7 +757,6 @@ struct SwCoef {
 // per thread), the 4 columns a thread owns, and one first-colour ring cell.
 constexpr int kSwLoadSlots = (kSwPlane + kBlock - 1) / kBlock;
 constexpr int kSwRingCells = 2 * (kSwTX + 2) / 2 + 2 * kSwTY / 2;  // 138
-constexpr int kSwCols = 4;
 
 template <class T>
 __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
@@ -865,40 +864,43 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     return xn;
   };
 
-  // Own columns: (ix, iy) = (lane + 64 dx, wave or 7 - wave): rows wave
-  // and 7 - wave have opposite parity, so every plane step gives each thread
-  // two first-colour and two second-colour updates.  kind 1: a cell of the
-  // domain (updated in its colour phase and stored); kind 2: the wrapped
-  // cell just past a periodic far face of a partial tile (first colour only,
+  // Own columns: pair 0 is row `wave`, pair 1 row `7 - wave` (opposite
+  // parity), each at x = lane and lane + 64.  On plane k one pair is first
+  // colour (updated at step k) and the other second colour (updated at step
+  // k + 1, as the second-colour pass of plane k).  kind 1: a cell of the
+  // domain (updated in its colour phase and stored); kind 2: the wrapped cell
+  // just past a periodic far face of a partial tile (first colour only,
   // recomputed like a ring cell, never stored); kind 0: nothing.
-  int kind[kSwCols], cix[kSwCols], ciy[kSwCols], cgi[kSwCols], cgj[kSwCols];
+  int kind[2][2], ciy[2][2], cgi[2][2], cgj[2][2];
 #pragma unroll
-  for (int q = 0; q < kSwCols; ++q) {
-    const int ix = lane + kWave * (q & 1);
-    const int iy = (q >> 1) == 0 ? wave : kSwTY - 1 - wave;
-    int gi = x0 + ix, gj = y0 + iy;
-    int kd = 0;
-    if (gi < nx && gj < ny) {
-      kd = 1;
-    } else if (gi <= nx && gj <= ny && (gi < nx || perx) &&
-               (gj < ny || pery)) {
-      kd = 2;
-      if (gi == nx) gi = 0;
-      if (gj == ny) gj = 0;
+  for (int pr = 0; pr < 2; ++pr)
+#pragma unroll
+    for (int cl = 0; cl < 2; ++cl) {
+      const int ix = lane + kWave * cl;
+      const int iy = pr == 0 ? wave : kSwTY - 1 - wave;
+      int gi = x0 + ix, gj = y0 + iy;
+      int kd = 0;
+      if (gi < nx && gj < ny) {
+        kd = 1;
+      } else if (gi <= nx && gj <= ny && (gi < nx || perx) &&
+                 (gj < ny || pery)) {
+        kd = 2;
+        if (gi == nx) gi = 0;
+        if (gj == ny) gj = 0;
+      }
+      kind[pr][cl] = kd;
+      ciy[pr][cl] = iy;
+      cgi[pr][cl] = gi;
+      cgj[pr][cl] = gj;
     }
-    kind[q] = kd;
-    cix[q] = ix;
-    ciy[q] = iy;
-    cgi[q] = gi;
-    cgj[q] = gj;
-  }
-  // column parity (of the unwrapped index, equal to the wrapped one: the
+  // parity of pair 0 (of the unwrapped index, equal to the wrapped one: the
   // fused route only runs with even periodic extents)
-  int cpar[kSwCols];
-#pragma unroll
-  for (int q = 0; q < kSwCols; ++q) cpar[q] = (x0 + cix[q] + y0 + ciy[q]) & 1;
-  const int c0 = a.first, c1 = a.first ^ 1;
-  (void)c1;
+  const int par0 = (x0 + lane + y0 + wave) & 1;
+  const int c0 = a.first;
+  // pair that is first colour on plane k
+  auto first_pair = [&](int k) -> int {
+    return ((par0 + k) & 1) == c0 ? 0 : 1;
+  };
   const int kstart = z0 - 1;
   {
     C v[kSwLoadSlots];
@@ -907,98 +909,123 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     fetch_plane(kstart, v);
     store_plane(kstart, v);
   }
-  SwCoef<C> prev[kSwCols];
-#pragma unroll
-  for (int q = 0; q < kSwCols; ++q) prev[q] = SwCoef<C>{};
-  for (int k = kstart; k <= z1; ++k) {
-    // ---- issue every global load of the step
+  // Everything step k needs from global memory, issued one step ahead: x of
+  // plane k + 1, the first-colour pair's coefficients on plane k (its
+  // first-colour update) and on plane k - 1 (its second-colour update), and
+  // the thread's first-colour ring cell on plane k.
+  struct StepLoads {
     C vx[kSwLoadSlots];
-    fetch_plane(k + 1, vx);
-    int kg;
-    const bool plane_ok = plane_of(k, kg);
-    SwCoef<C> cur[kSwCols];
+    SwCoef<C> c0[2];  // plane k
+    SwCoef<C> c1[2];  // plane k - 1
+    SwCoef<C> rc;
+    int rix, riy, rgi, rgj;
+    bool rok;
+  };
+  auto issue = [&](int k, StepLoads& n) {
+    fetch_plane(k + 1, n.vx);
+    int kg, kgb;
+    const bool pk = plane_of(k, kg);
+    const bool pb = plane_of(k - 1, kgb) && k - 1 >= z0 && k - 1 < z1;
+    const int pr = first_pair(k);
 #pragma unroll
-    for (int q = 0; q < kSwCols; ++q)
-      cur[q] = fetch_coef(plane_ok && kind[q] != 0, cgi[q], cgj[q], kg);
+    for (int cl = 0; cl < 2; ++cl) {
+      const int kd = pr == 0 ? kind[0][cl] : kind[1][cl];
+      const int gi = pr == 0 ? cgi[0][cl] : cgi[1][cl];
+      const int gj = pr == 0 ? cgj[0][cl] : cgj[1][cl];
+      n.c0[cl] = fetch_coef(pk && kd != 0, gi, gj, kg);
+      n.c1[cl] = fetch_coef(pb && kd == 1, gi, gj, kgb);
+    }
     // first-colour ring cell of this thread: rows iy = -1 and 8 (65 cells
     // each), columns ix = -1 and 128 (4 cells each)
-    int rix = 0, riy = 0, rgi = 0, rgj = 0;
-    bool rok = false;
-    {
-      const int s = threadIdx.x;
-      if (s < 2 * kSwRingRow) {
-        riy = s < kSwRingRow ? -1 : kSwTY;
-        const int m = s < kSwRingRow ? s : s - kSwRingRow;
-        const int par = (c0 - (x0 - 1 + y0 + riy + k)) & 1;
-        rix = -1 + par + 2 * m;
-      } else if (s < kSwRingCells) {
-        const int t = s - 2 * kSwRingRow;
-        rix = t < kSwTY / 2 ? -1 : kSwTX;
-        const int m = t < kSwTY / 2 ? t : t - kSwTY / 2;
-        const int par = (c0 - (x0 + rix + y0 + k)) & 1;
-        riy = par + 2 * m;
-      }
-      rgi = x0 + rix;
-      rgj = y0 + riy;
-      rok = plane_ok && s < kSwRingCells && rgi >= -1 && rgi <= nx &&
-            rgj >= -1 && rgj <= ny;
-      // a ring position past the domain's far face is the wrapped cell when
-      // the axis is periodic; on a partial tile the far face lies inside
-      // the tile and its wrapped cells are own columns of kind 2
-      if (rgi < 0 || rgi >= nx) {
-        if (perx) rgi = wrapi(rgi, nx); else rok = false;
-      }
-      if (rgj < 0 || rgj >= ny) {
-        if (pery) rgj = wrapi(rgj, ny); else rok = false;
-      }
+    int rix = 0, riy = 0;
+    const int s = threadIdx.x;
+    if (s < 2 * kSwRingRow) {
+      riy = s < kSwRingRow ? -1 : kSwTY;
+      const int m = s < kSwRingRow ? s : s - kSwRingRow;
+      const int par = (c0 - (x0 - 1 + y0 + riy + k)) & 1;
+      rix = -1 + par + 2 * m;
+    } else if (s < kSwRingCells) {
+      const int t = s - 2 * kSwRingRow;
+      rix = t < kSwTY / 2 ? -1 : kSwTX;
+      const int m = t < kSwTY / 2 ? t : t - kSwTY / 2;
+      const int par = (c0 - (x0 + rix + y0 + k)) & 1;
+      riy = par + 2 * m;
     }
-    const SwCoef<C> rc = fetch_coef(rok, rgi, rgj, kg);
-    // a column that is first colour on plane k is second colour on plane
-    // k - 1; the others wait for the next step
-    bool first_here[kSwCols];
-#pragma unroll
-    for (int q = 0; q < kSwCols; ++q) first_here[q] = ((cpar[q] + k) & 1) == c0;
+    int rgi = x0 + rix, rgj = y0 + riy;
+    bool rok = pk && s < kSwRingCells && rgi >= -1 && rgi <= nx &&
+               rgj >= -1 && rgj <= ny;
+    // a ring position past the domain's far face is the wrapped cell when
+    // the axis is periodic; on a partial tile the far face lies inside the
+    // tile and its wrapped cells are own columns of kind 2
+    if (rgi < 0 || rgi >= nx) {
+      if (perx) rgi = wrapi(rgi, nx); else rok = false;
+    }
+    if (rgj < 0 || rgj >= ny) {
+      if (pery) rgj = wrapi(rgj, ny); else rok = false;
+    }
+    n.rix = rix;
+    n.riy = riy;
+    n.rgi = rgi;
+    n.rgj = rgj;
+    n.rok = rok;
+    n.rc = fetch_coef(rok, rgi, rgj, kg);
+  };
+  StepLoads next;
+  issue(kstart, next);
+  for (int k = kstart; k <= z1; ++k) {
+    const StepLoads now = next;
+    if (k < z1) issue(k + 1, next);
+    int kg;
+    const bool plane_ok = plane_of(k, kg);
+    const int pr = first_pair(k);
     __syncthreads();  // plane k-3's buffer is free (read by the last step)
-    store_plane(k + 1, vx);
+    store_plane(k + 1, now.vx);
     __syncthreads();
-    // ---- first colour on plane k: own columns and the ring
+    // ---- first colour on plane k: the first-colour pair and the ring
     if (plane_ok) {
       const C* Pm = buf[(k - 1) & 3];
       C* P0 = buf[k & 3];
       const C* Pp = buf[(k + 1) & 3];
       const bool store = k >= z0 && k < z1;
 #pragma unroll
-      for (int q = 0; q < kSwCols; ++q) {
-        if (kind[q] == 0) continue;
-        if (first_here[q]) {
-          const C xn = update(cur[q], Pm, P0, Pp, cix[q], ciy[q], cgi[q],
-                              cgj[q], kg);
-          if (store && kind[q] == 1)
-            a.xout[cell_index(nx, ny, cgi[q], cgj[q], kg)] = st<T>(xn);
+      for (int cl = 0; cl < 2; ++cl) {
+        const int kd = pr == 0 ? kind[0][cl] : kind[1][cl];
+        if (kd == 0) continue;
+        const int ix = lane + kWave * cl;
+        const int iy = pr == 0 ? ciy[0][cl] : ciy[1][cl];
+        const int gi = pr == 0 ? cgi[0][cl] : cgi[1][cl];
+        const int gj = pr == 0 ? cgj[0][cl] : cgj[1][cl];
+        const C xn = update(now.c0[cl], Pm, P0, Pp, ix, iy, gi, gj, kg);
+        if (store && kd == 1) {
+          const int c = cell_index(nx, ny, gi, gj, kg);
+          a.xout[c] = st<T>(xn);
+          if (a.bout != nullptr) a.bout[c] = st<T>(now.c0[cl].b);
         }
-        if (store && kind[q] == 1 && a.bout != nullptr)
-          a.bout[cell_index(nx, ny, cgi[q], cgj[q], kg)] = st<T>(cur[q].b);
       }
-      if (rok) update(rc, Pm, P0, Pp, rix, riy, rgi, rgj, kg);
+      if (now.rok)
+        update(now.rc, Pm, P0, Pp, now.rix, now.riy, now.rgi, now.rgj, kg);
     }
     __syncthreads();
-    // ---- second colour on plane k - 1 with the coefficients kept from the
-    // last step
+    // ---- second colour on plane k - 1: the same pair
     const int kb = k - 1;
     if (kb >= z0 && kb < z1) {
       const C* Pm = buf[(kb - 1) & 3];
       C* P0 = buf[kb & 3];
       const C* Pp = buf[(kb + 1) & 3];
 #pragma unroll
-      for (int q = 0; q < kSwCols; ++q) {
-        if (kind[q] != 1 || !first_here[q]) continue;
-        const C xn = update(prev[q], Pm, P0, Pp, cix[q], ciy[q], cgi[q],
-                            cgj[q], kb);
-        a.xout[cell_index(nx, ny, cgi[q], cgj[q], kb)] = st<T>(xn);
+      for (int cl = 0; cl < 2; ++cl) {
+        const int kd = pr == 0 ? kind[0][cl] : kind[1][cl];
+        if (kd != 1) continue;
+        const int ix = lane + kWave * cl;
+        const int iy = pr == 0 ? ciy[0][cl] : ciy[1][cl];
+        const int gi = pr == 0 ? cgi[0][cl] : cgi[1][cl];
+        const int gj = pr == 0 ? cgj[0][cl] : cgj[1][cl];
+        const C xn = update(now.c1[cl], Pm, P0, Pp, ix, iy, gi, gj, kb);
+        const int c = cell_index(nx, ny, gi, gj, kb);
+        a.xout[c] = st<T>(xn);
+        if (a.bout != nullptr) a.bout[c] = st<T>(now.c1[cl].b);
       }
     }
-#pragma unroll
-    for (int q = 0; q < kSwCols; ++q) prev[q] = cur[q];
   }
 }
 
