@@ -753,10 +753,12 @@ struct SwCoef {
   C d, xl, xh, yl, yh, zl, zh, b;
 };
 
-// Work of one block per plane: the x plane with its two-cell halo (7 slots
-// per thread), the 4 columns a thread owns, and one first-colour ring cell.
+// Slots per thread of the three per-plane work lists.
 constexpr int kSwLoadSlots = (kSwPlane + kBlock - 1) / kBlock;
-constexpr int kSwRingCells = 2 * (kSwTX + 2) / 2 + 2 * kSwTY / 2;  // 138
+constexpr int kSwRingCells = (kSwTY + 2) * kSwRingRow;
+constexpr int kSwRingSlots = (kSwRingCells + kBlock - 1) / kBlock;
+constexpr int kSwTileCells = kSwTY * (kSwTX / 2);
+constexpr int kSwTileSlots = (kSwTileCells + kBlock - 1) / kBlock;
 
 template <class T>
 __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
@@ -774,12 +776,8 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
   const bool perx = a.bc.m[0] == 2, pery = a.bc.m[2] == 2,
              perz = a.bc.m[4] == 2;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
-  const int plane = nx * ny;
   const C omega = C(0.72);
   const double mean = a.mean != nullptr ? *a.mean : 0.0;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int c0 = a.first;
 
   auto plane_of = [&](int kk, int& kg) -> bool {
     kg = kk;
@@ -788,174 +786,66 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     kg = kk < 0 ? kk + nz : kk - nz;
     return true;
   };
-  // (gi, gj) -> the cell a position refers to: itself inside the domain, the
-  // wrapped cell across a periodic face, none (-1) across any other face
-  auto resolve = [&](int& gi, int& gj) -> bool {
-    bool ok = true;
-    if (gi < 0 || gi >= nx) {
-      if (perx) gi = wrapi(gi, nx); else ok = false;
-    }
-    if (gj < 0 || gj >= ny) {
-      if (pery) gj = wrapi(gj, ny); else ok = false;
-    }
-    return ok;
-  };
-
-  // ---- per-thread geometry, fixed for every plane of the block ----------
-  // x-plane slots (tile plus a two-cell halo): in-plane offset or -1
-  int xoff[kSwLoadSlots];
-#pragma unroll
-  for (int s = 0; s < kSwLoadSlots; ++s) {
-    const int p = threadIdx.x + s * kBlock;
-    const int iy = p / kSwPX;
-    const int ix = p - iy * kSwPX - 2;
-    int gi = x0 + ix, gj = y0 + iy - 2;
-    // only positions within two cells of the domain are ever read
-    bool ok = p < kSwPlane && gi >= -2 && gi <= nx + 1 && gj >= -2 &&
-              gj <= ny + 1;
-    ok = resolve(gi, gj) && ok;
-    xoff[s] = ok ? gi + nx * gj : -1;
-  }
-  // Own columns: pair 0 is row `wave`, pair 1 row `7 - wave` (opposite
-  // parity), each at x = lane and lane + 64.  On plane k one pair is first
-  // colour (updated at step k) and the other second colour (updated at step
-  // k + 1 as the second-colour pass of plane k).  kind 1: a cell of the
-  // domain (updated in its colour phase and stored); kind 2: the wrapped
-  // cell just past a periodic far face of a partial tile (first colour only,
-  // recomputed like a ring cell, never stored); kind 0: nothing.
-  int kind[2][2], ciy[2][2], cgi[2][2], cgj[2][2];
-#pragma unroll
-  for (int pr = 0; pr < 2; ++pr)
-#pragma unroll
-    for (int cl = 0; cl < 2; ++cl) {
-      const int ix = lane + kWave * cl;
-      const int iy = pr == 0 ? wave : kSwTY - 1 - wave;
-      int gi = x0 + ix, gj = y0 + iy;
-      int kd = 0;
-      if (gi < nx && gj < ny) {
-        kd = 1;
-      } else if (gi <= nx && gj <= ny && (gi < nx || perx) &&
-                 (gj < ny || pery)) {
-        kd = 2;
-        if (gi == nx) gi = 0;
-        if (gj == ny) gj = 0;
-      }
-      kind[pr][cl] = kd;
-      ciy[pr][cl] = iy;
-      cgi[pr][cl] = kd != 0 ? gi : 0;
-      cgj[pr][cl] = kd != 0 ? gj : 0;
-    }
-  // The thread's first-colour ring cell for even (e = 0) and odd (e = 1)
-  // planes: rows iy = -1 and 8 (65 cells each), columns ix = -1 and 128
-  // (4 cells each).  A ring position past the domain's far face is the
-  // wrapped cell when the axis is periodic; on a partial tile the far face
-  // lies inside the tile and its wrapped cells are own columns of kind 2.
-  struct Ring {
-    int ix, iy, gi, gj;
-    bool ok;
-  };
-  auto ring_of = [&](int e) -> Ring {
-    int ix = 0, iy = 0;
-    const int s = threadIdx.x;
-    if (s < 2 * kSwRingRow) {
-      iy = s < kSwRingRow ? -1 : kSwTY;
-      const int m = s < kSwRingRow ? s : s - kSwRingRow;
-      const int par = (c0 - (x0 - 1 + y0 + iy + e)) & 1;
-      ix = -1 + par + 2 * m;
-    } else if (s < kSwRingCells) {
-      const int t = s - 2 * kSwRingRow;
-      ix = t < kSwTY / 2 ? -1 : kSwTX;
-      const int m = t < kSwTY / 2 ? t : t - kSwTY / 2;
-      const int par = (c0 - (x0 + ix + y0 + e)) & 1;
-      iy = par + 2 * m;
-    }
-    int gi = x0 + ix, gj = y0 + iy;
-    bool ok = s < kSwRingCells && gi >= -1 && gi <= nx && gj >= -1 &&
-              gj <= ny;
-    ok = resolve(gi, gj) && ok;
-    return Ring{ix, iy, ok ? gi : 0, ok ? gj : 0, ok};
-  };
-  // plain scalars (a select between array elements would keep the arrays
-  // in scratch memory)
-  const Ring ring0 = ring_of(0), ring1 = ring_of(1);
-  const int rix0 = ring0.ix, riy0 = ring0.iy, rgi0 = ring0.gi,
-            rgj0 = ring0.gj;
-  const int rix1 = ring1.ix, riy1 = ring1.iy, rgi1 = ring1.gi,
-            rgj1 = ring1.gj;
-  const bool rok0 = ring0.ok, rok1 = ring1.ok;
-  // parity of pair 0 (of the unwrapped index, equal to the wrapped one: the
-  // fused route only runs with even periodic extents)
-  const int par0 = (x0 + lane + y0 + wave) & 1;
-
-  // ---- loads -------------------------------------------------------------
-  // Loads keep the raw stored values; masks, widening and the projection
-  // are applied where the values are used, so issuing the loads of a step
-  // never waits on them.
-  auto fetch_plane = [&](int kk, T (&v)[kSwLoadSlots]) {
+  // Every global load of one plane step is issued before any of its values
+  // is used, so a step waits on memory once rather than once per cell.
+  auto fetch_plane = [&](int kk, C (&v)[kSwLoadSlots]) {
     int kg;
     const bool valid = plane_of(kk, kg) && !a.from_zero;
-    const int base = valid ? plane * kg : 0;
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
-      const bool ok = valid && xoff[s] >= 0;
-      v[s] = L.x[ok ? base + xoff[s] : 0];
+      const int p = threadIdx.x + s * kBlock;
+      const int iy = p / kSwPX;
+      const int ix = p - iy * kSwPX - 2;
+      int gi = x0 + ix, gj = y0 + iy - 2;
+      // only positions within two cells of the domain are ever read
+      bool ok = valid && p < kSwPlane && gi >= -2 && gi <= nx + 1 &&
+                gj >= -2 && gj <= ny + 1;
+      if (gi < 0 || gi >= nx) {
+        if (perx) gi = wrapi(gi, nx); else ok = false;
+      }
+      if (gj < 0 || gj >= ny) {
+        if (pery) gj = wrapi(gj, ny); else ok = false;
+      }
+      const int c = ok ? cell_index(nx, ny, gi, gj, kg) : 0;
+      const C x = C(ld(L.x[c]));
+      v[s] = ok ? x : C(0);
     }
   };
-  auto store_plane = [&](int kk, const T (&v)[kSwLoadSlots]) {
-    int kg;
-    const bool valid = plane_of(kk, kg) && !a.from_zero;
+  auto store_plane = [&](int kk, const C (&v)[kSwLoadSlots]) {
     C* P = buf[kk & 3];
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
       const int p = threadIdx.x + s * kBlock;
-      const bool ok = valid && xoff[s] >= 0;
-      if (p < kSwPlane) P[p] = ok ? C(ld(v[s])) : C(0);
+      if (p < kSwPlane) P[p] = v[s];
     }
   };
-  // coefficients of the update at (gi, gj, kg); index 0 stands in for an
-  // inactive slot so every load address is in bounds
-  struct RawCoef {
-    T d, xl, xh, yl, yh, zl, zh, b;
-    uint8_t act;
+  auto bval = [&](int c) -> C {
+    if (a.mean == nullptr) return C(ld(L.b[c]));
+    const T projected = st<T>(static_cast<double>(ld(L.b[c])) - mean);
+    return L.act[c] ? C(ld(projected)) : C(0);
   };
-  auto fetch_coef = [&](bool ok, int gi, int gj, int kg) -> RawCoef {
-    const int c = ok ? gi + nx * gj + plane * kg : 0;
+  // coefficients of the update at (gi, gj, kg); c = 0 stands in for an
+  // inactive slot so every load address is in bounds
+  auto fetch_coef = [&](bool ok, int gi, int gj, int kg) -> SwCoef<C> {
+    const int c = ok ? cell_index(nx, ny, gi, gj, kg) : 0;
     const int row = ok ? gj + ny * kg : 0;
     const int fyo = ok ? c + nx * kg : 0;
-    RawCoef f;
-    f.d = L.D[c];
-    f.xl = L.fx[c + row];
-    f.xh = L.fx[c + row + 1];
-    f.yl = L.fy[fyo];
-    f.yh = L.fy[fyo + nx];
-    f.zl = L.fz[c];
-    f.zh = L.fz[c + plane];
-    f.b = L.b[c];
-    f.act = a.mean != nullptr ? L.act[c] : uint8_t(1);
-    return f;
-  };
-  auto widen = [&](const RawCoef& r) -> SwCoef<C> {
     SwCoef<C> f;
-    f.d = C(ld(r.d));
-    f.xl = C(ld(r.xl));
-    f.xh = C(ld(r.xh));
-    f.yl = C(ld(r.yl));
-    f.yh = C(ld(r.yh));
-    f.zl = C(ld(r.zl));
-    f.zh = C(ld(r.zh));
-    if (a.mean == nullptr) {
-      f.b = C(ld(r.b));
-    } else {
-      const T projected = st<T>(static_cast<double>(ld(r.b)) - mean);
-      f.b = r.act ? C(ld(projected)) : C(0);
-    }
+    f.d = C(ld(L.D[c]));
+    f.xl = C(ld(L.fx[c + row]));
+    f.xh = C(ld(L.fx[c + row + 1]));
+    f.yl = C(ld(L.fy[fyo]));
+    f.yh = C(ld(L.fy[fyo + nx]));
+    f.zl = C(ld(L.fz[c]));
+    f.zh = C(ld(L.fz[c + nx * ny]));
+    f.b = bval(c);
     return f;
   };
   auto mode = [](int m, C self) -> C { return m == 0 ? C(0) : self; };
   // x <- x + omega (b - A x) / D at one cell held in LDS, in the reference
-  // evaluation order; returns the new value
+  // evaluation order
   auto update = [&](const SwCoef<C>& f, const C* Pm, C* P0, const C* Pp,
-                    int ix, int iy, int gi, int gj, int kg) -> C {
+                    int ix, int iy, int gi, int gj, int kg) {
     const int q = (ix + 2) + kSwPX * (iy + 2);
     const C xc = P0[q];
     const C xl = (gi > 0 || perx) ? P0[q - 1] : mode(a.bc.m[0], xc);
@@ -971,116 +861,111 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     v -= f.yh * yh;
     v -= f.zl * zl;
     v -= f.zh * zh;
-    const C xn = xc + omega * (f.b - v) / f.d;
-    P0[q] = xn;
-    return xn;
+    P0[q] = xc + omega * (f.b - v) / f.d;
   };
 
-  // Everything step k needs from global memory, issued one step ahead: x of
-  // plane k + 1, the first-colour pair's coefficients on plane k (its
-  // first-colour update) and on plane k - 1 (its second-colour update), and
-  // the thread's first-colour ring cell on plane k.
-  struct StepLoads {
-    T vx[kSwLoadSlots];
-    RawCoef c0[2];  // plane k
-    RawCoef c1[2];  // plane k - 1
-    RawCoef rc;
-  };
-  auto first_pair = [&](int k) -> int {
-    return ((par0 + k) & 1) == c0 ? 0 : 1;
-  };
-  auto issue = [&](int k, StepLoads& n) {
-    fetch_plane(k + 1, n.vx);
-    int kg, kgb;
-    const bool pk = plane_of(k, kg);
-    const bool pb = plane_of(k - 1, kgb) && k - 1 >= z0 && k - 1 < z1;
-    const int pr = first_pair(k);
-#pragma unroll
-    for (int cl = 0; cl < 2; ++cl) {
-      const int kd = pr == 0 ? kind[0][cl] : kind[1][cl];
-      const int gi = pr == 0 ? cgi[0][cl] : cgi[1][cl];
-      const int gj = pr == 0 ? cgj[0][cl] : cgj[1][cl];
-      n.c0[cl] = fetch_coef(pk && kd != 0, gi, gj, kg);
-      n.c1[cl] = fetch_coef(pb && kd == 1, gi, gj, kgb);
-    }
-    const int e = k & 1;
-    n.rc = fetch_coef(pk && (e ? rok1 : rok0), e ? rgi1 : rgi0,
-                      e ? rgj1 : rgj0, kg);
-  };
-  auto compute = [&](int k, const StepLoads& now) {
-    int kg;
-    const bool plane_ok = plane_of(k, kg);
-    const int pr = first_pair(k);
-    __syncthreads();  // plane k-3's buffer is free (read by the last step)
-    store_plane(k + 1, now.vx);
-    __syncthreads();
-    // ---- first colour on plane k: the first-colour pair and the ring
-    if (plane_ok) {
-      const C* Pm = buf[(k - 1) & 3];
-      C* P0 = buf[k & 3];
-      const C* Pp = buf[(k + 1) & 3];
-      const bool store = k >= z0 && k < z1;
-#pragma unroll
-      for (int cl = 0; cl < 2; ++cl) {
-        const int kd = pr == 0 ? kind[0][cl] : kind[1][cl];
-        if (kd == 0) continue;
-        const int iy = pr == 0 ? ciy[0][cl] : ciy[1][cl];
-        const int gi = pr == 0 ? cgi[0][cl] : cgi[1][cl];
-        const int gj = pr == 0 ? cgj[0][cl] : cgj[1][cl];
-        const SwCoef<C> f = widen(now.c0[cl]);
-        const C xn = update(f, Pm, P0, Pp, lane + kWave * cl, iy, gi, gj, kg);
-        if (store && kd == 1) {
-          const int c = cell_index(nx, ny, gi, gj, kg);
-          a.xout[c] = st<T>(xn);
-          if (a.bout != nullptr) a.bout[c] = st<T>(f.b);
-        }
-      }
-      const int e = k & 1;
-      if (e ? rok1 : rok0)
-        update(widen(now.rc), Pm, P0, Pp, e ? rix1 : rix0, e ? riy1 : riy0,
-               e ? rgi1 : rgi0, e ? rgj1 : rgj0, kg);
-    }
-    __syncthreads();
-    // ---- second colour on plane k - 1: the same pair
-    const int kb = k - 1;
-    if (kb >= z0 && kb < z1) {
-      const C* Pm = buf[(kb - 1) & 3];
-      C* P0 = buf[kb & 3];
-      const C* Pp = buf[(kb + 1) & 3];
-#pragma unroll
-      for (int cl = 0; cl < 2; ++cl) {
-        const int kd = pr == 0 ? kind[0][cl] : kind[1][cl];
-        if (kd != 1) continue;
-        const int iy = pr == 0 ? ciy[0][cl] : ciy[1][cl];
-        const int gi = pr == 0 ? cgi[0][cl] : cgi[1][cl];
-        const int gj = pr == 0 ? cgj[0][cl] : cgj[1][cl];
-        const SwCoef<C> f = widen(now.c1[cl]);
-        const C xn = update(f, Pm, P0, Pp, lane + kWave * cl, iy, gi, gj, kb);
-        const int c = cell_index(nx, ny, gi, gj, kb);
-        a.xout[c] = st<T>(xn);
-        if (a.bout != nullptr) a.bout[c] = st<T>(f.b);
-      }
-    }
-  };
-
+  const int c0 = a.first, c1 = a.first ^ 1;
   const int kstart = z0 - 1;
   {
-    T v[kSwLoadSlots];
+    C v[kSwLoadSlots];
     fetch_plane(kstart - 1, v);
     store_plane(kstart - 1, v);
     fetch_plane(kstart, v);
     store_plane(kstart, v);
   }
-  // two step buffers in alternation: step k computes from one while the
-  // loads of step k + 1 land in the other
-  StepLoads s0, s1;
-  issue(kstart, s0);
-  for (int k = kstart; k <= z1; k += 2) {
-    if (k + 1 <= z1) issue(k + 1, s1);
-    compute(k, s0);
-    if (k + 1 > z1) break;
-    if (k + 2 <= z1) issue(k + 2, s0);
-    compute(k + 1, s1);
+  for (int k = kstart; k <= z1; ++k) {
+    // ---- issue: x of plane k+1, coefficients of both colour phases
+    C vx[kSwLoadSlots];
+    fetch_plane(k + 1, vx);
+    int kg;
+    const bool first_plane = plane_of(k, kg);
+    SwCoef<C> f0[kSwRingSlots];
+    int ix0[kSwRingSlots], iy0[kSwRingSlots], gi0[kSwRingSlots],
+        gj0[kSwRingSlots];
+    bool ok0[kSwRingSlots];
+#pragma unroll
+    for (int s = 0; s < kSwRingSlots; ++s) {
+      // first colour on the tile and its one-cell ring
+      const int q = threadIdx.x + s * kBlock;
+      const int r = q / kSwRingRow;
+      const int m = q - r * kSwRingRow;
+      const int iy = r - 1;
+      int gj = y0 + iy;
+      // parity with the unwrapped index equals the wrapped one: the fused
+      // route only runs with even periodic extents
+      const int par = (c0 - (x0 - 1 + gj + kg)) & 1;
+      const int ix = -1 + par + 2 * m;
+      int gi = x0 + ix;
+      // ring cells beyond a periodic face are the wrapped cells
+      bool ok = first_plane && q < kSwRingCells && gi >= -1 && gi <= nx &&
+                gj >= -1 && gj <= ny;
+      if (gi < 0 || gi >= nx) {
+        if (perx) gi = wrapi(gi, nx); else ok = false;
+      }
+      if (gj < 0 || gj >= ny) {
+        if (pery) gj = wrapi(gj, ny); else ok = false;
+      }
+      ok0[s] = ok;
+      ix0[s] = ix;
+      iy0[s] = iy;
+      gi0[s] = gi;
+      gj0[s] = gj;
+      f0[s] = fetch_coef(ok, gi, gj, kg);
+    }
+    const int kb = k - 1;
+    const bool second_plane = kb >= z0 && kb < z1;
+    SwCoef<C> f1[kSwTileSlots];
+    int ix1[kSwTileSlots], iy1[kSwTileSlots];
+    bool ok1[kSwTileSlots];
+#pragma unroll
+    for (int s = 0; s < kSwTileSlots; ++s) {
+      // second colour on the tile, one plane behind
+      const int q = threadIdx.x + s * kBlock;
+      const int iy = q / (kSwTX / 2);
+      const int m = q - iy * (kSwTX / 2);
+      const int gj = y0 + iy;
+      const int par = (c1 - (x0 + gj + kb)) & 1;
+      const int ix = par + 2 * m;
+      const int gi = x0 + ix;
+      const bool ok = second_plane && q < kSwTileCells && gi < nx && gj < ny;
+      ok1[s] = ok;
+      ix1[s] = ix;
+      iy1[s] = iy;
+      f1[s] = fetch_coef(ok, gi, gj, kb);
+    }
+    // ---- x of plane k+1 into LDS
+    store_plane(k + 1, vx);
+    __syncthreads();
+    if (first_plane) {
+      const C* Pm = buf[(k - 1) & 3];
+      C* P0 = buf[k & 3];
+      const C* Pp = buf[(k + 1) & 3];
+#pragma unroll
+      for (int s = 0; s < kSwRingSlots; ++s)
+        if (ok0[s])
+          update(f0[s], Pm, P0, Pp, ix0[s], iy0[s], gi0[s], gj0[s], kg);
+    }
+    __syncthreads();
+    if (second_plane) {
+      const C* Pm = buf[(kb - 1) & 3];
+      C* P0 = buf[kb & 3];
+      const C* Pp = buf[(kb + 1) & 3];
+#pragma unroll
+      for (int s = 0; s < kSwTileSlots; ++s)
+        if (ok1[s])
+          update(f1[s], Pm, P0, Pp, ix1[s], iy1[s], x0 + ix1[s], y0 + iy1[s],
+                 kb);
+      __syncthreads();
+      for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kBlock) {
+        const int iy = p / kSwTX;
+        const int ix = p - iy * kSwTX;
+        const int gi = x0 + ix, gj = y0 + iy;
+        if (gi >= nx || gj >= ny) continue;
+        const int c = cell_index(nx, ny, gi, gj, kb);
+        a.xout[c] = st<T>(P0[(ix + 2) + kSwPX * (iy + 2)]);
+        if (a.bout != nullptr) a.bout[c] = st<T>(bval(c));
+      }
+    }
   }
 }
 
