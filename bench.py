@@ -79,6 +79,11 @@ def measured_traffic(workload_tag, cls, precision, grid):
     with open(path) as f:
         data = json.load(f)
     entry = data["kernels"].get(f"{name}|{grid}")
+    if entry is None and cls == "sweep_finest":
+        # the fused sweep runs on the finest level only: one launch geometry
+        hits = [v for k, v in data["kernels"].items()
+                if k.split("|")[0] == name]
+        entry = hits[0] if len(hits) == 1 else None
     if entry is None:
         return None
     return int(entry["fetch_bytes"] + entry["write_bytes"])
@@ -370,12 +375,7 @@ def main():
             work = ((nx + 1) // 2) * ny * nz
             grid = -(-work // 256) * 256
         elif dominant == "sweep_finest":
-            # k_rb_sweep: 128 x 8 tiles, planes per block as the library
-            # chooses them
-            per_plane = -(-nx // 128) * -(-ny // 8)
-            want = max(1, 2048 // per_plane)
-            zc = min(32, max(4, -(-nz // want)))
-            grid = per_plane * -(-nz // zc) * 256
+            grid = None  # looked up by kernel name (one launch geometry)
         else:
             grid = -(-cells // 256) * 256
         tag = f"{args.workload}_{nx}x{ny}x{nz}_{main_prec}"

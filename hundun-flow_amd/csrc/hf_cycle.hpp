@@ -728,7 +728,8 @@ namespace hf {
 // constant projection (`mean`) of b is applied on the fly and stored to bout.
 // ---------------------------------------------------------------------------
 constexpr int kSwTX = 128;
-constexpr int kSwTY = 8;
+constexpr int kSwTY = 16;
+constexpr int kSwThreads = 512;  // threads per block of the fused sweep
 constexpr int kSwPX = kSwTX + 4;
 constexpr int kSwPY = kSwTY + 4;
 constexpr int kSwPlane = kSwPX * kSwPY;
@@ -754,14 +755,14 @@ struct SwCoef {
 };
 
 // Slots per thread of the three per-plane work lists.
-constexpr int kSwLoadSlots = (kSwPlane + kBlock - 1) / kBlock;
+constexpr int kSwLoadSlots = (kSwPlane + kSwThreads - 1) / kSwThreads;
 constexpr int kSwRingCells = (kSwTY + 2) * kSwRingRow;
-constexpr int kSwRingSlots = (kSwRingCells + kBlock - 1) / kBlock;
+constexpr int kSwRingSlots = (kSwRingCells + kSwThreads - 1) / kSwThreads;
 constexpr int kSwTileCells = kSwTY * (kSwTX / 2);
-constexpr int kSwTileSlots = (kSwTileCells + kBlock - 1) / kBlock;
+constexpr int kSwTileSlots = (kSwTileCells + kSwThreads - 1) / kSwThreads;
 
 template <class T>
-__global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
+__global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   using C = typename Compute<T>::type;
   __shared__ C buf[4][kSwPlane];
   const Lv<T>& L = a.L;
@@ -793,7 +794,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     const bool valid = plane_of(kk, kg) && !a.from_zero;
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
-      const int p = threadIdx.x + s * kBlock;
+      const int p = threadIdx.x + s * kSwThreads;
       const int iy = p / kSwPX;
       const int ix = p - iy * kSwPX - 2;
       int gi = x0 + ix, gj = y0 + iy - 2;
@@ -815,7 +816,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
     C* P = buf[kk & 3];
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
-      const int p = threadIdx.x + s * kBlock;
+      const int p = threadIdx.x + s * kSwThreads;
       if (p < kSwPlane) P[p] = v[s];
     }
   };
@@ -886,7 +887,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
 #pragma unroll
     for (int s = 0; s < kSwRingSlots; ++s) {
       // first colour on the tile and its one-cell ring
-      const int q = threadIdx.x + s * kBlock;
+      const int q = threadIdx.x + s * kSwThreads;
       const int r = q / kSwRingRow;
       const int m = q - r * kSwRingRow;
       const int iy = r - 1;
@@ -920,7 +921,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
 #pragma unroll
     for (int s = 0; s < kSwTileSlots; ++s) {
       // second colour on the tile, one plane behind
-      const int q = threadIdx.x + s * kBlock;
+      const int q = threadIdx.x + s * kSwThreads;
       const int iy = q / (kSwTX / 2);
       const int m = q - iy * (kSwTX / 2);
       const int gj = y0 + iy;
@@ -956,7 +957,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(SweepArgs<T> a) {
           update(f1[s], Pm, P0, Pp, ix1[s], iy1[s], x0 + ix1[s], y0 + iy1[s],
                  kb);
       __syncthreads();
-      for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kBlock) {
+      for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kSwThreads) {
         const int iy = p / kSwTX;
         const int ix = p - iy * kSwTX;
         const int gi = x0 + ix, gj = y0 + iy;

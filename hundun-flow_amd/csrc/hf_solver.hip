@@ -593,7 +593,7 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     a.gy = L.sweep_gy;
     {
       HF_PROF(p, rb_class);
-      HF_LAUNCH(k_rb_sweep<T>, L.sweep_blocks, kBlock, 0, p->stream, a);
+      HF_LAUNCH(k_rb_sweep<T>, L.sweep_blocks, kSwThreads, 0, p->stream, a);
     }
     std::swap(L.x, L.x2);
     if (a.bout != nullptr) std::swap(L.b, L.b2);
@@ -1016,14 +1016,15 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tcol = make_tiles((L.n[0] + 1) / 2, L.n[1], L.n[2], false);
     L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
     // fused sweeps on grid levels with the fused colour route and rows long
-    // enough for the 128 x 8 tiles to stay mostly busy
+    // enough for the tiles to stay mostly busy
     L.sweep = l == 0 && !L.tail && L.line_mask == 0 && L.fused &&
               L.n[0] >= 32 && L.n[1] >= 8 && sweep_enabled();
     if (L.sweep) {
       L.sweep_gx = (L.n[0] + kSwTX - 1) / kSwTX;
       L.sweep_gy = (L.n[1] + kSwTY - 1) / kSwTY;
       const int per_plane = L.sweep_gx * L.sweep_gy;
-      const int want = std::max(1, 2048 / per_plane);
+      // about half a million threads in flight, split into z-chunks
+      const int want = std::max(1, (1 << 19) / (kSwThreads * per_plane));
       L.sweep_zc = std::min(32, std::max(4, (L.n[2] + want - 1) / want));
       L.sweep_blocks = per_plane * ((L.n[2] + L.sweep_zc - 1) / L.sweep_zc);
     }
