@@ -432,6 +432,12 @@ bool line_relaxation_enabled() {
   return env != nullptr && std::atoi(env) != 0;
 }
 
+// HF_SWEEP_COLUMNS=0 keeps the tile schedule for every storage type.
+bool sweep_columns() {
+  const char* env = std::getenv("HF_SWEEP_COLUMNS");
+  return env == nullptr || std::atoi(env) != 0;
+}
+
 int64_t tail_threshold() {
   const char* env = std::getenv("HF_TAIL_CELLS");
   if (env != nullptr) {
@@ -593,7 +599,14 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     a.gy = L.sweep_gy;
     {
       HF_PROF(p, rb_class);
-      HF_LAUNCH(k_rb_sweep<T>, L.sweep_blocks, kSwThreads, 0, p->stream, a);
+      // 32- and 16-bit storage: the column-owner schedule (one fetch of
+      // every coefficient per sweep); FP64 keeps the tile schedule, whose
+      // register footprint is half as large
+      if (sizeof(T) < sizeof(double) && sweep_columns())
+        HF_LAUNCH(k_rb_sweep_cols<T>, L.sweep_blocks, kSwThreads, 0,
+                  p->stream, a);
+      else
+        HF_LAUNCH(k_rb_sweep<T>, L.sweep_blocks, kSwThreads, 0, p->stream, a);
     }
     std::swap(L.x, L.x2);
     if (a.bout != nullptr) std::swap(L.b, L.b2);

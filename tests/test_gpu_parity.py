@@ -269,3 +269,22 @@ def test_mixed_precision_reaches_tolerance(torch_cuda, monkeypatch, precision,
     assert diff <= 1e-6
     assert np.all(xg[act == 0] == 0.0)
     plan.close()
+
+
+@pytest.mark.parametrize("name", ["mixed_wide", "periodic_xy", "sphere32"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_fused_sweep_schedules_agree(torch_cuda, monkeypatch, name, precision):
+    """The column-owner and the tile schedule of the fused finest-level
+    sweep perform the same updates in the same arithmetic order: one cycle
+    under each gives the same correction bit for bit."""
+    torch = torch_cuda
+    _say(f"[sweep schedules] case={name} precision={precision}")
+    case = CASES[name]()
+    plan = _gpu_plan(torch, case, **_policy(name, precision=precision))
+    r = _t(torch, np.random.default_rng(8).standard_normal(plan.cells))
+    monkeypatch.setenv("HF_SWEEP_COLUMNS", "1")
+    z_cols = plan.mg_apply(r).cpu().numpy()
+    monkeypatch.setenv("HF_SWEEP_COLUMNS", "0")
+    z_tile = plan.mg_apply(r).cpu().numpy()
+    assert np.array_equal(z_cols, z_tile), np.max(np.abs(z_cols - z_tile))
+    plan.close()
