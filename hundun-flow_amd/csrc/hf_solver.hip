@@ -432,10 +432,14 @@ bool line_relaxation_enabled() {
   return env != nullptr && std::atoi(env) != 0;
 }
 
-// HF_SWEEP_COLUMNS=0 keeps the tile schedule for every storage type.
+// HF_SWEEP_COLUMNS=1 selects the column-owner schedule of the fused sweep
+// for 32- and 16-bit storage.  It fetches every coefficient once per sweep
+// (half the tile schedule's HBM traffic) but, at two waves per SIMD, waits
+// on its plane steps longer than the tile schedule saves: 0.42 against
+// 0.33 ms per 512x256x256 FP32 sweep.
 bool sweep_columns() {
   const char* env = std::getenv("HF_SWEEP_COLUMNS");
-  return env == nullptr || std::atoi(env) != 0;
+  return env != nullptr && std::atoi(env) != 0;
 }
 
 int64_t tail_threshold() {
@@ -599,9 +603,8 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     a.gy = L.sweep_gy;
     {
       HF_PROF(p, rb_class);
-      // 32- and 16-bit storage: the column-owner schedule (one fetch of
-      // every coefficient per sweep); FP64 keeps the tile schedule, whose
-      // register footprint is half as large
+      // the column-owner schedule exists for 32- and 16-bit storage only
+      // (its FP64 register footprint would leave one wave per SIMD)
       if (sizeof(T) < sizeof(double) && sweep_columns())
         HF_LAUNCH(k_rb_sweep_cols<T>, L.sweep_blocks, kSwThreads, 0,
                   p->stream, a);
