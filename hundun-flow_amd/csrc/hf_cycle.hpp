@@ -131,21 +131,45 @@ __global__ void k_copy(Chunk ch, double* dst, const double* src) {
 // Level 0 recomputes its fluid volumes from the axis widths and the activity
 // byte, (w_x*w_y)*w_z exactly as the hierarchy stored them.
 // ---------------------------------------------------------------------------
+// Reduction-only kernels take kRedPer consecutive tiles per block: the loads
+// of all of them are issued before any is used and one block sum serves
+// kRedPer * 256 cells (these kernels move only a few bytes per cell).
+constexpr int kRedPer = 8;
+
+// Cell of tile slot u of block lb in a kRedPer launch; ok = inside the level.
+__device__ __forceinline__ int red_cell(const Tiles& t, int lb, int u, int& i,
+                                        int& j, int& k, bool& ok) {
+  const int tile = lb * kRedPer + u;
+  tile_xyz(t, tile < t.count ? tile : 0, i, j, k);
+  ok = tile < t.count && i < t.nx && j < t.ny;
+  if (!ok) {
+    i = 0;
+    j = 0;
+    k = 0;
+  }
+  return ok ? cell_index(t.nx, t.ny, i, j, k) : 0;
+}
+
 template <class T>
 __global__ void k_vol_dot_finest(Tiles t, const T* field, const uint8_t* act,
                                  Widths w, double* partials) {
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
-  double v = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
+  double f[kRedPer], vol[kRedPer];
+  bool use[kRedPer];
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
     int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
-    if (act[c]) v += (w.x[i] * w.y[j] * w.z[k]) * static_cast<double>(ld(field[c]));
+    bool ok;
+    const int c = red_cell(t, lb, u, i, j, k, ok);
+    use[u] = ok && act[c] != 0;
+    f[u] = static_cast<double>(ld(field[c]));
+    vol[u] = w.x[i] * w.y[j] * w.z[k];
   }
+  double v = 0.0;
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u)
+    if (use[u]) v += vol[u] * f[u];
   v = block_sum(v, scratch);
   if (threadIdx.x == 0) partials[lb] = v;
 }
@@ -155,16 +179,21 @@ __global__ void k_vol_dot(Tiles t, const T* field, const double* vol,
                           double* partials) {
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
-  double v = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
+  double f[kRedPer], vv[kRedPer];
+  bool use[kRedPer];
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
     int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
-    v += vol[c] * static_cast<double>(ld(field[c]));
+    bool ok;
+    const int c = red_cell(t, lb, u, i, j, k, ok);
+    use[u] = ok;
+    f[u] = static_cast<double>(ld(field[c]));
+    vv[u] = vol[c];
   }
+  double v = 0.0;
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u)
+    if (use[u]) v += vv[u] * f[u];
   v = block_sum(v, scratch);
   if (threadIdx.x == 0) partials[lb] = v;
 }
@@ -193,20 +222,26 @@ __global__ void k_mg_entry(Tiles t, const double* r, const uint8_t* act,
                            Widths w, T* b, T* x, double* partials) {
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
-  double v = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
+  double rv[kRedPer], vol[kRedPer];
+  bool ok[kRedPer], fluid[kRedPer];
+  int cc[kRedPer];
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
     int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
-    const bool a = act[c] != 0;
-    const T bv = a ? st<T>(r[c]) : st<T>(0.0);
-    b[c] = bv;
-    if (x != nullptr) x[c] = st<T>(0.0);
-    if (partials != nullptr && a)
-      v += (w.x[i] * w.y[j] * w.z[k]) * static_cast<double>(ld(bv));
+    cc[u] = red_cell(t, lb, u, i, j, k, ok[u]);
+    fluid[u] = act[cc[u]] != 0;
+    rv[u] = r[cc[u]];
+    vol[u] = w.x[i] * w.y[j] * w.z[k];
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
+    if (!ok[u]) continue;
+    const bool a = fluid[u];
+    const T bv = a ? st<T>(rv[u]) : st<T>(0.0);
+    b[cc[u]] = bv;
+    if (x != nullptr) x[cc[u]] = st<T>(0.0);
+    if (partials != nullptr && a) v += vol[u] * static_cast<double>(ld(bv));
   }
   if (partials != nullptr) {
     v = block_sum(v, scratch);
@@ -436,25 +471,34 @@ __global__ void k_mg_exit(Tiles t, const T* x, const uint8_t* act,
                           int stride) {
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
   const double a = alpha == nullptr ? 1.0 : *alpha;
   const double m = mean == nullptr ? 0.0 : *mean;
-  double rz = 0.0, zq = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
+  T xs[kRedPer];
+  double rv[kRedPer], qv[kRedPer];
+  bool ok[kRedPer], fluid[kRedPer];
+  int cc[kRedPer];
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
     int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
+    cc[u] = red_cell(t, lb, u, i, j, k, ok[u]);
+    fluid[u] = act[cc[u]] != 0;
+    xs[u] = x[cc[u]];
+    rv[u] = r != nullptr ? r[cc[u]] : 0.0;
+    qv[u] = q != nullptr ? q[cc[u]] : 0.0;
+  }
+  double rz = 0.0, zq = 0.0;
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
+    if (!ok[u]) continue;
     double v = 0.0;
-    if (act[c]) {
-      double xv = static_cast<double>(ld(x[c]));
+    if (fluid[u]) {
+      double xv = static_cast<double>(ld(xs[u]));
       if (mean != nullptr) xv = static_cast<double>(ld(st<T>(xv - m)));
       v = xv * a;
     }
-    z[c] = v;
-    if (r != nullptr) rz += r[c] * v;
-    if (q != nullptr) zq += v * q[c];
+    z[cc[u]] = v;
+    if (r != nullptr) rz += rv[u] * v;
+    if (q != nullptr) zq += v * qv[u];
   }
   if (partials != nullptr) {
     rz = block_sum(rz, scratch);
@@ -473,16 +517,23 @@ __global__ void k_mg_proj3(Tiles t, const double* rin, const uint8_t* act,
                            const T* rec, double* partials, int stride) {
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
-  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
+  double rv[kRedPer];
+  T recv[kRedPer];
+  bool ok[kRedPer], fluid[kRedPer];
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
     int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
-    const double r = act[c] ? rin[c] : 0.0;
-    const double az = r - static_cast<double>(ld(rec[c]));
+    const int c = red_cell(t, lb, u, i, j, k, ok[u]);
+    fluid[u] = act[c] != 0;
+    rv[u] = rin[c];
+    recv[u] = rec[c];
+  }
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
+    if (!ok[u]) continue;
+    const double r = fluid[u] ? rv[u] : 0.0;
+    const double az = r - static_cast<double>(ld(recv[u]));
     p0 += r * r;
     p1 += r * az;
     p2 += az * az;

@@ -101,6 +101,7 @@ struct Level {
   Tiles tc{};
   Tiles tcol{};
   Tiles tr{};
+  Tiles tred{};  // kRedPer tiles per block: reduction-only kernels
 };
 
 constexpr int kFoldSpan = 2048;   // partials per block of the first fold
@@ -562,13 +563,13 @@ void level_mean(hf_plan* p, int l, const T* field) {
   {
     HF_PROF(p, P_PROJECT);
     if (l == 0)
-      HF_LAUNCH(k_vol_dot_finest<T>, L.tr.blocks, kBlock, 0, p->stream, L.tr,
-                field, L.act, widths(p), p->partials);
+      HF_LAUNCH(k_vol_dot_finest<T>, L.tred.blocks, kBlock, 0, p->stream,
+                L.tred, field, L.act, widths(p), p->partials);
     else
-      HF_LAUNCH(k_vol_dot<T>, L.tr.blocks, kBlock, 0, p->stream, L.tr, field,
-                static_cast<const double*>(L.vol), p->partials);
+      HF_LAUNCH(k_vol_dot<T>, L.tred.blocks, kBlock, 0, p->stream, L.tred,
+                field, static_cast<const double*>(L.vol), p->partials);
   }
-  finalize(p, L.tr.blocks, 1, F_MEAN, S_MEAN0 + l, L.volume);
+  finalize(p, L.tred.blocks, 1, F_MEAN, S_MEAN0 + l, L.volume);
 }
 
 template <class T>
@@ -826,13 +827,13 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
     T* x = static_cast<T*>(L0.x);
     {
       HF_PROF(p, P_MG_ENTRY_EXIT);
-      HF_LAUNCH(k_mg_entry<T>, L0.tr.blocks, kBlock, 0, p->stream, L0.tr, r,
+      HF_LAUNCH(k_mg_entry<T>, L0.tred.blocks, kBlock, 0, p->stream, L0.tred, r,
                 static_cast<const uint8_t*>(L0.act), widths(p), b,
                 tail0 ? x : static_cast<T*>(nullptr),
                 proj ? p->partials : static_cast<double*>(nullptr));
     }
     if (proj) {
-      finalize(p, L0.tr.blocks, 1, F_MEAN, S_MEAN0, L0.volume);
+      finalize(p, L0.tred.blocks, 1, F_MEAN, S_MEAN0, L0.volume);
       if (tail0) project_apply<T>(p, 0, b);
     }
   });
@@ -859,16 +860,16 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
       }
       {
         HF_PROF(p, P_MG_ENTRY_EXIT);
-        HF_LAUNCH(k_mg_proj3<T>, L0.tr.blocks, kBlock, 0, p->stream, L0.tr, r,
+        HF_LAUNCH(k_mg_proj3<T>, L0.tred.blocks, kBlock, 0, p->stream, L0.tred, r,
                   static_cast<const uint8_t*>(L0.act),
                   static_cast<const T*>(L0.r), p->partials,
                   static_cast<int>(p->partial_stride));
       }
-      finalize(p, L0.tr.blocks, 3, F_MGALPHA, 0, 0.0, 0);
+      finalize(p, L0.tred.blocks, 3, F_MGALPHA, 0, 0.0, 0);
       alpha = p->scalars + S_MGALPHA;
     }
     HF_PROF(p, P_MG_ENTRY_EXIT);
-    HF_LAUNCH(k_mg_exit<T>, L0.tr.blocks, kBlock, 0, p->stream, L0.tr,
+    HF_LAUNCH(k_mg_exit<T>, L0.tred.blocks, kBlock, 0, p->stream, L0.tred,
               static_cast<const T*>(x), static_cast<const uint8_t*>(L0.act),
               mean, alpha, z, dots ? r : static_cast<const double*>(nullptr),
               q, dots ? p->partials : static_cast<double*>(nullptr),
@@ -1031,6 +1032,9 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tc = make_tiles(L.n[0], L.n[1], L.n[2], false);
     L.tcol = make_tiles((L.n[0] + 1) / 2, L.n[1], L.n[2], false);
     L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
+    L.tred = L.tr;
+    L.tred.per = kRedPer;
+    L.tred.blocks = (L.tr.count + kRedPer - 1) / kRedPer;
     // fused sweeps on grid levels with the fused colour route and rows long
     // enough for the tiles to stay mostly busy
     L.sweep = l == 0 && !L.tail && L.line_mask == 0 && L.fused &&
@@ -1359,7 +1363,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   auto precondition_restart = [&]() -> hf_status {
     HF_TRY(mg_apply(p, r, z, nullptr, true));
     res->preconditioner_applies++;
-    finalize(p, p->levels[0].tr.blocks, 1, F_RHO, 0);
+    finalize(p, p->levels[0].tred.blocks, 1, F_RHO, 0);
     HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, pp, z);
     return ok();
   };
@@ -1422,7 +1426,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     }
     HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true));
     res->preconditioner_applies++;
-    finalize(p, tr.blocks, ctl->flexible ? 2 : 1,
+    finalize(p, p->levels[0].tred.blocks, ctl->flexible ? 2 : 1,
              ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
     HF_PROF(p, P_UPDATE_P);
     HF_LAUNCH(k_update_p, nbv, kBlock, 0, p->stream, ch, pp, z,
