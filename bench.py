@@ -63,7 +63,7 @@ def class_bytes_per_cell(cls: str, s: int):
 # kernel class -> symbol of its kernel for the rocprof traffic lookup
 CLASS_KERNELS = {
     "rb_finest": "void hf::k_rb<{t}>(hf::Tiles, hf::Lv<{t}>, hf::Bc, int)",
-    "sweep_finest": "void hf::k_rb_sweep<{t}>(hf::SweepArgs<{t}>)",
+    "sweep_finest": "void hf::k_rb_sweep<{t}, false>(hf::SweepArgs<{t}>)",
     "operator": "hf::k_op_dot(hf::Tiles, hf::Lv<double>, hf::Bc, double const*, double*, double*)",
 }
 

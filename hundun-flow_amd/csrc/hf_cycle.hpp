@@ -777,6 +777,14 @@ namespace hf {
 // the reference arithmetic, so the result equals two colour passes bit for
 // bit.  With `from_zero` the input x is zero (no loads) and a pending
 // constant projection (`mean`) of b is applied on the fly and stored to bout.
+//
+// Boundaries live in the LDS halo: a position outside the domain holds 0
+// (Dirichlet), the wrapped cell (periodic) or the mirrored boundary cell as
+// loaded (Neumann: the reference reads the cell's own value, which is still
+// its value at sweep start when the cell is updated, each cell being updated
+// once per sweep), so every update reads its six neighbours unconditionally.
+// Per-thread offsets are formed once per block, for both colour parities,
+// and each load is a 32-bit offset from a per-plane base address.
 // ---------------------------------------------------------------------------
 constexpr int kSwTX = 128;
 constexpr int kSwTY = 8;
@@ -812,24 +820,176 @@ constexpr int kSwRingSlots = (kSwRingCells + kSwThreads - 1) / kSwThreads;
 constexpr int kSwTileCells = kSwTY * (kSwTX / 2);
 constexpr int kSwTileSlots = (kSwTileCells + kSwThreads - 1) / kSwThreads;
 
+// the second-colour lists give each wave one tile row of kSwTX / 2 cells
+static_assert(kSwTX / 2 == 64 && kSwThreads % 64 == 0 &&
+                  kSwTileCells == kSwTileSlots * kSwThreads,
+              "fused sweep lane mapping");
+
+// Source index along one axis of a halo position g (-2 <= g <= n + 1), or -1
+// where the position holds zero.
+__device__ __forceinline__ int sw_source(int g, int n, int mlo, int mhi) {
+  if (g >= 0 && g < n) return g;
+  if (g < -2 || g > n + 1) return -1;  // never read
+  if (g < 0) {
+    if (mlo == 2) return g + n;
+    return (g == -1 && mlo != 0) ? 0 : -1;
+  }
+  if (mhi == 2) return g - n;
+  return (g == n && mhi != 0) ? n - 1 : -1;
+}
+
+// Element `off` (32-bit, non-negative) of a per-plane base pointer.
+// Raw buffer access: a scalar resource per array, a 32-bit per-lane byte
+// offset within a plane and a scalar byte offset of the plane, so a load
+// carries no 64-bit address arithmetic.  The host plans the fused sweep only
+// where every array of the level spans less than 4 GiB.
+using SwRsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kBufferDword3 = 0x00020000;  // gfx9 raw buffer descriptor
+
 template <class T>
+__device__ __forceinline__ SwRsrc sw_rsrc(const T* p, uint64_t count) {
+  const uint64_t bytes = count * sizeof(T);
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<T*>(p), 0,
+      static_cast<int>(bytes > 0xffffffffull ? 0xffffffffu
+                                             : static_cast<uint32_t>(bytes)),
+      kBufferDword3);
+}
+
+template <class T>
+__device__ __forceinline__ T sw_load(SwRsrc r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float sw_load<float>(SwRsrc r, uint32_t voff,
+                                                uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+      r, static_cast<int>(voff), static_cast<int>(soff), 0));
+}
+template <>
+__device__ __forceinline__ double sw_load<double>(SwRsrc r, uint32_t voff,
+                                                  uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                        r, static_cast<int>(voff),
+                                        static_cast<int>(soff), 0));
+}
+template <>
+__device__ __forceinline__ bf16 sw_load<bf16>(SwRsrc r, uint32_t voff,
+                                              uint32_t soff) {
+  bf16 v;
+  v.bits = __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(voff),
+                                                static_cast<int>(soff), 0);
+  return v;
+}
+
+__device__ __forceinline__ void sw_store(SwRsrc r, uint32_t voff,
+                                         uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r,
+                                        static_cast<int>(voff),
+                                        static_cast<int>(soff), 0);
+}
+__device__ __forceinline__ void sw_store(SwRsrc r, uint32_t voff,
+                                         uint32_t soff, double v) {
+  using u2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r,
+                                        static_cast<int>(voff),
+                                        static_cast<int>(soff), 0);
+}
+__device__ __forceinline__ void sw_store(SwRsrc r, uint32_t voff,
+                                         uint32_t soff, bf16 v) {
+  __builtin_amdgcn_raw_buffer_store_b16(v.bits, r, static_cast<int>(voff),
+                                        static_cast<int>(soff), 0);
+}
+
+// kZero: the first sweep of a smoothing from x = 0 (a.from_zero), which loads
+// no x and applies a pending projection of b; otherwise a.from_zero is 0 and
+// a.mean is null.
+template <class T, bool kZero>
 __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   using C = typename Compute<T>::type;
   __shared__ C buf[4][kSwPlane];
   const Lv<T>& L = a.L;
   int blk = xcd_block32(blockIdx.x, gridDim.x);
-  const int tx = blk % a.gx;
+  // the tile coordinates are block-uniform: held in scalar registers, every
+  // per-plane offset derived from them is scalar too (buffer soffsets)
+  const int tx = __builtin_amdgcn_readfirstlane(blk % a.gx);
   blk /= a.gx;
-  const int ty = blk % a.gy;
-  const int tz = blk / a.gy;
+  const int ty = __builtin_amdgcn_readfirstlane(blk % a.gy);
+  const int tz = __builtin_amdgcn_readfirstlane(blk / a.gy);
   const int x0 = tx * kSwTX, y0 = ty * kSwTY;
   const int z0 = tz * a.zc;
   const int z1 = min(L.nz, z0 + a.zc);
-  const bool perx = a.bc.m[0] == 2, pery = a.bc.m[2] == 2,
-             perz = a.bc.m[4] == 2;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const int mx0 = a.bc.m[0], mx1 = a.bc.m[1], my0 = a.bc.m[2],
+            my1 = a.bc.m[3], mz0 = a.bc.m[4], mz1 = a.bc.m[5];
+  const bool perx = mx0 == 2, pery = my0 == 2, perz = mz0 == 2;
+  const uint32_t nxy = static_cast<uint32_t>(nx) * ny;
+  constexpr uint32_t kS = sizeof(T);
   const C omega = C(0.72);
   const double mean = a.mean != nullptr ? *a.mean : 0.0;
+  const int tid = threadIdx.x;
+  const int c0 = a.first, c1 = a.first ^ 1;
+  const uint64_t cells = static_cast<uint64_t>(nxy) * nz;
+  const SwRsrc rx = sw_rsrc(L.x, cells);
+  const SwRsrc rd = sw_rsrc(L.D, cells);
+  const SwRsrc rfx = sw_rsrc(L.fx, cells + static_cast<uint64_t>(ny) * nz);
+  const SwRsrc rfy = sw_rsrc(L.fy, cells + static_cast<uint64_t>(nx) * nz);
+  const SwRsrc rfz = sw_rsrc(L.fz, cells + nxy);
+  const SwRsrc rb = sw_rsrc(L.b, cells);
+  const SwRsrc rout = sw_rsrc(a.xout, cells);
+
+  // ---- per-block set-up
+  // x plane: in-plane source byte offset of every LDS position this thread
+  // fills
+  uint32_t xo[kSwLoadSlots];
+  uint32_t xok = 0;
+#pragma unroll
+  for (int s = 0; s < kSwLoadSlots; ++s) {
+    const int p = tid + s * kSwThreads;
+    const int iy = p / kSwPX;
+    const int ix = p - iy * kSwPX - 2;
+    const int si = sw_source(x0 + ix, nx, mx0, mx1);
+    const int sj = sw_source(y0 + iy - 2, ny, my0, my1);
+    const bool ok = p < kSwPlane && si >= 0 && sj >= 0;
+    xo[s] = ok ? static_cast<uint32_t>(sj * nx + si) * kS : 0u;
+    xok |= ok ? (1u << s) : 0u;
+  }
+  // first colour, tile plus a one-cell ring: byte offsets (cells, x faces)
+  // for both parities (ix = -1 + parity + 2 m); ring cells beyond a periodic
+  // face are the wrapped cells
+  uint32_t ro[kSwRingSlots][2], rf[kSwRingSlots][2];
+  int rq[kSwRingSlots];
+  uint32_t rok = 0, rpar = 0;
+#pragma unroll
+  for (int s = 0; s < kSwRingSlots; ++s) {
+    const int q = tid + s * kSwThreads;
+    const int r = q / kSwRingRow;
+    const int m = q - r * kSwRingRow;
+    const int iy = r - 1;
+    int gj = y0 + iy;
+    // parity of the unwrapped index equals the wrapped one: the fused route
+    // only runs with even periodic extents (x0 is even)
+    rpar |= static_cast<uint32_t>((c0 + 1 - gj) & 1) << s;
+    bool okj = q < kSwRingCells && gj >= -1 && gj <= ny;
+    if (gj < 0 || gj >= ny) {
+      if (pery) gj = wrapi(gj, ny); else okj = false;
+    }
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int ix = -1 + par + 2 * m;
+      int gi = x0 + ix;
+      bool ok = okj && gi >= -1 && gi <= nx;
+      if (gi < 0 || gi >= nx) {
+        if (perx) gi = wrapi(gi, nx); else ok = false;
+      }
+      ro[s][par] = ok ? static_cast<uint32_t>(gj * nx + gi) * kS : 0u;
+      rf[s][par] = ok ? static_cast<uint32_t>(gj * (nx + 1) + gi) * kS : 0u;
+      rok |= ok ? (1u << (2 * s + par)) : 0u;
+    }
+    rq[s] = (2 * m + 1) + kSwPX * (iy + 2);  // parity 0; parity 1 adds one
+  }
+  // second colour on the tile (ix = parity + 2 m, m = lane): one row per
+  // wave and slot, so the parity is uniform across the wave
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
 
   auto plane_of = [&](int kk, int& kg) -> bool {
     kg = kk;
@@ -841,82 +1001,91 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   // Every global load of one plane step is issued before any of its values
   // is used, so a step waits on memory once rather than once per cell.
   auto fetch_plane = [&](int kk, C (&v)[kSwLoadSlots]) {
-    int kg;
-    const bool valid = plane_of(kk, kg) && !a.from_zero;
+    const int kz = kZero ? -1 : sw_source(kk, nz, mz0, mz1);
+    if (kz < 0) {
+#pragma unroll
+      for (int s = 0; s < kSwLoadSlots; ++s) v[s] = C(0);
+      return;
+    }
+    const uint32_t sx = static_cast<uint32_t>(kz) * nxy * kS;
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
-      const int p = threadIdx.x + s * kSwThreads;
-      const int iy = p / kSwPX;
-      const int ix = p - iy * kSwPX - 2;
-      int gi = x0 + ix, gj = y0 + iy - 2;
-      // only positions within two cells of the domain are ever read
-      bool ok = valid && p < kSwPlane && gi >= -2 && gi <= nx + 1 &&
-                gj >= -2 && gj <= ny + 1;
-      if (gi < 0 || gi >= nx) {
-        if (perx) gi = wrapi(gi, nx); else ok = false;
-      }
-      if (gj < 0 || gj >= ny) {
-        if (pery) gj = wrapi(gj, ny); else ok = false;
-      }
-      const int c = ok ? cell_index(nx, ny, gi, gj, kg) : 0;
-      const C x = C(ld(L.x[c]));
-      v[s] = ok ? x : C(0);
+      const C x = C(ld(sw_load<T>(rx, xo[s], sx)));
+      v[s] = ((xok >> s) & 1u) ? x : C(0);
     }
   };
   auto store_plane = [&](int kk, const C (&v)[kSwLoadSlots]) {
     C* P = buf[kk & 3];
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
-      const int p = threadIdx.x + s * kSwThreads;
+      const int p = tid + s * kSwThreads;
       if (p < kSwPlane) P[p] = v[s];
     }
   };
-  auto bval = [&](int c) -> C {
-    if (a.mean == nullptr) return C(ld(L.b[c]));
-    const T projected = st<T>(static_cast<double>(ld(L.b[c])) - mean);
-    return L.act[c] ? C(ld(projected)) : C(0);
+  // byte offsets of plane kg in the coefficient arrays (cells, x faces,
+  // y faces and the high y face, the high z face)
+  struct PlaneOff {
+    uint32_t c, fx, fy, fyh, fzh;
   };
-  // coefficients of the update at (gi, gj, kg); c = 0 stands in for an
-  // inactive slot so every load address is in bounds
-  auto fetch_coef = [&](bool ok, int gi, int gj, int kg) -> SwCoef<C> {
-    const int c = ok ? cell_index(nx, ny, gi, gj, kg) : 0;
-    const int row = ok ? gj + ny * kg : 0;
-    const int fyo = ok ? c + nx * kg : 0;
-    SwCoef<C> f;
-    f.d = C(ld(L.D[c]));
-    f.xl = C(ld(L.fx[c + row]));
-    f.xh = C(ld(L.fx[c + row + 1]));
-    f.yl = C(ld(L.fy[fyo]));
-    f.yh = C(ld(L.fy[fyo + nx]));
-    f.zl = C(ld(L.fz[c]));
-    f.zh = C(ld(L.fz[c + nx * ny]));
-    f.b = bval(c);
-    return f;
+  auto plane_off = [&](int kg) -> PlaneOff {
+    const uint32_t k = static_cast<uint32_t>(kg);
+    PlaneOff pl;
+    pl.c = k * nxy * kS;
+    pl.fx = k * (nxy + ny) * kS;
+    pl.fy = k * (nxy + nx) * kS;
+    pl.fyh = pl.fy + nx * kS;
+    pl.fzh = pl.c + nxy * kS;
+    return pl;
   };
-  auto mode = [](int m, C self) -> C { return m == 0 ? C(0) : self; };
-  // x <- x + omega (b - A x) / D at one cell held in LDS, in the reference
-  // evaluation order
+  auto bval_at = [&](const PlaneOff& pl, uint32_t ob) -> C {
+    const T braw = sw_load<T>(rb, ob, pl.c);
+    if (!kZero || a.mean == nullptr) return C(ld(braw));
+    const T projected = st<T>(static_cast<double>(ld(braw)) - mean);
+    return L.act[(pl.c + ob) / kS] ? C(ld(projected)) : C(0);
+  };
+  // coefficients of the update at in-plane byte offset ob (x faces: fb)
+  auto fetch_coef = [&](const PlaneOff& pl, uint32_t ob,
+                        uint32_t fb) -> SwCoef<C> {
+    SwCoef<C> c;
+    c.d = C(ld(sw_load<T>(rd, ob, pl.c)));
+    c.xl = C(ld(sw_load<T>(rfx, fb, pl.fx)));
+    c.xh = C(ld(sw_load<T>(rfx, fb + kS, pl.fx)));
+    c.yl = C(ld(sw_load<T>(rfy, ob, pl.fy)));
+    c.yh = C(ld(sw_load<T>(rfy, ob, pl.fyh)));
+    c.zl = C(ld(sw_load<T>(rfz, ob, pl.c)));
+    c.zh = C(ld(sw_load<T>(rfz, ob, pl.fzh)));
+    c.b = bval_at(pl, ob);
+    return c;
+  };
+  // x <- x + omega (b - A x) / D at LDS position q, in the reference
+  // evaluation order; the halo supplies boundary neighbours
   auto update = [&](const SwCoef<C>& f, const C* Pm, C* P0, const C* Pp,
-                    int ix, int iy, int gi, int gj, int kg) {
-    const int q = (ix + 2) + kSwPX * (iy + 2);
+                    int q) {
     const C xc = P0[q];
-    const C xl = (gi > 0 || perx) ? P0[q - 1] : mode(a.bc.m[0], xc);
-    const C xh = (gi + 1 < nx || perx) ? P0[q + 1] : mode(a.bc.m[1], xc);
-    const C yl = (gj > 0 || pery) ? P0[q - kSwPX] : mode(a.bc.m[2], xc);
-    const C yh = (gj + 1 < ny || pery) ? P0[q + kSwPX] : mode(a.bc.m[3], xc);
-    const C zl = (kg > 0 || perz) ? Pm[q] : mode(a.bc.m[4], xc);
-    const C zh = (kg + 1 < nz || perz) ? Pp[q] : mode(a.bc.m[5], xc);
     C v = f.d * xc;
-    v -= f.xl * xl;
-    v -= f.xh * xh;
-    v -= f.yl * yl;
-    v -= f.yh * yh;
-    v -= f.zl * zl;
-    v -= f.zh * zh;
+    v -= f.xl * P0[q - 1];
+    v -= f.xh * P0[q + 1];
+    v -= f.yl * P0[q - kSwPX];
+    v -= f.yh * P0[q + kSwPX];
+    v -= f.zl * Pm[q];
+    v -= f.zh * Pp[q];
     P0[q] = xc + omega * (f.b - v) / f.d;
   };
+  // second-colour tile cell of slot s on plane kb: in-plane byte offsets
+  // (cells, x faces), LDS position; false outside the domain or the chunk
+  auto tile_cell = [&](int s, int kb, uint32_t& ob, uint32_t& fb,
+                       int& q) -> bool {
+    const int iy = wave + s * (kSwThreads / 64);
+    const int gj = y0 + iy;
+    const int par = (c1 + gj + kb) & 1;
+    const int gi = x0 + par + 2 * lane;
+    q = (par + 2 * lane + 2) + kSwPX * (iy + 2);
+    const bool ok = kb >= z0 && kb < z1 && gi < nx && gj < ny;
+    ob = ok ? static_cast<uint32_t>(gj * nx + gi) * kS : 0u;
+    fb = ok ? ob + static_cast<uint32_t>(gj) * kS : 0u;
+    return ok;
+  };
 
-  const int c0 = a.first, c1 = a.first ^ 1;
   const int kstart = z0 - 1;
   {
     C v[kSwLoadSlots];
@@ -925,66 +1094,41 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     fetch_plane(kstart, v);
     store_plane(kstart, v);
   }
+  // Second-colour coefficients are fetched one step before their use, in
+  // the step whose first-colour loads touch the same cache lines of the
+  // same plane, so each coefficient line is fetched once per sweep.
+  SwCoef<C> f1[kSwTileSlots]{};
   for (int k = kstart; k <= z1; ++k) {
     // ---- issue: x of plane k+1, coefficients of both colour phases
     C vx[kSwLoadSlots];
     fetch_plane(k + 1, vx);
     int kg;
     const bool first_plane = plane_of(k, kg);
+    const PlaneOff pk = plane_off(first_plane ? kg : 0);
     SwCoef<C> f0[kSwRingSlots];
-    int ix0[kSwRingSlots], iy0[kSwRingSlots], gi0[kSwRingSlots],
-        gj0[kSwRingSlots];
-    bool ok0[kSwRingSlots];
+    int q0[kSwRingSlots];
+    uint32_t ok0 = 0;
 #pragma unroll
     for (int s = 0; s < kSwRingSlots; ++s) {
-      // first colour on the tile and its one-cell ring
-      const int q = threadIdx.x + s * kSwThreads;
-      const int r = q / kSwRingRow;
-      const int m = q - r * kSwRingRow;
-      const int iy = r - 1;
-      int gj = y0 + iy;
-      // parity with the unwrapped index equals the wrapped one: the fused
-      // route only runs with even periodic extents
-      const int par = (c0 - (x0 - 1 + gj + kg)) & 1;
-      const int ix = -1 + par + 2 * m;
-      int gi = x0 + ix;
-      // ring cells beyond a periodic face are the wrapped cells
-      bool ok = first_plane && q < kSwRingCells && gi >= -1 && gi <= nx &&
-                gj >= -1 && gj <= ny;
-      if (gi < 0 || gi >= nx) {
-        if (perx) gi = wrapi(gi, nx); else ok = false;
-      }
-      if (gj < 0 || gj >= ny) {
-        if (pery) gj = wrapi(gj, ny); else ok = false;
-      }
-      ok0[s] = ok;
-      ix0[s] = ix;
-      iy0[s] = iy;
-      gi0[s] = gi;
-      gj0[s] = gj;
-      f0[s] = fetch_coef(ok, gi, gj, kg);
+      const int par = static_cast<int>((rpar >> s) + static_cast<uint32_t>(kg)) & 1;
+      const bool ok = first_plane && ((rok >> (2 * s + par)) & 1u);
+      const uint32_t o = ok ? (par ? ro[s][1] : ro[s][0]) : 0u;
+      const uint32_t f = ok ? (par ? rf[s][1] : rf[s][0]) : 0u;
+      ok0 |= ok ? (1u << s) : 0u;
+      q0[s] = rq[s] + par;
+      f0[s] = fetch_coef(pk, o, f);
+    }
+    SwCoef<C> f1n[kSwTileSlots];
+#pragma unroll
+    for (int s = 0; s < kSwTileSlots; ++s) {
+      // second colour of plane k, used by the next step
+      uint32_t ob, fb;
+      int q;
+      tile_cell(s, k, ob, fb, q);
+      f1n[s] = fetch_coef(pk, ob, fb);
     }
     const int kb = k - 1;
     const bool second_plane = kb >= z0 && kb < z1;
-    SwCoef<C> f1[kSwTileSlots];
-    int ix1[kSwTileSlots], iy1[kSwTileSlots];
-    bool ok1[kSwTileSlots];
-#pragma unroll
-    for (int s = 0; s < kSwTileSlots; ++s) {
-      // second colour on the tile, one plane behind
-      const int q = threadIdx.x + s * kSwThreads;
-      const int iy = q / (kSwTX / 2);
-      const int m = q - iy * (kSwTX / 2);
-      const int gj = y0 + iy;
-      const int par = (c1 - (x0 + gj + kb)) & 1;
-      const int ix = par + 2 * m;
-      const int gi = x0 + ix;
-      const bool ok = second_plane && q < kSwTileCells && gi < nx && gj < ny;
-      ok1[s] = ok;
-      ix1[s] = ix;
-      iy1[s] = iy;
-      f1[s] = fetch_coef(ok, gi, gj, kb);
-    }
     // ---- x of plane k+1 into LDS
     store_plane(k + 1, vx);
     __syncthreads();
@@ -994,8 +1138,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
       const C* Pp = buf[(k + 1) & 3];
 #pragma unroll
       for (int s = 0; s < kSwRingSlots; ++s)
-        if (ok0[s])
-          update(f0[s], Pm, P0, Pp, ix0[s], iy0[s], gi0[s], gj0[s], kg);
+        if ((ok0 >> s) & 1u) update(f0[s], Pm, P0, Pp, q0[s]);
     }
     __syncthreads();
     if (second_plane) {
@@ -1003,21 +1146,28 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
       C* P0 = buf[kb & 3];
       const C* Pp = buf[(kb + 1) & 3];
 #pragma unroll
-      for (int s = 0; s < kSwTileSlots; ++s)
-        if (ok1[s])
-          update(f1[s], Pm, P0, Pp, ix1[s], iy1[s], x0 + ix1[s], y0 + iy1[s],
-                 kb);
+      for (int s = 0; s < kSwTileSlots; ++s) {
+        uint32_t ob, fb;
+        int q;
+        if (tile_cell(s, kb, ob, fb, q)) update(f1[s], Pm, P0, Pp, q);
+      }
       __syncthreads();
-      for (int p = threadIdx.x; p < kSwTX * kSwTY; p += kSwThreads) {
+      const PlaneOff pb = plane_off(kb);
+#pragma unroll
+      for (int i = 0; i < kSwTX * kSwTY / kSwThreads; ++i) {
+        const int p = tid + i * kSwThreads;
         const int iy = p / kSwTX;
         const int ix = p - iy * kSwTX;
         const int gi = x0 + ix, gj = y0 + iy;
         if (gi >= nx || gj >= ny) continue;
-        const int c = cell_index(nx, ny, gi, gj, kb);
-        a.xout[c] = st<T>(P0[(ix + 2) + kSwPX * (iy + 2)]);
-        if (a.bout != nullptr) a.bout[c] = st<T>(bval(c));
+        const uint32_t ob = static_cast<uint32_t>(gj * nx + gi) * kS;
+        sw_store(rout, ob, pb.c, st<T>(P0[(ix + 2) + kSwPX * (iy + 2)]));
+        if (kZero && a.bout != nullptr)
+          a.bout[(pb.c + ob) / kS] = st<T>(bval_at(pb, ob));
       }
     }
+#pragma unroll
+    for (int s = 0; s < kSwTileSlots; ++s) f1[s] = f1n[s];
   }
 }
 

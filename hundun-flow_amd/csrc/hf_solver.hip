@@ -41,7 +41,7 @@ enum ProfileClass : int {
   P_SWEEP_FINEST = 14,  // fused two-colour sweep on level 0
   P_SWEEP_COARSE = 15,  // fused two-colour sweep on grid levels >= 1
   P_LINE = 16,          // line relaxation (any level)
-  P_RB_FIRST = 17,      // first colour pass from x = 0 (any grid level)
+  P_RB_FIRST = 17,      // first colour pass, or fused sweep, from x = 0
   kProfileClasses = 18
 };
 constexpr uint32_t kDetailPlan = 9101U;
@@ -603,14 +603,18 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     a.gx = L.sweep_gx;
     a.gy = L.sweep_gy;
     {
-      HF_PROF(p, rb_class);
+      HF_PROF(p, zero ? P_RB_FIRST : rb_class);
       // the column-owner schedule exists for 32- and 16-bit storage only
       // (its FP64 register footprint would leave one wave per SIMD)
       if (sizeof(T) < sizeof(double) && sweep_columns())
         HF_LAUNCH(k_rb_sweep_cols<T>, L.sweep_blocks, kSwThreads, 0,
                   p->stream, a);
+      else if (zero)
+        HF_LAUNCH((k_rb_sweep<T, true>), L.sweep_blocks, kSwThreads, 0,
+                  p->stream, a);
       else
-        HF_LAUNCH(k_rb_sweep<T>, L.sweep_blocks, kSwThreads, 0, p->stream, a);
+        HF_LAUNCH((k_rb_sweep<T, false>), L.sweep_blocks, kSwThreads, 0,
+                  p->stream, a);
     }
     std::swap(L.x, L.x2);
     if (a.bout != nullptr) std::swap(L.b, L.b2);
@@ -1037,8 +1041,15 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tred.blocks = (L.tr.count + kRedPer - 1) / kRedPer;
     // fused sweeps on grid levels with the fused colour route and rows long
     // enough for the tiles to stay mostly busy
+    // (its buffer loads take 32-bit byte offsets: every array under 4 GiB)
+    const int64_t widest =
+        std::max({static_cast<int64_t>(L.n[0] + 1) * L.n[1] * L.n[2],
+                  static_cast<int64_t>(L.n[0]) * (L.n[1] + 1) * L.n[2],
+                  static_cast<int64_t>(L.n[0]) * L.n[1] * (L.n[2] + 1)});
     L.sweep = l == 0 && !L.tail && L.line_mask == 0 && L.fused &&
-              L.n[0] >= 32 && L.n[1] >= 8 && sweep_enabled();
+              L.n[0] >= 32 && L.n[1] >= 8 && sweep_enabled() &&
+              widest * static_cast<int64_t>(elem_size(L.prec)) <
+                  (int64_t{1} << 32);
     if (L.sweep) {
       L.sweep_gx = (L.n[0] + kSwTX - 1) / kSwTX;
       L.sweep_gy = (L.n[1] + kSwTY - 1) / kSwTY;
