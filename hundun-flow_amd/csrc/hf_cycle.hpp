@@ -783,8 +783,18 @@ namespace hf {
 // loaded (Neumann: the reference reads the cell's own value, which is still
 // its value at sweep start when the cell is updated, each cell being updated
 // once per sweep), so every update reads its six neighbours unconditionally.
-// Per-thread offsets are formed once per block, for both colour parities,
-// and each load is a 32-bit offset from a per-plane base address.
+//
+// Work is owned by cell pairs: lane l of a wave owns cells x0 + 2l and
+// x0 + 2l + 1 of a row, one of each colour.  At step k it loads the pair's
+// coefficients on plane k with one two-element load per array (whole cache
+// lines per instruction), updates the first-colour cell at once and keeps the
+// second-colour cell's coefficients in registers for step k + 1, where that
+// cell is updated.  Wave w owns tile rows w and w + 4; waves 0 and 1 also
+// own the ring rows -1 and 8 (first colour only) and wave 2 the ring cells
+// of columns -1 and 128 (one per tile row and plane, the column alternating
+// with the parity).  Each load is a raw buffer load: scalar resource, 32-bit
+// lane offset within the plane, scalar plane offset.  Rows hold an even
+// number of cells (the host plans this kernel for even nx only).
 // ---------------------------------------------------------------------------
 constexpr int kSwTX = 128;
 constexpr int kSwTY = 8;
@@ -813,17 +823,12 @@ struct SwCoef {
   C d, xl, xh, yl, yh, zl, zh, b;
 };
 
-// Slots per thread of the three per-plane work lists.
+// LDS positions of the x plane per thread.
 constexpr int kSwLoadSlots = (kSwPlane + kSwThreads - 1) / kSwThreads;
-constexpr int kSwRingCells = (kSwTY + 2) * kSwRingRow;
-constexpr int kSwRingSlots = (kSwRingCells + kSwThreads - 1) / kSwThreads;
-constexpr int kSwTileCells = kSwTY * (kSwTX / 2);
-constexpr int kSwTileSlots = (kSwTileCells + kSwThreads - 1) / kSwThreads;
 
-// the second-colour lists give each wave one tile row of kSwTX / 2 cells
-static_assert(kSwTX / 2 == 64 && kSwThreads % 64 == 0 &&
-                  kSwTileCells == kSwTileSlots * kSwThreads,
-              "fused sweep lane mapping");
+// one pair per lane covers a tile row; four waves cover the 8 tile rows twice
+static_assert(kSwTX == 128 && kSwTY == 8 && kSwThreads == 256,
+              "fused sweep pair mapping");
 
 // Source index along one axis of a halo position g (-2 <= g <= n + 1), or -1
 // where the position holds zero.
@@ -838,7 +843,6 @@ __device__ __forceinline__ int sw_source(int g, int n, int mlo, int mhi) {
   return (g == n && mhi != 0) ? n - 1 : -1;
 }
 
-// Element `off` (32-bit, non-negative) of a per-plane base pointer.
 // Raw buffer access: a scalar resource per array, a 32-bit per-lane byte
 // offset within a plane and a scalar byte offset of the plane, so a load
 // carries no 64-bit address arithmetic.  The host plans the fused sweep only
@@ -880,24 +884,72 @@ __device__ __forceinline__ bf16 sw_load<bf16>(SwRsrc r, uint32_t voff,
   return v;
 }
 
-__device__ __forceinline__ void sw_store(SwRsrc r, uint32_t voff,
-                                         uint32_t soff, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r,
-                                        static_cast<int>(voff),
-                                        static_cast<int>(soff), 0);
+// Two consecutive elements (the first at an even element offset).
+template <class T>
+struct SwPair {
+  T lo, hi;
+};
+template <class T>
+__device__ __forceinline__ SwPair<T> sw_load2(SwRsrc r, uint32_t voff,
+                                              uint32_t soff);
+template <>
+__device__ __forceinline__ SwPair<float> sw_load2<float>(SwRsrc r,
+                                                         uint32_t voff,
+                                                         uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(
+      r, static_cast<int>(voff), static_cast<int>(soff), 0);
+  return {__uint_as_float(v[0]), __uint_as_float(v[1])};
 }
-__device__ __forceinline__ void sw_store(SwRsrc r, uint32_t voff,
-                                         uint32_t soff, double v) {
+template <>
+__device__ __forceinline__ SwPair<double> sw_load2<double>(SwRsrc r,
+                                                           uint32_t voff,
+                                                           uint32_t soff) {
+  return __builtin_bit_cast(
+      SwPair<double>, __builtin_amdgcn_raw_buffer_load_b128(
+                          r, static_cast<int>(voff), static_cast<int>(soff), 0));
+}
+template <>
+__device__ __forceinline__ SwPair<bf16> sw_load2<bf16>(SwRsrc r,
+                                                       uint32_t voff,
+                                                       uint32_t soff) {
+  const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(
+      r, static_cast<int>(voff), static_cast<int>(soff), 0);
+  SwPair<bf16> p;
+  p.lo.bits = static_cast<uint16_t>(v & 0xffffu);
+  p.hi.bits = static_cast<uint16_t>(v >> 16);
+  return p;
+}
+
+__device__ __forceinline__ void sw_store2(SwRsrc r, uint32_t voff,
+                                          uint32_t soff, float lo, float hi) {
   using u2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r,
-                                        static_cast<int>(voff),
+  const u2 v = {__float_as_uint(lo), __float_as_uint(hi)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, static_cast<int>(voff),
                                         static_cast<int>(soff), 0);
 }
-__device__ __forceinline__ void sw_store(SwRsrc r, uint32_t voff,
-                                         uint32_t soff, bf16 v) {
-  __builtin_amdgcn_raw_buffer_store_b16(v.bits, r, static_cast<int>(voff),
+__device__ __forceinline__ void sw_store2(SwRsrc r, uint32_t voff,
+                                          uint32_t soff, double lo,
+                                          double hi) {
+  using u4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+  const SwPair<double> p = {lo, hi};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, p), r,
+                                         static_cast<int>(voff),
+                                         static_cast<int>(soff), 0);
+}
+__device__ __forceinline__ void sw_store2(SwRsrc r, uint32_t voff,
+                                          uint32_t soff, bf16 lo, bf16 hi) {
+  const uint32_t v = static_cast<uint32_t>(lo.bits) |
+                     (static_cast<uint32_t>(hi.bits) << 16);
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, static_cast<int>(voff),
                                         static_cast<int>(soff), 0);
 }
+
+// Coefficients of one cell pair on one plane.
+template <class T>
+struct SwPairCoef {
+  SwPair<T> d, yl, yh, zl, zh, b;
+  T fx0, fx1, fx2;  // x faces of the pair: low, middle, high
+};
 
 // kZero: the first sweep of a smoothing from x = 0 (a.from_zero), which loads
 // no x and applies a pending projection of b; otherwise a.from_zero is 0 and
@@ -926,7 +978,9 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   const C omega = C(0.72);
   const double mean = a.mean != nullptr ? *a.mean : 0.0;
   const int tid = threadIdx.x;
-  const int c0 = a.first, c1 = a.first ^ 1;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int c0 = a.first;
   const uint64_t cells = static_cast<uint64_t>(nxy) * nz;
   const SwRsrc rx = sw_rsrc(L.x, cells);
   const SwRsrc rd = sw_rsrc(L.D, cells);
@@ -952,44 +1006,58 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     xo[s] = ok ? static_cast<uint32_t>(sj * nx + si) * kS : 0u;
     xok |= ok ? (1u << s) : 0u;
   }
-  // first colour, tile plus a one-cell ring: byte offsets (cells, x faces)
-  // for both parities (ix = -1 + parity + 2 m); ring cells beyond a periodic
-  // face are the wrapped cells
-  uint32_t ro[kSwRingSlots][2], rf[kSwRingSlots][2];
-  int rq[kSwRingSlots];
-  uint32_t rok = 0, rpar = 0;
+  // pair rows of this wave: tile rows wave and wave + 4, and the ring row
+  // above the tile (wave 0) or below its last row inside the domain (wave 1).
+  // Per row: tile row index r, unwrapped row gj (parity), whether the row
+  // exists (tile rows: inside the domain; ring rows: inside, or wrapped
+  // across a periodic face), and the pair's byte offsets (cells, x faces).
+  constexpr int kRows = 3;  // two tile rows, one ring row (waves 0 and 1)
+  int rr[kRows], rgj[kRows];
+  bool rvalid[kRows];
+  uint32_t pob[kRows], pfb[kRows];
+  const bool lane_in = x0 + 2 * lane < nx;  // nx even: both cells inside
 #pragma unroll
-  for (int s = 0; s < kSwRingSlots; ++s) {
-    const int q = tid + s * kSwThreads;
-    const int r = q / kSwRingRow;
-    const int m = q - r * kSwRingRow;
-    const int iy = r - 1;
-    int gj = y0 + iy;
-    // parity of the unwrapped index equals the wrapped one: the fused route
-    // only runs with even periodic extents (x0 is even)
-    rpar |= static_cast<uint32_t>((c0 + 1 - gj) & 1) << s;
-    bool okj = q < kSwRingCells && gj >= -1 && gj <= ny;
-    if (gj < 0 || gj >= ny) {
-      if (pery) gj = wrapi(gj, ny); else okj = false;
-    }
-#pragma unroll
-    for (int par = 0; par < 2; ++par) {
-      const int ix = -1 + par + 2 * m;
-      int gi = x0 + ix;
-      bool ok = okj && gi >= -1 && gi <= nx;
-      if (gi < 0 || gi >= nx) {
-        if (perx) gi = wrapi(gi, nx); else ok = false;
+  for (int s = 0; s < kRows; ++s) {
+    const int r = s == 0   ? wave
+                  : s == 1 ? wave + 4
+                           : (wave == 0 ? -1 : min(kSwTY, ny - y0));
+    int gj = y0 + r;
+    rr[s] = r;
+    rgj[s] = gj;
+    bool ok;
+    if (s < 2) {
+      ok = gj < ny;
+    } else {
+      ok = wave < 2;
+      if (gj < 0 || gj >= ny) {
+        if (pery) gj = wrapi(gj, ny); else ok = false;
       }
-      ro[s][par] = ok ? static_cast<uint32_t>(gj * nx + gi) * kS : 0u;
-      rf[s][par] = ok ? static_cast<uint32_t>(gj * (nx + 1) + gi) * kS : 0u;
-      rok |= ok ? (1u << (2 * s + par)) : 0u;
     }
-    rq[s] = (2 * m + 1) + kSwPX * (iy + 2);  // parity 0; parity 1 adds one
+    rvalid[s] = ok;
+    const bool lok = ok && lane_in;
+    pob[s] = lok ? static_cast<uint32_t>(gj * nx + x0 + 2 * lane) * kS : 0u;
+    pfb[s] = lok ? static_cast<uint32_t>(gj * (nx + 1) + x0 + 2 * lane) * kS
+                 : 0u;
   }
-  // second colour on the tile (ix = parity + 2 m, m = lane): one row per
-  // wave and slot, so the parity is uniform across the wave
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
+  // ring columns (wave 2, lanes 0..7 = tile rows): byte offsets of the cell
+  // left of the tile (sel 0) and right of its last column inside the domain
+  // (sel 1, at an even x as nx is even); ring cells beyond a periodic face
+  // are the wrapped cells
+  uint32_t cob[2], cfb[2];
+  uint32_t cok = 0;
+  const int cgj = y0 + (lane & 7);
+  const int cright = min(x0 + kSwTX, nx);
+#pragma unroll
+  for (int sel = 0; sel < 2; ++sel) {
+    int gi = sel == 0 ? x0 - 1 : cright;
+    bool ok = wave == 2 && lane < kSwTY && cgj < ny;
+    if (gi < 0 || gi >= nx) {
+      if (perx) gi = wrapi(gi, nx); else ok = false;
+    }
+    cob[sel] = ok ? static_cast<uint32_t>(cgj * nx + gi) * kS : 0u;
+    cfb[sel] = ok ? static_cast<uint32_t>(cgj * (nx + 1) + gi) * kS : 0u;
+    cok |= ok ? (1u << sel) : 0u;
+  }
 
   auto plane_of = [&](int kk, int& kg) -> bool {
     kg = kk;
@@ -1037,25 +1105,60 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     pl.fzh = pl.c + nxy * kS;
     return pl;
   };
-  auto bval_at = [&](const PlaneOff& pl, uint32_t ob) -> C {
-    const T braw = sw_load<T>(rb, ob, pl.c);
+  // b at a cell (byte offset ob in plane pl), projected on the first sweep
+  auto bval = [&](const PlaneOff& pl, uint32_t ob, T braw) -> C {
     if (!kZero || a.mean == nullptr) return C(ld(braw));
     const T projected = st<T>(static_cast<double>(ld(braw)) - mean);
     return L.act[(pl.c + ob) / kS] ? C(ld(projected)) : C(0);
   };
-  // coefficients of the update at in-plane byte offset ob (x faces: fb)
-  auto fetch_coef = [&](const PlaneOff& pl, uint32_t ob,
-                        uint32_t fb) -> SwCoef<C> {
-    SwCoef<C> c;
-    c.d = C(ld(sw_load<T>(rd, ob, pl.c)));
-    c.xl = C(ld(sw_load<T>(rfx, fb, pl.fx)));
-    c.xh = C(ld(sw_load<T>(rfx, fb + kS, pl.fx)));
-    c.yl = C(ld(sw_load<T>(rfy, ob, pl.fy)));
-    c.yh = C(ld(sw_load<T>(rfy, ob, pl.fyh)));
-    c.zl = C(ld(sw_load<T>(rfz, ob, pl.c)));
-    c.zh = C(ld(sw_load<T>(rfz, ob, pl.fzh)));
-    c.b = bval_at(pl, ob);
+  auto fetch_pair = [&](const PlaneOff& pl, uint32_t ob,
+                        uint32_t fb) -> SwPairCoef<T> {
+    SwPairCoef<T> c;
+    c.d = sw_load2<T>(rd, ob, pl.c);
+    c.fx0 = sw_load<T>(rfx, fb, pl.fx);
+    c.fx1 = sw_load<T>(rfx, fb + kS, pl.fx);
+    c.fx2 = sw_load<T>(rfx, fb + 2 * kS, pl.fx);
+    c.yl = sw_load2<T>(rfy, ob, pl.fy);
+    c.yh = sw_load2<T>(rfy, ob, pl.fyh);
+    c.zl = sw_load2<T>(rfz, ob, pl.c);
+    c.zh = sw_load2<T>(rfz, ob, pl.fzh);
+    c.b = sw_load2<T>(rb, ob, pl.c);
     return c;
+  };
+  // coefficients of the pair's cell `hi` (0: x0 + 2l, 1: x0 + 2l + 1);
+  // values are widened before the select, so it picks between registers
+  auto pick = [&](const SwPairCoef<T>& c, const PlaneOff& pl, uint32_t ob,
+                  bool hi) -> SwCoef<C> {
+    auto sel = [hi](C lo, C h) -> C { return hi ? h : lo; };
+    SwCoef<C> f;
+    f.d = sel(C(ld(c.d.lo)), C(ld(c.d.hi)));
+    const C x0v = C(ld(c.fx0)), x1v = C(ld(c.fx1)), x2v = C(ld(c.fx2));
+    f.xl = sel(x0v, x1v);
+    f.xh = sel(x1v, x2v);
+    f.yl = sel(C(ld(c.yl.lo)), C(ld(c.yl.hi)));
+    f.yh = sel(C(ld(c.yh.lo)), C(ld(c.yh.hi)));
+    f.zl = sel(C(ld(c.zl.lo)), C(ld(c.zl.hi)));
+    f.zh = sel(C(ld(c.zh.lo)), C(ld(c.zh.hi)));
+    if (!kZero || a.mean == nullptr) {
+      f.b = sel(C(ld(c.b.lo)), C(ld(c.b.hi)));
+    } else {
+      const C blo = bval(pl, ob, c.b.lo), bhi = bval(pl, ob + kS, c.b.hi);
+      f.b = sel(blo, bhi);
+    }
+    return f;
+  };
+  auto fetch_cell = [&](const PlaneOff& pl, uint32_t ob,
+                        uint32_t fb) -> SwCoef<C> {
+    SwCoef<C> f;
+    f.d = C(ld(sw_load<T>(rd, ob, pl.c)));
+    f.xl = C(ld(sw_load<T>(rfx, fb, pl.fx)));
+    f.xh = C(ld(sw_load<T>(rfx, fb + kS, pl.fx)));
+    f.yl = C(ld(sw_load<T>(rfy, ob, pl.fy)));
+    f.yh = C(ld(sw_load<T>(rfy, ob, pl.fyh)));
+    f.zl = C(ld(sw_load<T>(rfz, ob, pl.c)));
+    f.zh = C(ld(sw_load<T>(rfz, ob, pl.fzh)));
+    f.b = bval(pl, ob, sw_load<T>(rb, ob, pl.c));
+    return f;
   };
   // x <- x + omega (b - A x) / D at LDS position q, in the reference
   // evaluation order; the halo supplies boundary neighbours
@@ -1071,19 +1174,9 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     v -= f.zh * Pp[q];
     P0[q] = xc + omega * (f.b - v) / f.d;
   };
-  // second-colour tile cell of slot s on plane kb: in-plane byte offsets
-  // (cells, x faces), LDS position; false outside the domain or the chunk
-  auto tile_cell = [&](int s, int kb, uint32_t& ob, uint32_t& fb,
-                       int& q) -> bool {
-    const int iy = wave + s * (kSwThreads / 64);
-    const int gj = y0 + iy;
-    const int par = (c1 + gj + kb) & 1;
-    const int gi = x0 + par + 2 * lane;
-    q = (par + 2 * lane + 2) + kSwPX * (iy + 2);
-    const bool ok = kb >= z0 && kb < z1 && gi < nx && gj < ny;
-    ob = ok ? static_cast<uint32_t>(gj * nx + gi) * kS : 0u;
-    fb = ok ? ob + static_cast<uint32_t>(gj) * kS : 0u;
-    return ok;
+  // first-colour cell of a pair in row gj on plane kg: 0 (x0 + 2l) or 1
+  auto first_hi = [&](int gj, int kg) -> bool {
+    return ((c0 + gj + kg) & 1) != 0;
   };
 
   const int kstart = z0 - 1;
@@ -1094,51 +1187,61 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     fetch_plane(kstart, v);
     store_plane(kstart, v);
   }
-  // Second-colour coefficients are fetched one step before their use, in
-  // the step whose first-colour loads touch the same cache lines of the
-  // same plane, so each coefficient line is fetched once per sweep.
-  SwCoef<C> f1[kSwTileSlots]{};
+  // second-colour coefficients of the two tile rows, carried one step
+  SwCoef<C> carry[2]{};
   for (int k = kstart; k <= z1; ++k) {
-    // ---- issue: x of plane k+1, coefficients of both colour phases
+    // ---- issue: x of plane k+1 and the coefficients of plane k
     C vx[kSwLoadSlots];
     fetch_plane(k + 1, vx);
     int kg;
     const bool first_plane = plane_of(k, kg);
     const PlaneOff pk = plane_off(first_plane ? kg : 0);
-    SwCoef<C> f0[kSwRingSlots];
-    int q0[kSwRingSlots];
-    uint32_t ok0 = 0;
+    SwPairCoef<T> pc[2];
 #pragma unroll
-    for (int s = 0; s < kSwRingSlots; ++s) {
-      const int par = static_cast<int>((rpar >> s) + static_cast<uint32_t>(kg)) & 1;
-      const bool ok = first_plane && ((rok >> (2 * s + par)) & 1u);
-      const uint32_t o = ok ? (par ? ro[s][1] : ro[s][0]) : 0u;
-      const uint32_t f = ok ? (par ? rf[s][1] : rf[s][0]) : 0u;
-      ok0 |= ok ? (1u << s) : 0u;
-      q0[s] = rq[s] + par;
-      f0[s] = fetch_coef(pk, o, f);
+    for (int s = 0; s < 2; ++s) {
+      if (first_plane && rvalid[s]) pc[s] = fetch_pair(pk, pob[s], pfb[s]);
     }
-    SwCoef<C> f1n[kSwTileSlots];
-#pragma unroll
-    for (int s = 0; s < kSwTileSlots; ++s) {
-      // second colour of plane k, used by the next step
-      uint32_t ob, fb;
-      int q;
-      tile_cell(s, k, ob, fb, q);
-      f1n[s] = fetch_coef(pk, ob, fb);
+    // ring cell of this lane (first colour only): in the ring row (waves 0
+    // and 1) or column (wave 2; column -1 when sel is 0)
+    const bool rhi = first_hi(rgj[2], kg);
+    const int sel = (c0 + 1 + cgj + kg) & 1;
+    bool ring_ok;
+    uint32_t ring_ob, ring_fb;
+    int ring_q;
+    if (wave < 2) {
+      ring_ok = rvalid[2] && lane_in;
+      ring_ob = pob[2] + (rhi ? kS : 0u);
+      ring_fb = pfb[2] + (rhi ? kS : 0u);
+      ring_q = (2 * lane + 2 + (rhi ? 1 : 0)) + kSwPX * (rr[2] + 2);
+    } else {
+      ring_ok = (cok >> sel) & 1u;
+      ring_ob = sel ? cob[1] : cob[0];
+      ring_fb = sel ? cfb[1] : cfb[0];
+      ring_q = (sel ? cright - x0 + 2 : 1) + kSwPX * ((lane & 7) + 2);
     }
+    ring_ok = ring_ok && first_plane;
+    SwCoef<C> fr{};
+    if (wave < 3 && first_plane)
+      fr = fetch_cell(pk, ring_ok ? ring_ob : 0u, ring_ok ? ring_fb : 0u);
     const int kb = k - 1;
     const bool second_plane = kb >= z0 && kb < z1;
     // ---- x of plane k+1 into LDS
     store_plane(k + 1, vx);
     __syncthreads();
+    SwCoef<C> next[2]{};
     if (first_plane) {
       const C* Pm = buf[(k - 1) & 3];
       C* P0 = buf[k & 3];
       const C* Pp = buf[(k + 1) & 3];
 #pragma unroll
-      for (int s = 0; s < kSwRingSlots; ++s)
-        if ((ok0 >> s) & 1u) update(f0[s], Pm, P0, Pp, q0[s]);
+      for (int s = 0; s < 2; ++s) {
+        if (!rvalid[s]) continue;
+        const bool hi = first_hi(rgj[s], kg);
+        const int q = (2 * lane + 2 + (hi ? 1 : 0)) + kSwPX * (rr[s] + 2);
+        if (lane_in) update(pick(pc[s], pk, pob[s], hi), Pm, P0, Pp, q);
+        next[s] = pick(pc[s], pk, pob[s], !hi);
+      }
+      if (ring_ok) update(fr, Pm, P0, Pp, ring_q);
     }
     __syncthreads();
     if (second_plane) {
@@ -1146,28 +1249,28 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
       C* P0 = buf[kb & 3];
       const C* Pp = buf[(kb + 1) & 3];
 #pragma unroll
-      for (int s = 0; s < kSwTileSlots; ++s) {
-        uint32_t ob, fb;
-        int q;
-        if (tile_cell(s, kb, ob, fb, q)) update(f1[s], Pm, P0, Pp, q);
+      for (int s = 0; s < 2; ++s) {
+        if (!rvalid[s]) continue;
+        const bool hi = !first_hi(rgj[s], kb);
+        const int q = (2 * lane + 2 + (hi ? 1 : 0)) + kSwPX * (rr[s] + 2);
+        if (lane_in) update(carry[s], Pm, P0, Pp, q);
       }
       __syncthreads();
       const PlaneOff pb = plane_off(kb);
 #pragma unroll
-      for (int i = 0; i < kSwTX * kSwTY / kSwThreads; ++i) {
-        const int p = tid + i * kSwThreads;
-        const int iy = p / kSwTX;
-        const int ix = p - iy * kSwTX;
-        const int gi = x0 + ix, gj = y0 + iy;
-        if (gi >= nx || gj >= ny) continue;
-        const uint32_t ob = static_cast<uint32_t>(gj * nx + gi) * kS;
-        sw_store(rout, ob, pb.c, st<T>(P0[(ix + 2) + kSwPX * (iy + 2)]));
-        if (kZero && a.bout != nullptr)
-          a.bout[(pb.c + ob) / kS] = st<T>(bval_at(pb, ob));
+      for (int s = 0; s < 2; ++s) {
+        if (!rvalid[s] || !lane_in) continue;
+        const int q = (2 * lane + 2) + kSwPX * (rr[s] + 2);
+        sw_store2(rout, pob[s], pb.c, st<T>(P0[q]), st<T>(P0[q + 1]));
+        if (kZero && a.bout != nullptr) {
+          const size_t c = (pb.c + pob[s]) / kS;
+          a.bout[c] = st<T>(bval(pb, pob[s], L.b[c]));
+          a.bout[c + 1] = st<T>(bval(pb, pob[s] + kS, L.b[c + 1]));
+        }
       }
     }
-#pragma unroll
-    for (int s = 0; s < kSwTileSlots; ++s) f1[s] = f1n[s];
+    carry[0] = next[0];
+    carry[1] = next[1];
   }
 }
 

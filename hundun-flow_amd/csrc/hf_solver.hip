@@ -1046,8 +1046,10 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
         std::max({static_cast<int64_t>(L.n[0] + 1) * L.n[1] * L.n[2],
                   static_cast<int64_t>(L.n[0]) * (L.n[1] + 1) * L.n[2],
                   static_cast<int64_t>(L.n[0]) * L.n[1] * (L.n[2] + 1)});
+    // (each lane owns a pair of cells of a row: nx even)
     L.sweep = l == 0 && !L.tail && L.line_mask == 0 && L.fused &&
-              L.n[0] >= 32 && L.n[1] >= 8 && sweep_enabled() &&
+              L.n[0] >= 32 && L.n[0] % 2 == 0 && L.n[1] >= 8 &&
+              sweep_enabled() &&
               widest * static_cast<int64_t>(elem_size(L.prec)) <
                   (int64_t{1} << 32);
     if (L.sweep) {
