@@ -949,6 +949,7 @@ template <class T>
 struct SwPairCoef {
   SwPair<T> d, yl, yh, zl, zh, b;
   T fx0, fx1, fx2;  // x faces of the pair: low, middle, high
+  uint32_t act;     // fluid flags of the pair (first sweep with projection)
 };
 
 // kZero: the first sweep of a smoothing from x = 0 (a.from_zero), which loads
@@ -989,6 +990,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   const SwRsrc rfz = sw_rsrc(L.fz, cells + nxy);
   const SwRsrc rb = sw_rsrc(L.b, cells);
   const SwRsrc rout = sw_rsrc(a.xout, cells);
+  const SwRsrc rbout = sw_rsrc(kZero ? a.bout : a.xout, cells);
 
   // ---- per-block set-up
   // x plane: in-plane source byte offset of every LDS position this thread
@@ -1123,11 +1125,26 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     c.zl = sw_load2<T>(rfz, ob, pl.c);
     c.zh = sw_load2<T>(rfz, ob, pl.fzh);
     c.b = sw_load2<T>(rb, ob, pl.c);
+    c.act = 0u;
+    if (kZero && a.mean != nullptr)
+      c.act = *reinterpret_cast<const uint16_t*>(L.act + (pl.c + ob) / kS);
     return c;
+  };
+  // b of the pair's two cells, projected on the first sweep
+  auto pair_b = [&](const SwPairCoef<T>& c, C& blo, C& bhi) {
+    if (!kZero || a.mean == nullptr) {
+      blo = C(ld(c.b.lo));
+      bhi = C(ld(c.b.hi));
+      return;
+    }
+    const T plo = st<T>(static_cast<double>(ld(c.b.lo)) - mean);
+    const T phi = st<T>(static_cast<double>(ld(c.b.hi)) - mean);
+    blo = (c.act & 0xffu) ? C(ld(plo)) : C(0);
+    bhi = (c.act & 0xff00u) ? C(ld(phi)) : C(0);
   };
   // coefficients of the pair's cell `hi` (0: x0 + 2l, 1: x0 + 2l + 1);
   // values are widened before the select, so it picks between registers
-  auto pick = [&](const SwPairCoef<T>& c, const PlaneOff& pl, uint32_t ob,
+  auto pick = [&](const SwPairCoef<T>& c, C blo, C bhi,
                   bool hi) -> SwCoef<C> {
     auto sel = [hi](C lo, C h) -> C { return hi ? h : lo; };
     SwCoef<C> f;
@@ -1139,12 +1156,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     f.yh = sel(C(ld(c.yh.lo)), C(ld(c.yh.hi)));
     f.zl = sel(C(ld(c.zl.lo)), C(ld(c.zl.hi)));
     f.zh = sel(C(ld(c.zh.lo)), C(ld(c.zh.hi)));
-    if (!kZero || a.mean == nullptr) {
-      f.b = sel(C(ld(c.b.lo)), C(ld(c.b.hi)));
-    } else {
-      const C blo = bval(pl, ob, c.b.lo), bhi = bval(pl, ob + kS, c.b.hi);
-      f.b = sel(blo, bhi);
-    }
+    f.b = sel(blo, bhi);
     return f;
   };
   auto fetch_cell = [&](const PlaneOff& pl, uint32_t ob,
@@ -1238,8 +1250,13 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
         if (!rvalid[s]) continue;
         const bool hi = first_hi(rgj[s], kg);
         const int q = (2 * lane + 2 + (hi ? 1 : 0)) + kSwPX * (rr[s] + 2);
-        if (lane_in) update(pick(pc[s], pk, pob[s], hi), Pm, P0, Pp, q);
-        next[s] = pick(pc[s], pk, pob[s], !hi);
+        C blo, bhi;
+        pair_b(pc[s], blo, bhi);
+        if (lane_in) update(pick(pc[s], blo, bhi, hi), Pm, P0, Pp, q);
+        next[s] = pick(pc[s], blo, bhi, !hi);
+        // the projected b of this block's planes, stored once
+        if (kZero && a.bout != nullptr && lane_in && k >= z0 && k < z1)
+          sw_store2(rbout, pob[s], pk.c, st<T>(blo), st<T>(bhi));
       }
       if (ring_ok) update(fr, Pm, P0, Pp, ring_q);
     }
@@ -1262,11 +1279,6 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
         if (!rvalid[s] || !lane_in) continue;
         const int q = (2 * lane + 2) + kSwPX * (rr[s] + 2);
         sw_store2(rout, pob[s], pb.c, st<T>(P0[q]), st<T>(P0[q + 1]));
-        if (kZero && a.bout != nullptr) {
-          const size_t c = (pb.c + pob[s]) / kS;
-          a.bout[c] = st<T>(bval(pb, pob[s], L.b[c]));
-          a.bout[c + 1] = st<T>(bval(pb, pob[s] + kS, L.b[c + 1]));
-        }
       }
     }
     carry[0] = next[0];
