@@ -64,7 +64,7 @@ def class_bytes_per_cell(cls: str, s: int):
 CLASS_KERNELS = {
     "rb_finest": "void hf::k_rb<{t}>(hf::Tiles, hf::Lv<{t}>, hf::Bc, int)",
     "sweep_finest": "void hf::k_rb_sweep<{t}, false>(hf::SweepArgs<{t}>)",
-    "operator": "hf::k_op_dot(hf::Tiles, hf::Lv<double>, hf::Bc, double const*, double*, double*)",
+    "operator": "void hf::k_stencil_z<double, 0>(hf::Tiles, hf::Lv<double>, hf::Bc, double const*, double const*, double*, double*, int)",
 }
 
 

@@ -314,17 +314,6 @@ __global__ void k_rb_separated(Tiles t, Lv<T> L, int color) {
                  omega * (C(ld(L.b[c])) - C(ld(L.r[c]))) / C(ld(L.D[c])));
 }
 
-// r := A x (defect = 0) or r := b - A x (defect = 1).
-template <class T>
-__global__ void k_level_apply(Tiles t, Lv<T> L, Bc bc, int defect) {
-  using C = typename Compute<T>::type;
-  int i, j, k;
-  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), i, j, k);
-  if (i >= t.nx || j >= t.ny) return;
-  const int c = cell_index(t.nx, t.ny, i, j, k);
-  const C v = stencil<C>(L, bc, L.x, i, j, k, c, C(ld(L.x[c])));
-  L.r[c] = defect ? st<T>(C(ld(L.b[c])) - v) : st<T>(v);
-}
 
 // Restriction (src/solver_mg.cpp:3633-3720): coarse b = sum of children in
 // dz, dy, dx order, coarse x := 0 (skipped when the next smoothing starts
@@ -714,52 +703,95 @@ k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
 // level-0 coefficients, which equal the exact IBM operator.
 // ---------------------------------------------------------------------------
 
-// q = A p fused with the partial sum of p.q.
-__global__ void k_op_dot(Tiles t, Lv<double> L, Bc bc, const double* p,
-                         double* q, double* partials) {
+
+// ---------------------------------------------------------------------------
+// z-marching stencil kernels.  A block owns one 2-D tile of the plane (the
+// Tiles layout with k = 0) and t.per consecutive planes of it; each thread
+// walks its cell column upwards, carrying x at k-1 and k and the z-face
+// coefficient between steps, so a step loads x and the coefficients of one
+// new plane and the in-plane neighbours (cache hits from the tile's rows).
+// Operand values and the order of operations are those of stencil():
+//   kZOpDot    q = A p and the partial sum of p.q (Krylov operator)
+//   kZResidual r = 1 b + (-1) A x and the partial sum of r.r (the true
+//              residual, axpby(r, 1, b, -1, Ax): src/solver_krylov.cpp:
+//              1082-1091)
+//   kZApply    r = b - A x (defect) or r = A x at level storage
+// ---------------------------------------------------------------------------
+constexpr int kZOpDot = 0, kZResidual = 1, kZApply = 2;
+
+template <class T, int kMode>
+__global__ void k_stencil_z(Tiles t, Lv<T> L, Bc bc, const T* x, const T* b,
+                            T* out, double* partials, int defect) {
+  using C = typename Compute<T>::type;
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
   double d = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
-    int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
-    const double pc = p[c];
-    const double v = stencil<double>(L, bc, p, i, j, k, c, pc);
-    q[c] = v;
-    d += pc * v;
+  if (i < nx && j < ny) {
+    const int plane = nx * ny;
+    int c = cell_index(nx, ny, i, j, k0);
+    C xc = C(ld(x[c]));
+    C zl = k0 > 0 ? C(ld(x[c - plane]))
+                  : neighbour_or_boundary<C>(bc.m[4], xc, x,
+                                             c + (nz - 1) * plane);
+    C fzl = C(ld(L.fz[c]));
+    for (int k = k0; k < k1; ++k) {
+      const C zh = k + 1 < nz
+                       ? C(ld(x[c + plane]))
+                       : neighbour_or_boundary<C>(bc.m[5], xc, x,
+                                                  c - (nz - 1) * plane);
+      const C fzh = C(ld(L.fz[c + plane]));
+      const int row = j + ny * k;
+      const C xl = i > 0 ? C(ld(x[c - 1]))
+                         : neighbour_or_boundary<C>(bc.m[0], xc, x,
+                                                    c + (nx - 1));
+      const C xh = i + 1 < nx
+                       ? C(ld(x[c + 1]))
+                       : neighbour_or_boundary<C>(bc.m[1], xc, x,
+                                                  c - (nx - 1));
+      const C yl = j > 0 ? C(ld(x[c - nx]))
+                         : neighbour_or_boundary<C>(bc.m[2], xc, x,
+                                                    c + (ny - 1) * nx);
+      const C yh = j + 1 < ny
+                       ? C(ld(x[c + nx]))
+                       : neighbour_or_boundary<C>(bc.m[3], xc, x,
+                                                  c - (ny - 1) * nx);
+      const int fyo = c + nx * k;
+      C v = C(ld(L.D[c])) * xc;
+      v -= C(ld(L.fx[c + row])) * xl;
+      v -= C(ld(L.fx[c + row + 1])) * xh;
+      v -= C(ld(L.fy[fyo])) * yl;
+      v -= C(ld(L.fy[fyo + nx])) * yh;
+      v -= fzl * zl;
+      v -= fzh * zh;
+      if constexpr (kMode == kZOpDot) {
+        out[c] = v;
+        d += xc * v;
+      } else if constexpr (kMode == kZResidual) {
+        const double rc = 1.0 * b[c] + -1.0 * v;
+        out[c] = rc;
+        d += rc * rc;
+      } else {
+        out[c] = defect ? st<T>(C(ld(b[c])) - v) : st<T>(v);
+      }
+      zl = xc;
+      xc = zh;
+      fzl = fzh;
+      c += plane;
+    }
   }
-  if (partials != nullptr) {
+  if (kMode != kZApply && partials != nullptr) {
     d = block_sum(d, scratch);
     if (threadIdx.x == 0) partials[lb] = d;
   }
 }
 
-// r = b - A x (axpby(r, 1, b, -1, Ax), src/solver_krylov.cpp:1082-1091)
-// fused with the partial sum of r.r.
-__global__ void k_true_residual(Tiles t, Lv<double> L, Bc bc, const double* b,
-                                const double* x, double* r, double* partials) {
-  __shared__ double scratch[16];
-  const int lb = xcd_block32(blockIdx.x, gridDim.x);
-  const int t0 = lb * t.per;
-  const int t1 = min(t.count, t0 + t.per);
-  double d = 0.0;
-  for (int tile = t0; tile < t1; ++tile) {
-    int i, j, k;
-    tile_xyz(t, tile, i, j, k);
-    if (i >= t.nx || j >= t.ny) continue;
-    const int c = cell_index(t.nx, t.ny, i, j, k);
-    const double v = stencil<double>(L, bc, x, i, j, k, c, x[c]);
-    const double rc = 1.0 * b[c] + -1.0 * v;
-    r[c] = rc;
-    d += rc * rc;
-  }
-  d = block_sum(d, scratch);
-  if (threadIdx.x == 0) partials[lb] = d;
-}
 
 }  // namespace hf
 
@@ -816,6 +848,14 @@ struct SweepArgs {
   int zc;              // planes per block
   int gx, gy;          // tiles along x and y
 };
+
+// A value updated in LDS rounded to the storage type, as if it had been
+// stored and loaded again (the two colour passes read the first colour's
+// values from storage); exact for 32- and 64-bit storage.
+template <class T, class C>
+__device__ __forceinline__ C sw_round(C v) {
+  return C(ld(st<T>(v)));
+}
 
 // Coefficients of one cell update, fetched ahead of the LDS phase.
 template <class C>
@@ -1184,7 +1224,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     v -= f.yh * P0[q + kSwPX];
     v -= f.zl * Pm[q];
     v -= f.zh * Pp[q];
-    P0[q] = xc + omega * (f.b - v) / f.d;
+    P0[q] = sw_round<T>(xc + omega * (f.b - v) / f.d);
   };
   // first-colour cell of a pair in row gj on plane kg: 0 (x0 + 2l) or 1
   auto first_hi = [&](int gj, int kg) -> bool {
@@ -1502,7 +1542,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep_cols(SweepArgs<T> a) {
     v -= f.yh * yh;
     v -= f.zl * zl;
     v -= f.zh * zh;
-    const C xn = xc + omega * (f.b - v) / f.d;
+    const C xn = sw_round<T>(xc + omega * (f.b - v) / f.d);
     P0[q] = xn;
     return xn;
   };

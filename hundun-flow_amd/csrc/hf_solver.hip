@@ -102,6 +102,7 @@ struct Level {
   Tiles tcol{};
   Tiles tr{};
   Tiles tred{};  // kRedPer tiles per block: reduction-only kernels
+  Tiles tz{};    // 2-D tiles x z-chunks of t.per planes: z-marching stencils
 };
 
 constexpr int kFoldSpan = 2048;   // partials per block of the first fold
@@ -556,6 +557,15 @@ Widths widths(const hf_plan* p) {
   return Widths{p->d_widths[0], p->d_widths[1], p->d_widths[2]};
 }
 
+// r = b - A x (defect) or r = A x on level l, z-marching.
+template <class T>
+void level_apply(hf_plan* p, const Level& L, const Lv<T>& v, int defect) {
+  HF_LAUNCH((k_stencil_z<T, kZApply>), L.tz.blocks, kBlock, 0, p->stream,
+            L.tz, v, p->bc, static_cast<const T*>(v.x),
+            static_cast<const T*>(v.b), v.r, static_cast<double*>(nullptr),
+            defect);
+}
+
 // Mean of a level field for the constant projection: scalars[S_MEAN0 + l].
 template <class T>
 void level_mean(hf_plan* p, int l, const T* field) {
@@ -621,8 +631,7 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
   }
   if (retain) {
     HF_PROF(p, P_DEFECT);
-    HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
-              view<T>(L), p->bc, 1);
+    level_apply<T>(p, L, view<T>(L), 1);
   }
 }
 
@@ -699,8 +708,7 @@ void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     line_smooth<T>(p, l, sweeps, reverse);
     if (retain) {
       HF_PROF(p, P_DEFECT);
-      HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
-                view<T>(L), p->bc, 1);
+      level_apply<T>(p, L, view<T>(L), 1);
     }
     return;
   }
@@ -723,8 +731,7 @@ void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
                            : nullptr;
   if (!L.fused && !from_zero) {
     HF_PROF(p, P_DEFECT);
-    HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc, v,
-              p->bc, 0);
+    level_apply<T>(p, L, v, 0);
   }
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     const int first = reverse ? 1 : 0;
@@ -747,8 +754,7 @@ void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
       if (!final_update || retain) {
         if (!L.fused || (final_update && retain)) {
           HF_PROF(p, P_DEFECT);
-          HF_LAUNCH(k_level_apply<T>, L.tc.blocks, kBlock, 0, p->stream, L.tc,
-                    v, p->bc, (retain && final_update) ? 1 : 0);
+          level_apply<T>(p, L, v, (retain && final_update) ? 1 : 0);
         }
       }
     }
@@ -859,8 +865,7 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
       }
       {
         HF_PROF(p, P_DEFECT);
-        HF_LAUNCH(k_level_apply<T>, L0.tc.blocks, kBlock, 0, p->stream, L0.tc,
-                  view<T>(L0), p->bc, 1);
+        level_apply<T>(p, L0, view<T>(L0), 1);
       }
       {
         HF_PROF(p, P_MG_ENTRY_EXIT);
@@ -1039,6 +1044,14 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tred = L.tr;
     L.tred.per = kRedPer;
     L.tred.blocks = (L.tr.count + kRedPer - 1) / kRedPer;
+    {
+      // z-chunks long enough to reuse planes, about 2048 blocks in all
+      L.tz = L.tr;
+      const int gxy = L.tz.gx * L.tz.gy;
+      const int64_t want = static_cast<int64_t>(L.n[2]) * gxy / 2048;
+      L.tz.per = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(64, want)));
+      L.tz.blocks = gxy * ((L.n[2] + L.tz.per - 1) / L.tz.per);
+    }
     // fused sweeps on grid levels with the fused colour route and rows long
     // enough for the tiles to stay mostly busy
     // (its buffer loads take 32-bit byte offsets: every array under 4 GiB)
@@ -1306,9 +1319,10 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
 hf_status hf_operator_apply(hf_plan* p, const double* x, double* y) {
   if (p == nullptr || x == nullptr || y == nullptr || !p->coefficients_ready)
     return fail(HF_INVALID_PLAN, kDetailPlan);
-  const Tiles& t = p->levels[0].tr;
-  HF_LAUNCH(k_op_dot, t.blocks, kBlock, 0, p->stream, t, krylov_view(p),
-            p->bc, x, y, static_cast<double*>(nullptr));
+  const Tiles& t = p->levels[0].tz;
+  HF_LAUNCH((k_stencil_z<double, kZOpDot>), t.blocks, kBlock, 0, p->stream, t,
+            krylov_view(p), p->bc, x, static_cast<const double*>(nullptr), y,
+            static_cast<double*>(nullptr), 0);
   return hip_check(hipGetLastError());
 }
 
@@ -1326,7 +1340,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                           const hf_solve_control* ctl, hf_solve_result* res) {
   const int64_t N = p->cells;
   const Lv<double> A = krylov_view(p);
-  const Tiles& tr = p->levels[0].tr;
+  const Tiles& tr = p->levels[0].tz;  // z-marching Krylov stencils
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
   double* x = p->kx;
@@ -1358,8 +1372,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, x, solution);
   {
     HF_PROF(p, P_TRUE_RESIDUAL);
-    HF_LAUNCH(k_true_residual, tr.blocks, kBlock, 0, p->stream, tr, A, p->bc,
-              rhs, x, r, p->partials);
+    HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
+              p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs, r,
+              p->partials, 0);
   }
   finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
   res->operator_applies++;
@@ -1390,8 +1405,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   while (res->iterations < ctl->maximum_iterations) {
     {
       HF_PROF(p, P_OPERATOR);
-      HF_LAUNCH(k_op_dot, tr.blocks, kBlock, 0, p->stream, tr, A, p->bc, pp, q,
-                p->partials);
+      HF_LAUNCH((k_stencil_z<double, kZOpDot>), tr.blocks, kBlock, 0,
+                p->stream, tr, A, p->bc, static_cast<const double*>(pp),
+                static_cast<const double*>(nullptr), q, p->partials, 0);
     }
     res->operator_applies++;
     finalize(p, tr.blocks, 1, F_ALPHA, 0);
@@ -1418,8 +1434,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     if (verify) {
       {
         HF_PROF(p, P_TRUE_RESIDUAL);
-        HF_LAUNCH(k_true_residual, tr.blocks, kBlock, 0, p->stream, tr, A,
-                  p->bc, rhs, x, r, p->partials);
+        HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
+                  p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs,
+                  r, p->partials, 0);
       }
       finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
       res->operator_applies++;

@@ -288,3 +288,70 @@ def test_fused_sweep_schedules_agree(torch_cuda, monkeypatch, name, precision):
     z_tile = plan.mg_apply(r).cpu().numpy()
     assert np.array_equal(z_cols, z_tile), np.max(np.abs(z_cols - z_tile))
     plan.close()
+
+
+def _walls(n, bc, seed):
+    case = hf_cases.lognormal_sphere(n=n, seed=seed, radius=0.2)
+    act = case["cell_active"].reshape(n[2], n[1], n[0])
+    fx, fy, fz = hf_cases.face_activity(act, bc)
+    case.update(bc=bc, x_active=fx, y_active=fy, z_active=fz)
+    return case
+
+
+# Finest levels run by the fused sweep with every boundary kind in its halo:
+# Neumann and Dirichlet mixes per axis, periodic faces on tiles cut short by
+# the domain (the wrapped ring cell then sits inside the tile's last column
+# or row), and an odd x extent (planned on the two-pass route instead).
+SWEEP_EDGES = {
+    "neumann_all": lambda: _walls((160, 20, 12), [1, 1, 1, 1, 1, 1], 75),
+    "mixed_dn": lambda: _walls((136, 18, 10), [0, 1, 1, 0, 0, 1], 76),
+    "perz_neumann_xy": lambda: _walls((128, 16, 8), [1, 1, 0, 1, 2, 2], 77),
+    "periodic_x_partial": lambda: _walls((160, 16, 8), [2, 2, 1, 1, 0, 0], 79),
+    "periodic_xy_partial": lambda: _walls((200, 12, 6), [2, 2, 2, 2, 1, 1],
+                                          80),
+    "odd_x": lambda: _walls((97, 13, 9), [0, 0, 0, 0, 0, 0], 78),
+}
+
+
+@pytest.mark.parametrize("name", list(SWEEP_EDGES))
+@pytest.mark.parametrize("f_cycle", [0, 1])
+def test_fused_sweep_edges_fp64_bitwise(torch_cuda, name, f_cycle):
+    """FP64 cycle through the fused sweep equals the oracle bit for bit."""
+    torch = torch_cuda
+    _say(f"[sweep edges fp64] case={name} f_cycle={f_cycle}")
+    case = SWEEP_EDGES[name]()
+    pol = dict(null_space_constant=0, f_cycle=f_cycle,
+               anisotropy_threshold=1e9)
+    plan = _gpu_plan(torch, case, **pol)
+    _, mg = O.from_case(case, O.policy(**pol))
+    r = np.random.default_rng(5).standard_normal(plan.cells)
+    z = plan.mg_apply(_t(torch, r)).cpu().numpy()
+    zo = mg.apply(r)
+    assert np.array_equal(z, zo), np.max(np.abs(z - zo))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", list(SWEEP_EDGES))
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("null", [0, 1])
+def test_fused_sweep_edges_match_two_pass(torch_cuda, monkeypatch, name,
+                                          precision, null):
+    """32- and 16-bit storage: the fused sweep and the two colour passes
+    give the same cycle bit for bit (with the constant null space too: the
+    first sweep then applies the projection of b on the fly)."""
+    torch = torch_cuda
+    _say(f"[sweep edges] case={name} precision={precision} null={null}")
+    case = SWEEP_EDGES[name]()
+    pol = dict(null_space_constant=null, anisotropy_threshold=1e9,
+               precision=precision)
+    r = _t(torch, np.random.default_rng(8).standard_normal(
+        int(np.prod(case["n"]))))
+    out = {}
+    for sweep in ("1", "0"):
+        monkeypatch.setenv("HF_FUSED_SWEEP", sweep)
+        plan = _gpu_plan(torch, case, **pol)
+        out[sweep] = plan.mg_apply(r).cpu().numpy()
+        plan.close()
+    # the projection means come from the same reductions on either route
+    assert np.array_equal(out["1"], out["0"]), \
+        np.max(np.abs(out["1"] - out["0"]))
