@@ -1045,10 +1045,11 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     L.tred.per = kRedPer;
     L.tred.blocks = (L.tr.count + kRedPer - 1) / kRedPer;
     {
-      // z-chunks long enough to reuse planes, about 2048 blocks in all
+      // z-chunks of a few planes (measured fastest: many blocks in flight,
+      // each reusing its planes)
       L.tz = L.tr;
       const int gxy = L.tz.gx * L.tz.gy;
-      const int64_t want = static_cast<int64_t>(L.n[2]) * gxy / 2048;
+      const int64_t want = static_cast<int64_t>(L.n[2]) * gxy / 32768;
       L.tz.per = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(64, want)));
       L.tz.blocks = gxy * ((L.n[2] + L.tz.per - 1) / L.tz.per);
     }
