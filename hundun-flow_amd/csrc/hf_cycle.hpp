@@ -397,10 +397,9 @@ __device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
                                              int mask, int per, int i, int j,
                                              int k, int c) {
   using C = typename Compute<Tf>::type;
-  if (!F.act[c]) {
-    F.x[c] = st<Tf>(0.0);
-    return;
-  }
+  // the fluid flag, x and the corners are loaded together (one memory
+  // round trip); solid cells store zero
+  const bool fluid = F.act[c] != 0;
   int xl, xr, yl, yr, zl, zr, xw, yw, zw;
   interp(i, mask & 1, Cl.nx, per & 1, &xl, &xr, &xw);
   interp(j, (mask >> 1) & 1, Cl.ny, (per >> 1) & 1, &yl, &yr, &yw);
@@ -438,7 +437,7 @@ __device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
       }
     }
   }
-  F.x[c] = st<Tf>(xf + value);
+  F.x[c] = fluid ? st<Tf>(xf + value) : st<Tf>(0.0);
 }
 
 template <class Tc, class Tf>
