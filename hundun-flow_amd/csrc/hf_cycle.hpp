@@ -198,6 +198,61 @@ __global__ void k_vol_dot(Tiles t, const T* field, const double* vol,
   if (threadIdx.x == 0) partials[lb] = v;
 }
 
+// The PCG update x += a p, r -= a q of k_update_xr fused with the
+// preconditioner's entry (k_mg_entry): b = fluid ? r : 0 at the level-0
+// storage type.  Partial sums: r.r in accumulator 0 and, with the constant
+// null space, vol*b in accumulator 1 (stride apart).  Tiles of the reduction
+// layout (kRedPer per block).
+template <class T>
+__global__ void k_update_xr_entry(Tiles t, double* x, const double* pv,
+                                  double* r, const double* q,
+                                  const double* alpha, const uint8_t* act,
+                                  Widths w, T* b, double* partials,
+                                  int stride, int proj) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const double a = *alpha;
+  // two groups of four tiles: each group's loads are issued together (eight
+  // tiles of five FP64 operands would not fit the register file)
+  constexpr int kGroup = 4;
+  static_assert(kRedPer % kGroup == 0, "tile groups");
+  double d = 0.0, m = 0.0;
+#pragma unroll
+  for (int g = 0; g < kRedPer; g += kGroup) {
+    double xv[kGroup], pp[kGroup], rv[kGroup], qv[kGroup], vol[kGroup];
+    bool ok[kGroup], fluid[kGroup];
+    int cc[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      int i, j, k;
+      cc[u] = red_cell(t, lb, g + u, i, j, k, ok[u]);
+      xv[u] = x[cc[u]];
+      pp[u] = pv[cc[u]];
+      rv[u] = r[cc[u]];
+      qv[u] = q[cc[u]];
+      fluid[u] = act[cc[u]] != 0;
+      vol[u] = w.x[i] * w.y[j] * w.z[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      if (!ok[u]) continue;
+      x[cc[u]] = xv[u] + a * pp[u];
+      const double rc = rv[u] + -a * qv[u];
+      r[cc[u]] = rc;
+      d += rc * rc;
+      const T bv = fluid[u] ? st<T>(rc) : st<T>(0.0);
+      b[cc[u]] = bv;
+      if (proj && fluid[u]) m += vol[u] * static_cast<double>(ld(bv));
+    }
+  }
+  d = block_sum(d, scratch);
+  if (threadIdx.x == 0) partials[lb] = d;
+  if (proj) {
+    m = block_sum(m, scratch);
+    if (threadIdx.x == 0) partials[stride + lb] = m;
+  }
+}
+
 // f := fluid ? f - mean : 0
 template <class T>
 __global__ void k_project_apply(Tiles t, T* field, const uint8_t* act,

@@ -511,10 +511,13 @@ void build_exec(hf_plan* p) {
 }
 
 // ---- reductions -----------------------------------------------------------
+// acc0: index of the first accumulator read (partials of accumulator a sit
+// at p->partials + a * partial_stride).
 void finalize(hf_plan* p, int64_t blocks, int nacc, int op, int slot,
-              double volume = 0.0, int unit_linear = 1) {
+              double volume = 0.0, int unit_linear = 1, int acc0 = 0) {
   HF_PROF(p, P_FINALIZE);
-  const double* src = p->partials;
+  double* first = p->partials + static_cast<int64_t>(acc0) * p->partial_stride;
+  const double* src = first;
   int64_t count = blocks;
   int64_t stride = p->partial_stride;
   if (blocks > kFoldSpan) {
@@ -522,7 +525,7 @@ void finalize(hf_plan* p, int64_t blocks, int nacc, int op, int slot,
     const int64_t spans = (blocks + kFoldSpan - 1) / kFoldSpan;
     hipLaunchKernelGGL(k_fold_partials, dim3(static_cast<uint32_t>(spans),
                                              static_cast<uint32_t>(nacc)),
-                       dim3(kBlock), 0, p->stream, p->partials, blocks,
+                       dim3(kBlock), 0, p->stream, first, blocks,
                        p->partial_stride, p->folded, kFoldStride);
     src = p->folded;
     count = spans;
@@ -824,8 +827,11 @@ void run_cycle(hf_plan* p) {
 }
 
 // z = M r on device arrays; optional fused Krylov dots (r.z, z.q).
+// entered: b of level 0 (and, with the null space, the partial sums of its
+// mean in accumulator 1) were stored by the PCG update (k_update_xr_entry);
+// only when level 0 is a grid level (not the tail, not line relaxed).
 hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
-                   bool dots) {
+                   bool dots, bool entered = false) {
   Level& L0 = p->levels[0];
   const bool proj = p->policy.null_space_constant != 0;
   // whole cycle inside the tail kernel, or a line level 0: x = 0 and the
@@ -835,7 +841,7 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
     T* x = static_cast<T*>(L0.x);
-    {
+    if (!entered) {
       HF_PROF(p, P_MG_ENTRY_EXIT);
       HF_LAUNCH(k_mg_entry<T>, L0.tred.blocks, kBlock, 0, p->stream, L0.tred, r,
                 static_cast<const uint8_t*>(L0.act), widths(p), b,
@@ -843,7 +849,8 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
                 proj ? p->partials : static_cast<double*>(nullptr));
     }
     if (proj) {
-      finalize(p, L0.tred.blocks, 1, F_MEAN, S_MEAN0, L0.volume);
+      finalize(p, L0.tred.blocks, 1, F_MEAN, S_MEAN0, L0.volume, 1,
+               entered ? 1 : 0);
       if (tail0) project_apply<T>(p, 0, b);
     }
   });
@@ -1342,6 +1349,10 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   const int64_t N = p->cells;
   const Lv<double> A = krylov_view(p);
   const Tiles& tr = p->levels[0].tz;  // z-marching Krylov stencils
+  // the PCG update stores the preconditioner's entry when level 0 is a grid
+  // level (the tail and line-relaxed entries also zero x and project b)
+  const bool fuse_entry =
+      !p->levels[0].tail && p->levels[0].line_mask == 0;
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
   double* x = p->kx;
@@ -1412,12 +1423,28 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     }
     res->operator_applies++;
     finalize(p, tr.blocks, 1, F_ALPHA, 0);
-    {
+    if (fuse_entry) {
+      // the update also stores the preconditioner's input b (used unless
+      // this iteration verifies the true residual, which replaces r)
+      HF_PROF(p, P_UPDATE_XR);
+      Level& L0 = p->levels[0];
+      dispatch(L0.prec, [&](auto t) {
+        using T = decltype(t);
+        HF_LAUNCH(k_update_xr_entry<T>, L0.tred.blocks, kBlock, 0, p->stream,
+                  L0.tred, x, static_cast<const double*>(pp), r,
+                  static_cast<const double*>(q), p->scalars + S_ALPHA,
+                  static_cast<const uint8_t*>(L0.act), widths(p),
+                  static_cast<T*>(L0.b), p->partials,
+                  static_cast<int>(p->partial_stride),
+                  p->policy.null_space_constant != 0 ? 1 : 0);
+      });
+    } else {
       HF_PROF(p, P_UPDATE_XR);
       HF_LAUNCH(k_update_xr, nbv, kBlock, 0, p->stream, ch, x, pp, r, q,
                 p->scalars + S_ALPHA, p->partials);
     }
-    finalize(p, nbv, 1, F_STORE, S_RNORM2);
+    finalize(p, fuse_entry ? p->levels[0].tred.blocks : nbv, 1, F_STORE,
+             S_RNORM2);
     res->iterations++;
     HF_TRY(read_scalars(p));
     const double pap = p->host_scalars[S_PAP];
@@ -1455,7 +1482,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       HF_TRY(precondition_restart());
       continue;
     }
-    HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true));
+    HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true, fuse_entry));
     res->preconditioner_applies++;
     finalize(p, p->levels[0].tred.blocks, ctl->flexible ? 2 : 1,
              ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
