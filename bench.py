@@ -47,6 +47,9 @@ def class_bytes_per_cell(cls: str, s: int):
         # fused two-colour sweep: read x, b, D, 3 face arrays, write x (the
         # first sweep from zero reads no x and writes the projected b)
         "sweep_finest": 7 * s,
+        # the cycle's last fused sweep, which also sums the level-0 mean:
+        # the sweep's arrays and the fluid flags
+        "sweep_mean": 7 * s + 1,
         # retained defect r = b - A x: x, b, D, 3 faces, r
         "defect": 7 * s,
         # Krylov q = A p + dot: p, D, 3 faces, q (FP64)
@@ -63,7 +66,7 @@ def class_bytes_per_cell(cls: str, s: int):
 # kernel class -> symbol of its kernel for the rocprof traffic lookup
 CLASS_KERNELS = {
     "rb_finest": "void hf::k_rb<{t}>(hf::Tiles, hf::Lv<{t}>, hf::Bc, int)",
-    "sweep_finest": "void hf::k_rb_sweep<{t}, false>(hf::SweepArgs<{t}>)",
+    "sweep_finest": "void hf::k_rb_sweep<{t}, false, false>(hf::SweepArgs<{t}>)",
     "operator": "void hf::k_stencil_z<double, 0>(hf::Tiles, hf::Lv<double>, hf::Bc, double const*, double const*, double*, double*, int)",
 }
 

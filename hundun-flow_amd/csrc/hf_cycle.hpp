@@ -901,6 +901,12 @@ struct SweepArgs {
   int from_zero;
   int zc;              // planes per block
   int gx, gy;          // tiles along x and y
+  // with kMean: per-block partial sums of vol * x over fluid cells of the
+  // output (the constant projection's mean), cell widths per axis
+  double* partials;
+  const double* wx;
+  const double* wy;
+  const double* wz;
 };
 
 // A value updated in LDS rounded to the storage type, as if it had been
@@ -1048,11 +1054,14 @@ struct SwPairCoef {
 
 // kZero: the first sweep of a smoothing from x = 0 (a.from_zero), which loads
 // no x and applies a pending projection of b; otherwise a.from_zero is 0 and
-// a.mean is null.
-template <class T, bool kZero>
+// a.mean is null.  kMean: the block also sums vol * x over the fluid cells
+// it stores (the terms of k_vol_dot_finest) into a.partials.
+template <class T, bool kZero, bool kMean = false>
 __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   using C = typename Compute<T>::type;
   __shared__ C buf[4][kSwPlane];
+  __shared__ double red[16];
+  double msum = 0.0;
   const Lv<T>& L = a.L;
   int blk = xcd_block32(blockIdx.x, gridDim.x);
   // the tile coordinates are block-uniform: held in scalar registers, every
@@ -1134,6 +1143,18 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     pob[s] = lok ? static_cast<uint32_t>(gj * nx + x0 + 2 * lane) * kS : 0u;
     pfb[s] = lok ? static_cast<uint32_t>(gj * (nx + 1) + x0 + 2 * lane) * kS
                  : 0u;
+  }
+  // kMean: widths of the pair's cells and of the two tile rows (constant
+  // over the block's planes)
+  double mwx0 = 0.0, mwx1 = 0.0, mwy[2] = {0.0, 0.0};
+  if (kMean) {
+    if (lane_in) {
+      mwx0 = a.wx[x0 + 2 * lane];
+      mwx1 = a.wx[x0 + 2 * lane + 1];
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+      if (rvalid[s]) mwy[s] = a.wy[rgj[s]];
   }
   // ring columns (wave 2, lanes 0..7 = tile rows): byte offsets of the cell
   // left of the tile (sel 0) and right of its last column inside the domain
@@ -1220,7 +1241,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     c.zh = sw_load2<T>(rfz, ob, pl.fzh);
     c.b = sw_load2<T>(rb, ob, pl.c);
     c.act = 0u;
-    if (kZero && a.mean != nullptr)
+    if ((kZero && a.mean != nullptr) || kMean)
       c.act = *reinterpret_cast<const uint16_t*>(L.act + (pl.c + ob) / kS);
     return c;
   };
@@ -1293,8 +1314,10 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     fetch_plane(kstart, v);
     store_plane(kstart, v);
   }
-  // second-colour coefficients of the two tile rows, carried one step
+  // second-colour coefficients of the two tile rows, carried one step (with
+  // kMean, the fluid flags of the pair too)
   SwCoef<C> carry[2]{};
+  uint32_t act_carry[2] = {0u, 0u};
   for (int k = kstart; k <= z1; ++k) {
     // ---- issue: x of plane k+1 and the coefficients of plane k
     C vx[kSwLoadSlots];
@@ -1335,6 +1358,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     store_plane(k + 1, vx);
     __syncthreads();
     SwCoef<C> next[2]{};
+    uint32_t act_next[2] = {0u, 0u};
     if (first_plane) {
       const C* Pm = buf[(k - 1) & 3];
       C* P0 = buf[k & 3];
@@ -1348,6 +1372,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
         pair_b(pc[s], blo, bhi);
         if (lane_in) update(pick(pc[s], blo, bhi, hi), Pm, P0, Pp, q);
         next[s] = pick(pc[s], blo, bhi, !hi);
+        if (kMean) act_next[s] = pc[s].act;
         // the projected b of this block's planes, stored once
         if (kZero && a.bout != nullptr && lane_in && k >= z0 && k < z1)
           sw_store2(rbout, pob[s], pk.c, st<T>(blo), st<T>(bhi));
@@ -1372,11 +1397,26 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
       for (int s = 0; s < 2; ++s) {
         if (!rvalid[s] || !lane_in) continue;
         const int q = (2 * lane + 2) + kSwPX * (rr[s] + 2);
-        sw_store2(rout, pob[s], pb.c, st<T>(P0[q]), st<T>(P0[q + 1]));
+        const T lo = st<T>(P0[q]), hi = st<T>(P0[q + 1]);
+        sw_store2(rout, pob[s], pb.c, lo, hi);
+        if (kMean) {
+          // the terms of k_vol_dot_finest: ((wx wy) wz) x on fluid cells
+          const double wz = a.wz[kb];
+          const double vlo = mwx0 * mwy[s] * wz;
+          const double vhi = mwx1 * mwy[s] * wz;
+          if (act_carry[s] & 0xffu) msum += vlo * static_cast<double>(ld(lo));
+          if (act_carry[s] & 0xff00u) msum += vhi * static_cast<double>(ld(hi));
+        }
       }
     }
     carry[0] = next[0];
     carry[1] = next[1];
+    act_carry[0] = act_next[0];
+    act_carry[1] = act_next[1];
+  }
+  if (kMean) {
+    msum = block_sum(msum, red);
+    if (threadIdx.x == 0) a.partials[xcd_block32(blockIdx.x, gridDim.x)] = msum;
   }
 }
 

@@ -42,7 +42,8 @@ enum ProfileClass : int {
   P_SWEEP_COARSE = 15,  // fused two-colour sweep on grid levels >= 1
   P_LINE = 16,          // line relaxation (any level)
   P_RB_FIRST = 17,      // first colour pass, or fused sweep, from x = 0
-  kProfileClasses = 18
+  P_SWEEP_MEAN = 18,    // fused sweep that also sums the level-0 mean
+  kProfileClasses = 19
 };
 constexpr uint32_t kDetailPlan = 9101U;
 constexpr uint32_t kDetailAlloc = 9102U;
@@ -594,9 +595,11 @@ void project_apply(hf_plan* p, int l, T* field) {
             static_cast<const double*>(p->scalars + S_MEAN0 + l));
 }
 
+// mean_partials: the last sweep also writes the per-block partial sums of the
+// level's constant-projection mean of its output (k_rb_sweep with kMean).
 template <class T>
 void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
-                  int from_zero) {
+                  int from_zero, double* mean_partials = nullptr) {
   Level& L = p->levels[l];
   const int rb_class = l == 0 ? P_SWEEP_FINEST : P_SWEEP_COARSE;
   const double* mean = (from_zero && p->policy.null_space_constant)
@@ -615,8 +618,13 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     a.zc = L.sweep_zc;
     a.gx = L.sweep_gx;
     a.gy = L.sweep_gy;
+    const bool with_mean = mean_partials != nullptr && sweep == sweeps - 1;
+    a.partials = with_mean ? mean_partials : nullptr;
+    a.wx = p->d_widths[0];
+    a.wy = p->d_widths[1];
+    a.wz = p->d_widths[2];
     {
-      HF_PROF(p, zero ? P_RB_FIRST : rb_class);
+      HF_PROF(p, zero ? P_RB_FIRST : with_mean ? P_SWEEP_MEAN : rb_class);
       // the column-owner schedule exists for 32- and 16-bit storage only
       // (its FP64 register footprint would leave one wave per SIMD)
       if (sizeof(T) < sizeof(double) && sweep_columns())
@@ -624,6 +632,9 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
                   p->stream, a);
       else if (zero)
         HF_LAUNCH((k_rb_sweep<T, true>), L.sweep_blocks, kSwThreads, 0,
+                  p->stream, a);
+      else if (with_mean)
+        HF_LAUNCH((k_rb_sweep<T, false, true>), L.sweep_blocks, kSwThreads, 0,
                   p->stream, a);
       else
         HF_LAUNCH((k_rb_sweep<T, false>), L.sweep_blocks, kSwThreads, 0,
@@ -640,7 +651,7 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
 
 template <class T>
 void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
-                  int from_zero);
+                  int from_zero, double* mean_partials);
 
 // Line geometry of axis ax on level L: lines (a, b), E cells each.
 template <class T>
@@ -676,7 +687,7 @@ void line_smooth(hf_plan* p, int l, int sweeps, int reverse) {
     const int ax = reverse ? 2 - i : i;
     if ((L.line_mask & (1 << ax)) == 0) continue;
     if (L.n[ax] > kLineMaximumExtent) {
-      point_smooth<T>(p, l, sweeps, reverse, 0, 0);
+      point_smooth<T>(p, l, sweeps, reverse, 0, 0, nullptr);
       continue;
     }
     const LineArgs<T> g = line_args<T>(p, L, ax);
@@ -703,7 +714,7 @@ void line_smooth(hf_plan* p, int l, int sweeps, int reverse) {
 
 template <class T>
 void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
-                 int from_zero) {
+                 int from_zero, double* mean_partials = nullptr) {
   Level& L = p->levels[l];
   if (L.line_mask != 0) {
     // x starts from an explicit zero on line levels; the retained defect is
@@ -715,15 +726,15 @@ void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     }
     return;
   }
-  point_smooth<T>(p, l, sweeps, reverse, retain, from_zero);
+  point_smooth<T>(p, l, sweeps, reverse, retain, from_zero, mean_partials);
 }
 
 template <class T>
 void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
-                  int from_zero) {
+                  int from_zero, double* mean_partials) {
   Level& L = p->levels[l];
   if (L.sweep) {
-    sweep_smooth<T>(p, l, sweeps, reverse, retain, from_zero);
+    sweep_smooth<T>(p, l, sweeps, reverse, retain, from_zero, mean_partials);
     return;
   }
   const Lv<T> v = view<T>(L);
@@ -764,13 +775,14 @@ void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
   }
 }
 
-void exec_grid_op(hf_plan* p, const CycleOp& op) {
+void exec_grid_op(hf_plan* p, const CycleOp& op,
+                  double* mean_partials = nullptr) {
   Level& L = p->levels[op.level];
   if (op.kind == T_SMOOTH) {
     dispatch(L.prec, [&](auto t) {
       using T = decltype(t);
       grid_smooth<T>(p, op.level, op.sweeps, op.reverse, op.retain,
-                     op.from_zero);
+                     op.from_zero, mean_partials);
     });
     return;
   }
@@ -812,10 +824,25 @@ void exec_grid_op(hf_plan* p, const CycleOp& op) {
   });
 }
 
-void run_cycle(hf_plan* p) {
-  for (const ExecItem& e : p->exec) {
+// Whether the cycle ends with a fused level-0 sweep that can sum the mean of
+// its output: the last op smooths level 0, its last sweep is not the
+// from-zero one, and the tile schedule runs it.
+bool mean_in_sweep(const hf_plan* p) {
+  if (p->exec.empty() || p->exec.back().tail) return false;
+  const CycleOp& op = p->exec.back().op;
+  const Level& L0 = p->levels[0];
+  if (op.kind != T_SMOOTH || op.level != 0 || !L0.sweep || op.sweeps < 1 ||
+      (op.sweeps == 1 && op.from_zero))
+    return false;
+  return !(L0.prec != HF_PRECISION_FP64 && sweep_columns());
+}
+
+// mean_partials: passed to the cycle's final op (see mean_in_sweep).
+void run_cycle(hf_plan* p, double* mean_partials = nullptr) {
+  for (size_t n = 0; n < p->exec.size(); ++n) {
+    const ExecItem& e = p->exec[n];
     if (!e.tail) {
-      exec_grid_op(p, e.op);
+      exec_grid_op(p, e.op, n + 1 == p->exec.size() ? mean_partials : nullptr);
       continue;
     }
     HF_PROF(p, P_TAIL);
@@ -854,14 +881,19 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
       if (tail0) project_apply<T>(p, 0, b);
     }
   });
-  run_cycle(p);
+  // the cycle's last fused sweep also sums the level-0 mean of its output
+  const bool sweep_mean = proj && mean_in_sweep(p);
+  run_cycle(p, sweep_mean ? p->partials : nullptr);
   const double* alpha = nullptr;
   const double* mean = nullptr;
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* x = static_cast<T*>(L0.x);
     if (proj) {
-      level_mean<T>(p, 0, x);
+      if (sweep_mean)
+        finalize(p, L0.sweep_blocks, 1, F_MEAN, S_MEAN0, L0.volume);
+      else
+        level_mean<T>(p, 0, x);
       mean = p->scalars + S_MEAN0;
     }
     if (!p->policy.unit_linear) {

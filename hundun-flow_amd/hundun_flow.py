@@ -32,7 +32,7 @@ PROFILE_CLASSES = ["rb_finest", "rb_coarse", "defect", "restrict", "prolong",
                    "tail", "operator", "update_xr", "true_residual",
                    "mg_entry_exit", "project", "update_p", "finalize",
                    "other", "sweep_finest", "sweep_coarse", "line",
-                   "rb_first"]
+                   "rb_first", "sweep_mean"]
 
 ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",
