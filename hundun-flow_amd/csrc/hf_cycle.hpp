@@ -495,12 +495,88 @@ __device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
   F.x[c] = fluid ? st<Tf>(xf + value) : st<Tf>(0.0);
 }
 
+// z-marching prolongation: a block owns one 2-D tile of the fine level and
+// t.per consecutive fine planes.  Each thread walks its fine column; the x
+// and y interpolation is set up once, and the four corners of a coarse
+// plane are loaded once and reused while consecutive fine planes need that
+// plane (fine planes 2K+1 and 2K+2 both interpolate between coarse planes K
+// and K+1).  Terms are added in the order of prolong_cell (the tail's
+// per-cell form).
 template <class Tc, class Tf>
-__global__ void k_prolong(Tiles t, Lv<Tc> Cl, Lv<Tf> F, int mask, int per) {
-  int i, j, k;
-  tile_xyz(t, xcd_block32(blockIdx.x, gridDim.x), i, j, k);
+__global__ void k_prolong_z(Tiles t, Lv<Tc> Cl, Lv<Tf> F, int mask, int per) {
+  using C = typename Compute<Tf>::type;
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
   if (i >= t.nx || j >= t.ny) return;
-  prolong_cell(Cl, F, mask, per, i, j, k, cell_index(t.nx, t.ny, i, j, k));
+  const int k0 = zb * t.per;
+  const int k1 = min(t.nz, k0 + t.per);
+  int xl, xr, xw, yl, yr, yw;
+  interp(i, mask & 1, Cl.nx, per & 1, &xl, &xr, &xw);
+  interp(j, (mask >> 1) & 1, Cl.ny, (per >> 1) & 1, &yl, &yr, &yw);
+  const int cx[2] = {wrapi(xl, Cl.nx), wrapi(xr, Cl.nx)};
+  const int cy[2] = {wrapi(yl, Cl.ny), wrapi(yr, Cl.ny)};
+  const int ny2 = yl == yr ? 1 : 2, nx2 = xl == xr ? 1 : 2;
+  const int cplane = Cl.nx * Cl.ny;
+  int off[4];  // in-plane offsets of the corners, index ay * 2 + ax
+#pragma unroll
+  for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax) off[ay * 2 + ax] = cy[ay] * Cl.nx + cx[ax];
+  // the two coarse planes held (index, corner values)
+  int held0 = -1, held1 = -1;
+  C h0[4] = {C(0), C(0), C(0), C(0)}, h1[4] = {C(0), C(0), C(0), C(0)};
+  int c = cell_index(t.nx, t.ny, i, j, k0);
+  const int plane = t.nx * t.ny;
+  for (int k = k0; k < k1; ++k, c += plane) {
+    int zl, zr, zw;
+    interp(k, (mask >> 2) & 1, Cl.nz, (per >> 2) & 1, &zl, &zr, &zw);
+    const int gz[2] = {wrapi(zl, Cl.nz), wrapi(zr, Cl.nz)};
+    const bool fluid = F.act[c] != 0;
+    const C xf = C(ld(F.x[c]));
+    C v[2][4];
+#pragma unroll
+    for (int az = 0; az < 2; ++az) {
+      const int g = gz[az];
+      if (g == held0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[az][e] = h0[e];
+      } else if (g == held1) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[az][e] = h1[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[az][e] = C(ld(Cl.x[g * cplane + off[e]]));
+      }
+    }
+    held0 = gz[0];
+    held1 = gz[1];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      h0[e] = v[0][e];
+      h1[e] = v[1][e];
+    }
+    const int nz2 = zl == zr ? 1 : 2;
+    // the reference order: az, ay, ax; value += ((wx*wy)*wz)*x_c
+    C value = C(0);
+#pragma unroll
+    for (int az = 0; az < 2; ++az) {
+      const C wz = az == 0 ? C(1) - C(zw) * C(0.25) : C(zw) * C(0.25);
+#pragma unroll
+      for (int ay = 0; ay < 2; ++ay) {
+        const C wy = ay == 0 ? C(1) - C(yw) * C(0.25) : C(yw) * C(0.25);
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax) {
+          const C wx = ax == 0 ? C(1) - C(xw) * C(0.25) : C(xw) * C(0.25);
+          if (az < nz2 && ay < ny2 && ax < nx2)
+            value += wx * wy * wz * v[az][ay * 2 + ax];
+        }
+      }
+    }
+    F.x[c] = fluid ? st<Tf>(xf + value) : st<Tf>(0.0);
+  }
 }
 
 // apply_impl exit (src/solver_mg.cpp:4954-4976, :5082-5118): the candidate's

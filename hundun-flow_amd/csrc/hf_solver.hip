@@ -818,8 +818,9 @@ void exec_grid_op(hf_plan* p, const CycleOp& op,
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       HF_PROF(p, P_PROLONG);
-      HF_LAUNCH((k_prolong<Tc, Tf>), L.tc.blocks, kBlock, 0, p->stream, L.tc,
-                view<Tc>(Cn), view<Tf>(L), L.coarsen_mask, p->periodic_mask);
+      HF_LAUNCH((k_prolong_z<Tc, Tf>), L.tz.blocks, kBlock, 0, p->stream,
+                L.tz, view<Tc>(Cn), view<Tf>(L), L.coarsen_mask,
+                p->periodic_mask);
     });
   });
 }
