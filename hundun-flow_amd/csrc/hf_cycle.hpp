@@ -951,10 +951,10 @@ namespace hf {
 // coefficients on plane k with one two-element load per array (whole cache
 // lines per instruction), updates the first-colour cell at once and keeps the
 // second-colour cell's coefficients in registers for step k + 1, where that
-// cell is updated.  Wave w owns tile rows w and w + 4; waves 0 and 1 also
-// own the ring rows -1 and 8 (first colour only) and wave 2 the ring cells
-// of columns -1 and 128 (one per tile row and plane, the column alternating
-// with the parity).  Each load is a raw buffer load: scalar resource, 32-bit
+// cell is updated.  Wave w of W owns tile rows w and w + W; waves 0 and 1 also
+// own the ring rows above and below the tile (first colour only) and wave 2
+// the ring cells left and right of it (one per tile row and plane, the side
+// alternating with the parity).  Each load is a raw buffer load: scalar resource, 32-bit
 // lane offset within the plane, scalar plane offset.  Rows hold an even
 // number of cells (the host plans this kernel for even nx only).
 // ---------------------------------------------------------------------------
@@ -1002,8 +1002,11 @@ struct SwCoef {
 // LDS positions of the x plane per thread.
 constexpr int kSwLoadSlots = (kSwPlane + kSwThreads - 1) / kSwThreads;
 
-// one pair per lane covers a tile row; four waves cover the 8 tile rows twice
-static_assert(kSwTX == 128 && kSwTY == 8 && kSwThreads == 256,
+// one pair per lane covers a tile row; each wave owns two tile rows, and the
+// ring column cells of the tile rows fit one wave
+constexpr int kSwWaves = kSwThreads / 64;
+static_assert(kSwTX == 128 && kSwThreads == 32 * kSwTY && kSwTY <= 64 &&
+                  kSwWaves >= 3,
               "fused sweep pair mapping");
 
 // Source index along one axis of a halo position g (-2 <= g <= n + 1), or -1
@@ -1187,7 +1190,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     xo[s] = ok ? static_cast<uint32_t>(sj * nx + si) * kS : 0u;
     xok |= ok ? (1u << s) : 0u;
   }
-  // pair rows of this wave: tile rows wave and wave + 4, and the ring row
+  // pair rows of this wave: tile rows wave and wave + kSwWaves, and the ring row
   // above the tile (wave 0) or below its last row inside the domain (wave 1).
   // Per row: tile row index r, unwrapped row gj (parity), whether the row
   // exists (tile rows: inside the domain; ring rows: inside, or wrapped
@@ -1200,7 +1203,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
 #pragma unroll
   for (int s = 0; s < kRows; ++s) {
     const int r = s == 0   ? wave
-                  : s == 1 ? wave + 4
+                  : s == 1 ? wave + kSwWaves
                            : (wave == 0 ? -1 : min(kSwTY, ny - y0));
     int gj = y0 + r;
     rr[s] = r;
@@ -1232,13 +1235,14 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     for (int s = 0; s < 2; ++s)
       if (rvalid[s]) mwy[s] = a.wy[rgj[s]];
   }
-  // ring columns (wave 2, lanes 0..7 = tile rows): byte offsets of the cell
+  // ring columns (wave 2, lane = tile row): byte offsets of the cell
   // left of the tile (sel 0) and right of its last column inside the domain
   // (sel 1, at an even x as nx is even); ring cells beyond a periodic face
   // are the wrapped cells
   uint32_t cob[2], cfb[2];
   uint32_t cok = 0;
-  const int cgj = y0 + (lane & 7);
+  const int crow = lane % kSwTY;
+  const int cgj = y0 + crow;
   const int cright = min(x0 + kSwTX, nx);
 #pragma unroll
   for (int sel = 0; sel < 2; ++sel) {
@@ -1422,7 +1426,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
       ring_ok = (cok >> sel) & 1u;
       ring_ob = sel ? cob[1] : cob[0];
       ring_fb = sel ? cfb[1] : cfb[0];
-      ring_q = (sel ? cright - x0 + 2 : 1) + kSwPX * ((lane & 7) + 2);
+      ring_q = (sel ? cright - x0 + 2 : 1) + kSwPX * (crow + 2);
     }
     ring_ok = ring_ok && first_plane;
     SwCoef<C> fr{};
@@ -1493,332 +1497,6 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   if (kMean) {
     msum = block_sum(msum, red);
     if (threadIdx.x == 0) a.partials[xcd_block32(blockIdx.x, gridDim.x)] = msum;
-  }
-}
-
-
-// ---------------------------------------------------------------------------
-// Column-owner fused sweep for 32- and 16-bit storage.  Same arithmetic and
-// order as k_rb_sweep, other schedule: each thread owns four columns of the
-// 128 x 8 tile, at x = lane, lane + 64 and y = wave, wave + 4 (one parity), so
-// every load instruction covers contiguous row segments and each cell's
-// coefficients are fetched once per sweep.  At step k a thread issues the
-// loads of x on plane k + 2 and of its columns' coefficients on plane k + 1;
-// they are used one step later (first colour) or two steps later (second
-// colour), so no update waits on a load issued in its own step.
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(256) k_rb_sweep_cols(SweepArgs<T> a) {
-  using C = typename Compute<T>::type;
-  __shared__ C buf[4][kSwPlane];
-  const Lv<T>& L = a.L;
-  int blk = xcd_block32(blockIdx.x, gridDim.x);
-  const int tx = blk % a.gx;
-  blk /= a.gx;
-  const int ty = blk % a.gy;
-  const int tz = blk / a.gy;
-  const int x0 = tx * kSwTX, y0 = ty * kSwTY;
-  const int z0 = tz * a.zc;
-  const int z1 = min(L.nz, z0 + a.zc);
-  const bool perx = a.bc.m[0] == 2, pery = a.bc.m[2] == 2,
-             perz = a.bc.m[4] == 2;
-  const int nx = L.nx, ny = L.ny, nz = L.nz;
-  const int plane = nx * ny;
-  const C omega = C(0.72);
-  const double mean = a.mean != nullptr ? *a.mean : 0.0;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int c0 = a.first;
-
-  auto plane_of = [&](int kk, int& kg) -> bool {
-    kg = kk;
-    if (kk >= 0 && kk < nz) return true;
-    if (!perz) return false;
-    kg = kk < 0 ? kk + nz : kk - nz;
-    return true;
-  };
-  // (gi, gj) -> the cell a position refers to: itself inside the domain, the
-  // wrapped cell across a periodic face, none across any other face
-  auto resolve = [&](int& gi, int& gj) -> bool {
-    bool ok = true;
-    if (gi < 0 || gi >= nx) {
-      if (perx) gi = wrapi(gi, nx); else ok = false;
-    }
-    if (gj < 0 || gj >= ny) {
-      if (pery) gj = wrapi(gj, ny); else ok = false;
-    }
-    return ok;
-  };
-
-  // ---- per-thread geometry, fixed for every plane of the block ----------
-  int xoff[kSwLoadSlots];  // x-plane slot: in-plane offset or -1
-#pragma unroll
-  for (int s = 0; s < kSwLoadSlots; ++s) {
-    const int p = threadIdx.x + s * kBlock;
-    const int iy = p / kSwPX;
-    const int ix = p - iy * kSwPX - 2;
-    int gi = x0 + ix, gj = y0 + iy - 2;
-    // only positions within two cells of the domain are ever read
-    bool ok = p < kSwPlane && gi >= -2 && gi <= nx + 1 && gj >= -2 &&
-              gj <= ny + 1;
-    ok = resolve(gi, gj) && ok;
-    xoff[s] = ok ? gi + nx * gj : -1;
-  }
-  // own columns q = (x half, y half).  kind 1: a cell of the domain; kind 2:
-  // the wrapped cell just past a periodic far face of a partial tile (first
-  // colour only, recomputed like a ring cell, never stored); kind 0: none.
-  int kind0, kind1, kind2, kind3, cgi0, cgi1, cgi2, cgi3, cgj0, cgj1, cgj2,
-      cgj3;
-  auto own = [&](int q, int& kd, int& gi_out, int& gj_out) {
-    const int ix = lane + kWave * (q & 1);
-    const int iy = wave + 4 * (q >> 1);
-    int gi = x0 + ix, gj = y0 + iy;
-    kd = 0;
-    if (gi < nx && gj < ny) {
-      kd = 1;
-    } else if (gi <= nx && gj <= ny && (gi < nx || perx) &&
-               (gj < ny || pery)) {
-      kd = 2;
-      if (gi == nx) gi = 0;
-      if (gj == ny) gj = 0;
-    }
-    gi_out = kd != 0 ? gi : 0;
-    gj_out = kd != 0 ? gj : 0;
-  };
-  own(0, kind0, cgi0, cgj0);
-  own(1, kind1, cgi1, cgj1);
-  own(2, kind2, cgi2, cgj2);
-  own(3, kind3, cgi3, cgj3);
-  // the thread's first-colour ring cell for even (e = 0) and odd (e = 1)
-  // planes: rows iy = -1 and 8 (65 cells each), columns ix = -1 and 128
-  // (4 cells each).  A ring position past the domain's far face is the
-  // wrapped cell when the axis is periodic; on a partial tile that face lies
-  // inside the tile and its wrapped cells are own columns of kind 2.
-  struct Ring {
-    int ix, iy, gi, gj;
-    bool ok;
-  };
-  auto ring_of = [&](int e) -> Ring {
-    int ix = 0, iy = 0;
-    const int s = threadIdx.x;
-    if (s < 2 * kSwRingRow) {
-      iy = s < kSwRingRow ? -1 : kSwTY;
-      const int m = s < kSwRingRow ? s : s - kSwRingRow;
-      const int par = (c0 - (x0 - 1 + y0 + iy + e)) & 1;
-      ix = -1 + par + 2 * m;
-    } else if (s < 2 * kSwRingRow + kSwTY) {
-      const int t = s - 2 * kSwRingRow;
-      ix = t < kSwTY / 2 ? -1 : kSwTX;
-      const int m = t < kSwTY / 2 ? t : t - kSwTY / 2;
-      const int par = (c0 - (x0 + ix + y0 + e)) & 1;
-      iy = par + 2 * m;
-    }
-    int gi = x0 + ix, gj = y0 + iy;
-    bool ok = s < 2 * kSwRingRow + kSwTY && gi >= -1 && gi <= nx &&
-              gj >= -1 && gj <= ny;
-    ok = resolve(gi, gj) && ok;
-    return Ring{ix, iy, ok ? gi : 0, ok ? gj : 0, ok};
-  };
-  // plain scalars: a select between array elements would keep the arrays
-  // in scratch memory
-  const Ring ring0 = ring_of(0), ring1 = ring_of(1);
-  const int rix0 = ring0.ix, riy0 = ring0.iy, rgi0 = ring0.gi,
-            rgj0 = ring0.gj, rix1 = ring1.ix, riy1 = ring1.iy,
-            rgi1 = ring1.gi, rgj1 = ring1.gj;
-  const bool rok0 = ring0.ok, rok1 = ring1.ok;
-  // the four own columns share the parity of (x0 + lane + y0 + wave): 64
-  // and 4 are even (parity of the unwrapped index equals the wrapped one:
-  // the fused route only runs with even periodic extents)
-  const int par = (x0 + lane + y0 + wave) & 1;
-
-  // ---- loads: raw stored values; masks, widening and the projection are
-  // applied where the values are used
-  struct RawCoef {
-    T d, xl, xh, yl, yh, zl, zh, b;
-    uint8_t act;
-  };
-  auto fetch_plane = [&](int kk, T (&v)[kSwLoadSlots]) {
-    int kg;
-    const bool valid = plane_of(kk, kg) && !a.from_zero;
-    const int base = valid ? plane * kg : 0;
-#pragma unroll
-    for (int s = 0; s < kSwLoadSlots; ++s) {
-      const bool ok = valid && xoff[s] >= 0;
-      v[s] = L.x[ok ? base + xoff[s] : 0];
-    }
-  };
-  auto store_plane = [&](int kk, const T (&v)[kSwLoadSlots]) {
-    int kg;
-    const bool valid = plane_of(kk, kg) && !a.from_zero;
-    C* P = buf[kk & 3];
-#pragma unroll
-    for (int s = 0; s < kSwLoadSlots; ++s) {
-      const int p = threadIdx.x + s * kBlock;
-      const bool ok = valid && xoff[s] >= 0;
-      if (p < kSwPlane) P[p] = ok ? C(ld(v[s])) : C(0);
-    }
-  };
-  // index 0 stands in for an inactive slot so every address is in bounds
-  auto fetch_coef = [&](bool ok, int gi, int gj, int kg) -> RawCoef {
-    const int c = ok ? gi + nx * gj + plane * kg : 0;
-    const int row = ok ? gj + ny * kg : 0;
-    const int fyo = ok ? c + nx * kg : 0;
-    RawCoef f;
-    f.d = L.D[c];
-    f.xl = L.fx[c + row];
-    f.xh = L.fx[c + row + 1];
-    f.yl = L.fy[fyo];
-    f.yh = L.fy[fyo + nx];
-    f.zl = L.fz[c];
-    f.zh = L.fz[c + plane];
-    f.b = L.b[c];
-    f.act = a.mean != nullptr ? L.act[c] : uint8_t(1);
-    return f;
-  };
-  auto widen = [&](const RawCoef& r) -> SwCoef<C> {
-    SwCoef<C> f;
-    f.d = C(ld(r.d));
-    f.xl = C(ld(r.xl));
-    f.xh = C(ld(r.xh));
-    f.yl = C(ld(r.yl));
-    f.yh = C(ld(r.yh));
-    f.zl = C(ld(r.zl));
-    f.zh = C(ld(r.zh));
-    if (a.mean == nullptr) {
-      f.b = C(ld(r.b));
-    } else {
-      const T projected = st<T>(static_cast<double>(ld(r.b)) - mean);
-      f.b = r.act ? C(ld(projected)) : C(0);
-    }
-    return f;
-  };
-  auto mode = [](int m, C self) -> C { return m == 0 ? C(0) : self; };
-  // x <- x + omega (b - A x) / D at one cell held in LDS, in the reference
-  // evaluation order; returns the new value
-  auto update = [&](const SwCoef<C>& f, const C* Pm, C* P0, const C* Pp,
-                    int ix, int iy, int gi, int gj, int kg) -> C {
-    const int q = (ix + 2) + kSwPX * (iy + 2);
-    const C xc = P0[q];
-    const C xl = (gi > 0 || perx) ? P0[q - 1] : mode(a.bc.m[0], xc);
-    const C xh = (gi + 1 < nx || perx) ? P0[q + 1] : mode(a.bc.m[1], xc);
-    const C yl = (gj > 0 || pery) ? P0[q - kSwPX] : mode(a.bc.m[2], xc);
-    const C yh = (gj + 1 < ny || pery) ? P0[q + kSwPX] : mode(a.bc.m[3], xc);
-    const C zl = (kg > 0 || perz) ? Pm[q] : mode(a.bc.m[4], xc);
-    const C zh = (kg + 1 < nz || perz) ? Pp[q] : mode(a.bc.m[5], xc);
-    C v = f.d * xc;
-    v -= f.xl * xl;
-    v -= f.xh * xh;
-    v -= f.yl * yl;
-    v -= f.yh * yh;
-    v -= f.zl * zl;
-    v -= f.zh * zh;
-    const C xn = sw_round<T>(xc + omega * (f.b - v) / f.d);
-    P0[q] = xn;
-    return xn;
-  };
-
-  // One plane's worth of raw loads: the four columns' coefficients and the
-  // thread's first-colour ring cell on plane k, and x of plane k + 1.
-  struct PlaneLoads {
-    T vx[kSwLoadSlots];
-    RawCoef c0, c1, c2, c3, rc;
-  };
-  auto issue = [&](int k, PlaneLoads& n) {
-    fetch_plane(k + 1, n.vx);
-    int kg;
-    const bool pk = plane_of(k, kg);
-    n.c0 = fetch_coef(pk && kind0 != 0, cgi0, cgj0, kg);
-    n.c1 = fetch_coef(pk && kind1 != 0, cgi1, cgj1, kg);
-    n.c2 = fetch_coef(pk && kind2 != 0, cgi2, cgj2, kg);
-    n.c3 = fetch_coef(pk && kind3 != 0, cgi3, cgj3, kg);
-    const bool odd = (k & 1) != 0;
-    n.rc = fetch_coef(pk && (odd ? rok1 : rok0), odd ? rgi1 : rgi0,
-                      odd ? rgj1 : rgj0, kg);
-  };
-  // plane k: x of plane k + 1 into LDS, the first colour on plane k (own
-  // columns when first colour there, and the ring), then the second colour
-  // on plane k - 1 (own columns when second colour there, with the
-  // coefficients of plane k - 1 in `prev`)
-  auto compute = [&](int k, const PlaneLoads& now, const PlaneLoads& prev) {
-    int kg;
-    const bool plane_ok = plane_of(k, kg);
-    const bool first = ((par + k) & 1) == c0;
-    __syncthreads();  // plane k-3's buffer is free (read by the last step)
-    store_plane(k + 1, now.vx);
-    __syncthreads();
-    if (plane_ok) {
-      const C* Pm = buf[(k - 1) & 3];
-      C* P0 = buf[k & 3];
-      const C* Pp = buf[(k + 1) & 3];
-      const bool store = k >= z0 && k < z1;
-      auto col = [&](const RawCoef& r, int kd, int q, int gi, int gj) {
-        if (kd == 0) return;
-        const SwCoef<C> f = widen(r);
-        const C xn = update(f, Pm, P0, Pp, lane + kWave * (q & 1),
-                            wave + 4 * (q >> 1), gi, gj, kg);
-        if (store && kd == 1) {
-          const int c = cell_index(nx, ny, gi, gj, kg);
-          a.xout[c] = st<T>(xn);
-          if (a.bout != nullptr) a.bout[c] = st<T>(f.b);
-        }
-      };
-      if (first) {
-        col(now.c0, kind0, 0, cgi0, cgj0);
-        col(now.c1, kind1, 1, cgi1, cgj1);
-        col(now.c2, kind2, 2, cgi2, cgj2);
-        col(now.c3, kind3, 3, cgi3, cgj3);
-      }
-      const bool odd = (k & 1) != 0;
-      if (odd ? rok1 : rok0)
-        update(widen(now.rc), Pm, P0, Pp, odd ? rix1 : rix0,
-               odd ? riy1 : riy0, odd ? rgi1 : rgi0, odd ? rgj1 : rgj0, kg);
-    }
-    __syncthreads();
-    const int kb = k - 1;
-    if (first && kb >= z0 && kb < z1) {
-      const C* Pm = buf[(kb - 1) & 3];
-      C* P0 = buf[kb & 3];
-      const C* Pp = buf[(kb + 1) & 3];
-      auto col = [&](const RawCoef& r, int kd, int q, int gi, int gj) {
-        if (kd != 1) return;
-        const SwCoef<C> f = widen(r);
-        const C xn = update(f, Pm, P0, Pp, lane + kWave * (q & 1),
-                            wave + 4 * (q >> 1), gi, gj, kb);
-        const int c = cell_index(nx, ny, gi, gj, kb);
-        a.xout[c] = st<T>(xn);
-        if (a.bout != nullptr) a.bout[c] = st<T>(f.b);
-      };
-      col(prev.c0, kind0, 0, cgi0, cgj0);
-      col(prev.c1, kind1, 1, cgi1, cgj1);
-      col(prev.c2, kind2, 2, cgi2, cgj2);
-      col(prev.c3, kind3, 3, cgi3, cgj3);
-    }
-  };
-
-  const int kstart = z0 - 1;
-  {
-    T v[kSwLoadSlots];
-    fetch_plane(kstart - 1, v);
-    store_plane(kstart - 1, v);
-  }
-  // three plane buffers in rotation: step k computes from plane k's and
-  // plane k - 1's loads while plane k + 1's land
-  PlaneLoads b0, b1, b2;
-  issue(kstart - 1, b2);  // plane kstart - 1 (only its x of plane kstart)
-  issue(kstart, b0);
-  {
-    // x of plane kstart into LDS (carried by plane kstart - 1's loads)
-    store_plane(kstart, b2.vx);
-  }
-  for (int k = kstart; k <= z1; k += 3) {
-    if (k + 1 <= z1) issue(k + 1, b1);
-    compute(k, b0, b2);
-    if (k + 1 > z1) break;
-    if (k + 2 <= z1) issue(k + 2, b2);
-    compute(k + 1, b1, b0);
-    if (k + 2 > z1) break;
-    if (k + 3 <= z1) issue(k + 3, b0);
-    compute(k + 2, b2, b1);
   }
 }
 

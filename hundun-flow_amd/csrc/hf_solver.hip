@@ -435,15 +435,6 @@ bool line_relaxation_enabled() {
   return env != nullptr && std::atoi(env) != 0;
 }
 
-// HF_SWEEP_COLUMNS=1 selects the column-owner schedule of the fused sweep
-// for 32- and 16-bit storage.  It fetches every coefficient once per sweep
-// (half the tile schedule's HBM traffic) but, at two waves per SIMD, waits
-// on its plane steps longer than the tile schedule saves: 0.42 against
-// 0.33 ms per 512x256x256 FP32 sweep.
-bool sweep_columns() {
-  const char* env = std::getenv("HF_SWEEP_COLUMNS");
-  return env != nullptr && std::atoi(env) != 0;
-}
 
 int64_t tail_threshold() {
   const char* env = std::getenv("HF_TAIL_CELLS");
@@ -625,12 +616,7 @@ void sweep_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
     a.wz = p->d_widths[2];
     {
       HF_PROF(p, zero ? P_RB_FIRST : with_mean ? P_SWEEP_MEAN : rb_class);
-      // the column-owner schedule exists for 32- and 16-bit storage only
-      // (its FP64 register footprint would leave one wave per SIMD)
-      if (sizeof(T) < sizeof(double) && sweep_columns())
-        HF_LAUNCH(k_rb_sweep_cols<T>, L.sweep_blocks, kSwThreads, 0,
-                  p->stream, a);
-      else if (zero)
+      if (zero)
         HF_LAUNCH((k_rb_sweep<T, true>), L.sweep_blocks, kSwThreads, 0,
                   p->stream, a);
       else if (with_mean)
@@ -826,16 +812,14 @@ void exec_grid_op(hf_plan* p, const CycleOp& op,
 }
 
 // Whether the cycle ends with a fused level-0 sweep that can sum the mean of
-// its output: the last op smooths level 0, its last sweep is not the
-// from-zero one, and the tile schedule runs it.
+// its output: the last op smooths level 0 and its last sweep is not the
+// from-zero one.
 bool mean_in_sweep(const hf_plan* p) {
   if (p->exec.empty() || p->exec.back().tail) return false;
   const CycleOp& op = p->exec.back().op;
   const Level& L0 = p->levels[0];
-  if (op.kind != T_SMOOTH || op.level != 0 || !L0.sweep || op.sweeps < 1 ||
-      (op.sweeps == 1 && op.from_zero))
-    return false;
-  return !(L0.prec != HF_PRECISION_FP64 && sweep_columns());
+  return op.kind == T_SMOOTH && op.level == 0 && L0.sweep && op.sweeps >= 1 &&
+         !(op.sweeps == 1 && op.from_zero);
 }
 
 // mean_partials: passed to the cycle's final op (see mean_in_sweep).

@@ -273,21 +273,24 @@ def test_mixed_precision_reaches_tolerance(torch_cuda, monkeypatch, precision,
 
 @pytest.mark.parametrize("name", ["mixed_wide", "periodic_xy", "sphere32"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_fused_sweep_schedules_agree(torch_cuda, monkeypatch, name, precision):
-    """The column-owner and the tile schedule of the fused finest-level
-    sweep perform the same updates in the same arithmetic order: one cycle
-    under each gives the same correction bit for bit."""
+def test_fused_sweep_matches_two_pass(torch_cuda, monkeypatch, name,
+                                      precision):
+    """The fused finest-level sweep and the two colour passes perform the
+    same updates in the same arithmetic order: one cycle on each route gives
+    the same correction bit for bit."""
     torch = torch_cuda
-    _say(f"[sweep schedules] case={name} precision={precision}")
+    _say(f"[sweep routes] case={name} precision={precision}")
     case = CASES[name]()
-    plan = _gpu_plan(torch, case, **_policy(name, precision=precision))
-    r = _t(torch, np.random.default_rng(8).standard_normal(plan.cells))
-    monkeypatch.setenv("HF_SWEEP_COLUMNS", "1")
-    z_cols = plan.mg_apply(r).cpu().numpy()
-    monkeypatch.setenv("HF_SWEEP_COLUMNS", "0")
-    z_tile = plan.mg_apply(r).cpu().numpy()
-    assert np.array_equal(z_cols, z_tile), np.max(np.abs(z_cols - z_tile))
-    plan.close()
+    r = _t(torch, np.random.default_rng(8).standard_normal(
+        int(np.prod(case["n"]))))
+    out = {}
+    for sweep in ("1", "0"):
+        monkeypatch.setenv("HF_FUSED_SWEEP", sweep)
+        plan = _gpu_plan(torch, case, **_policy(name, precision=precision))
+        out[sweep] = plan.mg_apply(r).cpu().numpy()
+        plan.close()
+    assert np.array_equal(out["1"], out["0"]), \
+        np.max(np.abs(out["1"] - out["0"]))
 
 
 def _walls(n, bc, seed):
