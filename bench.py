@@ -58,8 +58,9 @@ def class_bytes_per_cell(cls: str, s: int):
         "update_xr": 6 * 8,
         # r = b - A x: x, b, D, 3 faces, r
         "true_residual": 7 * 8,
-        # p = z + beta p: read z p, write p
-        "update_p": 3 * 8,
+        # p = z + beta p, z formed from the level-0 candidate: read x (s),
+        # the fluid flags and p, write p
+        "update_p": s + 1 + 2 * 8,
     }.get(cls)
 
 

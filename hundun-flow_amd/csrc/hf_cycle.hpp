@@ -110,15 +110,6 @@ __global__ void k_update_xr(Chunk ch, double* x, const double* p, double* r,
   if (threadIdx.x == 0) partials[lb] = d;
 }
 
-// p = 1.0*z + beta*p (axpby, src/solver_krylov.cpp:321-339)
-__global__ void k_update_p(Chunk ch, double* p, const double* z,
-                           const double* beta) {
-  int lb, lo, hi;
-  chunk_range(ch, lb, lo, hi);
-  const double be = *beta;
-  for (int c = lo + threadIdx.x; c < hi; c += kBlock)
-    p[c] = 1.0 * z[c] + be * p[c];
-}
 
 __global__ void k_copy(Chunk ch, double* dst, const double* src) {
   int lb, lo, hi;
@@ -615,7 +606,7 @@ __global__ void k_mg_exit(Tiles t, const T* x, const uint8_t* act,
       if (mean != nullptr) xv = static_cast<double>(ld(st<T>(xv - m)));
       v = xv * a;
     }
-    z[cc[u]] = v;
+    if (z != nullptr) z[cc[u]] = v;
     if (r != nullptr) rz += rv[u] * v;
     if (q != nullptr) zq += v * qv[u];
   }
@@ -626,6 +617,43 @@ __global__ void k_mg_exit(Tiles t, const T* x, const uint8_t* act,
       zq = block_sum(zq, scratch);
       if (threadIdx.x == 0) partials[stride + lb] = zq;
     }
+  }
+}
+
+// p = 1.0*z + beta*p (axpby, src/solver_krylov.cpp:321-339) with z formed
+// from the cycle's level-0 candidate as
+// k_mg_exit forms it (fluid ? (x - mean, stored and reloaded) * alpha : 0),
+// so the exit need not store z.  Tiles of the reduction layout.
+template <class T>
+__global__ void k_update_p_exit(Tiles t, const T* x, const uint8_t* act,
+                                const double* mean, const double* alpha,
+                                double* p, const double* beta) {
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const double a = alpha == nullptr ? 1.0 : *alpha;
+  const double m = mean == nullptr ? 0.0 : *mean;
+  const double be = *beta;
+  T xs[kRedPer];
+  double pv[kRedPer];
+  bool ok[kRedPer], fluid[kRedPer];
+  int cc[kRedPer];
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
+    int i, j, k;
+    cc[u] = red_cell(t, lb, u, i, j, k, ok[u]);
+    fluid[u] = act[cc[u]] != 0;
+    xs[u] = x[cc[u]];
+    pv[u] = p[cc[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < kRedPer; ++u) {
+    if (!ok[u]) continue;
+    double v = 0.0;
+    if (fluid[u]) {
+      double xv = static_cast<double>(ld(xs[u]));
+      if (mean != nullptr) xv = static_cast<double>(ld(st<T>(xv - m)));
+      v = xv * a;
+    }
+    p[cc[u]] = 1.0 * v + be * pv[u];
   }
 }
 

@@ -842,8 +842,19 @@ void run_cycle(hf_plan* p, double* mean_partials = nullptr) {
 // entered: b of level 0 (and, with the null space, the partial sums of its
 // mean in accumulator 1) were stored by the PCG update (k_update_xr_entry);
 // only when level 0 is a grid level (not the tail, not line relaxed).
+// The operands from which the exit forms z (level-0 candidate, projection
+// mean, scale), for a consumer that forms z itself (k_update_p_exit).
+struct ExitOperands {
+  const void* x = nullptr;
+  const double* mean = nullptr;
+  const double* alpha = nullptr;
+};
+
+// z == nullptr: the exit stores no z (it still forms the dots), and `exit`
+// receives the operands z is formed from.
 hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
-                   bool dots, bool entered = false) {
+                   bool dots, bool entered = false,
+                   ExitOperands* exit = nullptr) {
   Level& L0 = p->levels[0];
   const bool proj = p->policy.null_space_constant != 0;
   // whole cycle inside the tail kernel, or a line level 0: x = 0 and the
@@ -907,6 +918,11 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
               mean, alpha, z, dots ? r : static_cast<const double*>(nullptr),
               q, dots ? p->partials : static_cast<double*>(nullptr),
               static_cast<int>(p->partial_stride));
+    if (exit != nullptr) {
+      exit->x = x;
+      exit->mean = mean;
+      exit->alpha = alpha;
+    }
   });
   return ok();
 }
@@ -1499,13 +1515,24 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       HF_TRY(precondition_restart());
       continue;
     }
-    HF_TRY(mg_apply(p, r, z, ctl->flexible ? q : nullptr, true, fuse_entry));
+    // z is formed again by the direction update instead of being stored
+    ExitOperands ex;
+    HF_TRY(mg_apply(p, r, nullptr, ctl->flexible ? q : nullptr, true,
+                    fuse_entry, &ex));
     res->preconditioner_applies++;
     finalize(p, p->levels[0].tred.blocks, ctl->flexible ? 2 : 1,
              ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
-    HF_PROF(p, P_UPDATE_P);
-    HF_LAUNCH(k_update_p, nbv, kBlock, 0, p->stream, ch, pp, z,
-              p->scalars + S_BETA);
+    {
+      HF_PROF(p, P_UPDATE_P);
+      Level& L0 = p->levels[0];
+      dispatch(L0.prec, [&](auto t) {
+        using T = decltype(t);
+        HF_LAUNCH(k_update_p_exit<T>, L0.tred.blocks, kBlock, 0, p->stream,
+                  L0.tred, static_cast<const T*>(ex.x),
+                  static_cast<const uint8_t*>(L0.act), ex.mean, ex.alpha, pp,
+                  static_cast<const double*>(p->scalars + S_BETA));
+      });
+    }
   }
   return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
 }
