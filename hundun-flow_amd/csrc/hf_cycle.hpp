@@ -486,6 +486,97 @@ __device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
   F.x[c] = fluid ? st<Tf>(xf + value) : st<Tf>(0.0);
 }
 
+// Retained defect fused with the restriction that consumes it
+// (src/solver_mg.cpp:3633-3720).  The z-marching layout of k_stencil_z with
+// full tiles of even width and height and even z-chunks, on a level with
+// even extents coarsened along every axis: each step's defects, stored and
+// reloaded at the fine storage type, go through LDS to the thread owning the
+// coarse cell (even i, even j), which adds its eight children in the order
+// of k_restrict (dz, dy, dx) and stores the coarse b at the end of the odd
+// plane.  The fine r is not stored.  zero_x and partials as in k_restrict.
+template <class Tf, class Tc>
+__global__ void k_defect_restrict(Tiles t, Lv<Tf> L, Bc bc, Lv<Tc> Cl,
+                                  int zero_x, double* partials) {
+  using C = typename Compute<Tf>::type;
+  using Cc = typename Compute<Tc>::type;
+  __shared__ double scratch[16];
+  __shared__ Cc defect[kBlock];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  const int plane = nx * ny;
+  const int tw = 1 << t.txl;
+  const int tid = threadIdx.x;
+  const bool owner = ((i | j) & 1) == 0;
+  const Tf* x = L.x;
+  int c = cell_index(nx, ny, i, j, k0);
+  C xc = C(ld(x[c]));
+  C zl = k0 > 0 ? C(ld(x[c - plane]))
+                : neighbour_or_boundary<C>(bc.m[4], xc, x, c + (nz - 1) * plane);
+  C fzl = C(ld(L.fz[c]));
+  Cc sum = Cc(0);
+  double pv = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const C zh = k + 1 < nz ? C(ld(x[c + plane]))
+                            : neighbour_or_boundary<C>(bc.m[5], xc, x,
+                                                       c - (nz - 1) * plane);
+    const C fzh = C(ld(L.fz[c + plane]));
+    const int row = j + ny * k;
+    const C xl = i > 0 ? C(ld(x[c - 1]))
+                       : neighbour_or_boundary<C>(bc.m[0], xc, x, c + (nx - 1));
+    const C xh = i + 1 < nx ? C(ld(x[c + 1]))
+                            : neighbour_or_boundary<C>(bc.m[1], xc, x,
+                                                       c - (nx - 1));
+    const C yl = j > 0 ? C(ld(x[c - nx]))
+                       : neighbour_or_boundary<C>(bc.m[2], xc, x,
+                                                  c + (ny - 1) * nx);
+    const C yh = j + 1 < ny ? C(ld(x[c + nx]))
+                            : neighbour_or_boundary<C>(bc.m[3], xc, x,
+                                                       c - (ny - 1) * nx);
+    const int fyo = c + nx * k;
+    C v = C(ld(L.D[c])) * xc;
+    v -= C(ld(L.fx[c + row])) * xl;
+    v -= C(ld(L.fx[c + row + 1])) * xh;
+    v -= C(ld(L.fy[fyo])) * yl;
+    v -= C(ld(L.fy[fyo + nx])) * yh;
+    v -= fzl * zl;
+    v -= fzh * zh;
+    const Tf r = st<Tf>(C(ld(L.b[c])) - v);
+    __syncthreads();  // the previous step's children have been read
+    defect[tid] = Cc(ld(r));
+    __syncthreads();
+    if (owner) {
+      // children (dx, dy) of this plane: dx fastest, as k_restrict adds them
+      sum += defect[tid];
+      sum += defect[tid + 1];
+      sum += defect[tid + tw];
+      sum += defect[tid + tw + 1];
+      if (k & 1) {
+        const int cc = cell_index(Cl.nx, Cl.ny, i >> 1, j >> 1, k >> 1);
+        const Tc stored = st<Tc>(sum);
+        Cl.b[cc] = stored;
+        if (zero_x) Cl.x[cc] = st<Tc>(0.0);
+        if (partials != nullptr)
+          pv += Cl.vol[cc] * static_cast<double>(ld(stored));
+        sum = Cc(0);
+      }
+    }
+    zl = xc;
+    xc = zh;
+    fzl = fzh;
+    c += plane;
+  }
+  if (partials != nullptr) {
+    pv = block_sum(pv, scratch);
+    if (threadIdx.x == 0) partials[lb] = pv;
+  }
+}
+
 // z-marching prolongation: a block owns one 2-D tile of the fine level and
 // t.per consecutive fine planes.  Each thread walks its fine column; the x
 // and y interpolation is set up once, and the four corners of a coarse

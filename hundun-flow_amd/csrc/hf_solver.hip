@@ -162,6 +162,10 @@ struct ExecItem {
   CycleOp op{};
   int tail_begin = 0;  // index into tail_ops
   int tail_count = 0;
+  // a retaining smoothing whose defect and the following restriction run
+  // as one kernel (k_defect_restrict); that restriction item is skipped
+  bool defect_restrict = false;
+  bool skip = false;
 };
 
 size_t elem_size(int prec) {
@@ -427,6 +431,14 @@ bool sweep_enabled() {
   return env == nullptr || std::atoi(env) != 0;
 }
 
+// The retained defect of a fused-sweep level and its restriction run as one
+// kernel where the layout allows (defect_restrict_ok); HF_DEFECT_RESTRICT=0
+// keeps the two launches.
+bool defect_restrict_enabled() {
+  const char* env = std::getenv("HF_DEFECT_RESTRICT");
+  return env == nullptr || std::atoi(env) != 0;
+}
+
 // Line-relaxation plans are created only with HF_LINE_RELAXATION=1: their
 // device path matches the oracle in a CPU build of these kernels but has not
 // yet run cleanly on a device, so it stays off by default.
@@ -469,6 +481,23 @@ bool op_on_tail(const hf_plan* p, const CycleOp& op) {
   return lo >= p->tail_first;
 }
 
+// k_defect_restrict applies on level l: a fused-sweep level (its retained
+// defect comes from one stencil pass), coarsened along every axis with even
+// extents, full z-marching tiles of even width and height, even z-chunks.
+bool defect_restrict_ok(const hf_plan* p, int l) {
+  if (!defect_restrict_enabled() || l + 1 >= static_cast<int>(p->levels.size()))
+    return false;
+  const Level& L = p->levels[l];
+  const Level& Cn = p->levels[l + 1];
+  const Tiles& t = L.tz;
+  const int tw = 1 << t.txl, th = kBlock >> t.txl;
+  return L.sweep && L.coarsen_mask == 7 && L.n[0] % 2 == 0 &&
+         L.n[1] % 2 == 0 && L.n[2] % 2 == 0 && Cn.n[0] * 2 == L.n[0] &&
+         Cn.n[1] * 2 == L.n[1] && Cn.n[2] * 2 == L.n[2] && tw % 2 == 0 &&
+         th % 2 == 0 && L.n[0] % tw == 0 && L.n[1] % th == 0 &&
+         t.per % 2 == 0;
+}
+
 void build_exec(hf_plan* p) {
   p->program.clear();
   p->exec.clear();
@@ -499,6 +528,16 @@ void build_exec(hf_plan* p) {
     }
     e.tail_count = static_cast<int>(p->tail_ops_host.size()) - e.tail_begin;
     p->exec.push_back(e);
+  }
+  for (size_t n = 0; n + 1 < p->exec.size(); ++n) {
+    ExecItem& a = p->exec[n];
+    ExecItem& b = p->exec[n + 1];
+    if (a.tail || b.tail || a.op.kind != T_SMOOTH || !a.op.retain ||
+        b.op.kind != T_RESTRICT || b.op.level != a.op.level ||
+        !defect_restrict_ok(p, a.op.level))
+      continue;
+    a.defect_restrict = true;
+    b.skip = true;
   }
 }
 
@@ -762,13 +801,39 @@ void point_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
 }
 
 void exec_grid_op(hf_plan* p, const CycleOp& op,
-                  double* mean_partials = nullptr) {
+                  double* mean_partials = nullptr,
+                  bool defect_restrict = false) {
   Level& L = p->levels[op.level];
   if (op.kind == T_SMOOTH) {
     dispatch(L.prec, [&](auto t) {
       using T = decltype(t);
-      grid_smooth<T>(p, op.level, op.sweeps, op.reverse, op.retain,
-                     op.from_zero, mean_partials);
+      grid_smooth<T>(p, op.level, op.sweeps, op.reverse,
+                     defect_restrict ? 0 : op.retain, op.from_zero,
+                     mean_partials);
+    });
+    if (!defect_restrict) return;
+    // the retained defect and the restriction of the next op, in one pass
+    Level& Cd = p->levels[op.level + 1];
+    const bool proj = p->policy.null_space_constant != 0;
+    const bool into_tail = Cd.tail || Cd.line_mask != 0;
+    dispatch(L.prec, [&](auto tf) {
+      using Tf = decltype(tf);
+      dispatch(Cd.prec, [&](auto tc) {
+        using Tc = decltype(tc);
+        {
+          HF_PROF(p, P_DEFECT);
+          HF_LAUNCH((k_defect_restrict<Tf, Tc>), L.tz.blocks, kBlock, 0,
+                    p->stream, L.tz, view<Tf>(L), p->bc, view<Tc>(Cd),
+                    into_tail ? 1 : 0,
+                    proj ? p->partials : static_cast<double*>(nullptr));
+        }
+        if (proj) {
+          finalize(p, L.tz.blocks, 1, F_MEAN, S_MEAN0 + op.level + 1,
+                   Cd.volume);
+          if (into_tail)
+            project_apply<Tc>(p, op.level + 1, static_cast<Tc*>(Cd.b));
+        }
+      });
     });
     return;
   }
@@ -826,8 +891,10 @@ bool mean_in_sweep(const hf_plan* p) {
 void run_cycle(hf_plan* p, double* mean_partials = nullptr) {
   for (size_t n = 0; n < p->exec.size(); ++n) {
     const ExecItem& e = p->exec[n];
+    if (e.skip) continue;
     if (!e.tail) {
-      exec_grid_op(p, e.op, n + 1 == p->exec.size() ? mean_partials : nullptr);
+      exec_grid_op(p, e.op, n + 1 == p->exec.size() ? mean_partials : nullptr,
+                   e.defect_restrict);
       continue;
     }
     HF_PROF(p, P_TAIL);
@@ -1090,7 +1157,9 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       L.tz = L.tr;
       const int gxy = L.tz.gx * L.tz.gy;
       const int64_t want = static_cast<int64_t>(L.n[2]) * gxy / 32768;
-      L.tz.per = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(64, want)));
+      // (an even number of planes, so defect and restriction can fuse)
+      int per = static_cast<int>(std::max<int64_t>(2, std::min<int64_t>(64, want)));
+      L.tz.per = per + (per & 1);
       L.tz.blocks = gxy * ((L.n[2] + L.tz.per - 1) / L.tz.per);
     }
     // fused sweeps on grid levels with the fused colour route and rows long

@@ -358,3 +358,43 @@ def test_fused_sweep_edges_match_two_pass(torch_cuda, monkeypatch, name,
     # the projection means come from the same reductions on either route
     assert np.array_equal(out["1"], out["0"]), \
         np.max(np.abs(out["1"] - out["0"]))
+
+
+# Levels where the retained defect and its restriction run as one kernel:
+# fused-sweep levels with even extents and full z-marching tiles.
+DEFECT_RESTRICT = {
+    "periodic_xy": CASES["periodic_xy"],
+    "sphere32": CASES["sphere32"],
+    "perz_neumann_xy": SWEEP_EDGES["perz_neumann_xy"],
+    "two_tiles_periodic": lambda: _walls((256, 24, 8), [2, 2, 2, 2, 0, 0], 72),
+}
+
+
+@pytest.mark.parametrize("name", list(DEFECT_RESTRICT))
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("null", [0, 1])
+def test_defect_restrict_fused_matches_two_kernels(torch_cuda, monkeypatch,
+                                                    name, precision, null):
+    """The fused defect + restriction stores the coarse b of the two
+    kernels bit for bit, so the cycle is unchanged without the null space;
+    with it, the coarse mean's partial sums come in another block order."""
+    torch = torch_cuda
+    _say(f"[defect restrict] case={name} precision={precision} null={null}")
+    case = DEFECT_RESTRICT[name]()
+    pol = dict(null_space_constant=null, anisotropy_threshold=1e9,
+               precision=precision)
+    r = _t(torch, np.random.default_rng(6).standard_normal(
+        int(np.prod(case["n"]))))
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("HF_DEFECT_RESTRICT", fused)
+        plan = _gpu_plan(torch, case, **pol)
+        out[fused] = plan.mg_apply(r).cpu().numpy()
+        plan.close()
+    if null == 0:
+        assert np.array_equal(out["1"], out["0"]), \
+            np.max(np.abs(out["1"] - out["0"]))
+    else:
+        rel = np.linalg.norm(out["1"] - out["0"]) / np.linalg.norm(out["0"])
+        bound = 1e-12 if precision == "fp64" else 1e-4
+        assert rel <= bound, rel
