@@ -424,11 +424,23 @@ void plan_levels(hf_plan* p) {
 
 // The fused two-colour sweep (one pass over the level instead of two) runs on
 // the finest level; HF_FUSED_SWEEP=0 selects two colour passes there too.
-// Coarser levels have too few tiles per plane to hide its plane-step
-// barriers and keep the two-pass route.
+// Coarser levels of fewer than sweep_min_cells() cells have too few tiles per
+// plane to hide its plane-step barriers and keep the two-pass route.
 bool sweep_enabled() {
   const char* env = std::getenv("HF_FUSED_SWEEP");
   return env == nullptr || std::atoi(env) != 0;
+}
+
+// Coarse levels run the fused sweep from this many cells (measured: at
+// 512x256x256 the 256x128x128 level gains, the 128x64x64 one does not);
+// HF_SWEEP_MIN_CELLS overrides it.
+int64_t sweep_min_cells() {
+  const char* env = std::getenv("HF_SWEEP_MIN_CELLS");
+  if (env != nullptr) {
+    const long long v = std::atoll(env);
+    if (v >= 0) return v;
+  }
+  return int64_t{1} << 21;
 }
 
 // The retained defect of a fused-sweep level and its restriction run as one
@@ -1170,7 +1182,8 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
                   static_cast<int64_t>(L.n[0]) * (L.n[1] + 1) * L.n[2],
                   static_cast<int64_t>(L.n[0]) * L.n[1] * (L.n[2] + 1)});
     // (each lane owns a pair of cells of a row: nx even)
-    L.sweep = l == 0 && !L.tail && L.line_mask == 0 && L.fused &&
+    L.sweep = (l == 0 || L.cells >= sweep_min_cells()) && !L.tail &&
+              L.line_mask == 0 && L.fused &&
               L.n[0] >= 32 && L.n[0] % 2 == 0 && L.n[1] >= 8 &&
               sweep_enabled() &&
               widest * static_cast<int64_t>(elem_size(L.prec)) <

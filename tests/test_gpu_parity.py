@@ -316,12 +316,18 @@ SWEEP_EDGES = {
 }
 
 
+# "all": every grid level large enough runs the fused sweep (coarse levels
+# too, as the large grids do), not only the finest
+@pytest.mark.parametrize("levels", ["finest", "all"])
 @pytest.mark.parametrize("name", list(SWEEP_EDGES))
 @pytest.mark.parametrize("f_cycle", [0, 1])
-def test_fused_sweep_edges_fp64_bitwise(torch_cuda, name, f_cycle):
+def test_fused_sweep_edges_fp64_bitwise(torch_cuda, monkeypatch, name,
+                                        f_cycle, levels):
     """FP64 cycle through the fused sweep equals the oracle bit for bit."""
     torch = torch_cuda
-    _say(f"[sweep edges fp64] case={name} f_cycle={f_cycle}")
+    _say(f"[sweep edges fp64] case={name} f_cycle={f_cycle} levels={levels}")
+    if levels == "all":
+        monkeypatch.setenv("HF_SWEEP_MIN_CELLS", "0")
     case = SWEEP_EDGES[name]()
     pol = dict(null_space_constant=0, f_cycle=f_cycle,
                anisotropy_threshold=1e9)
@@ -334,16 +340,20 @@ def test_fused_sweep_edges_fp64_bitwise(torch_cuda, name, f_cycle):
     plan.close()
 
 
+@pytest.mark.parametrize("levels", ["finest", "all"])
 @pytest.mark.parametrize("name", list(SWEEP_EDGES))
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("null", [0, 1])
 def test_fused_sweep_edges_match_two_pass(torch_cuda, monkeypatch, name,
-                                          precision, null):
+                                          precision, null, levels):
     """32- and 16-bit storage: the fused sweep and the two colour passes
     give the same cycle bit for bit (with the constant null space too: the
     first sweep then applies the projection of b on the fly)."""
     torch = torch_cuda
-    _say(f"[sweep edges] case={name} precision={precision} null={null}")
+    _say(f"[sweep edges] case={name} precision={precision} null={null} "
+         f"levels={levels}")
+    if levels == "all":
+        monkeypatch.setenv("HF_SWEEP_MIN_CELLS", "0")
     case = SWEEP_EDGES[name]()
     pol = dict(null_space_constant=null, anisotropy_threshold=1e9,
                precision=precision)
