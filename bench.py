@@ -84,10 +84,12 @@ def measured_traffic(workload_tag, cls, precision, grid):
         data = json.load(f)
     entry = data["kernels"].get(f"{name}|{grid}")
     if entry is None and cls == "sweep_finest":
-        # the fused sweep runs on the finest level only: one launch geometry
+        # the fused sweep also runs on large coarse levels: the finest
+        # level's launches are the ones moving the most bytes
         hits = [v for k, v in data["kernels"].items()
                 if k.split("|")[0] == name]
-        entry = hits[0] if len(hits) == 1 else None
+        entry = max(hits, key=lambda v: v["fetch_bytes"] + v["write_bytes"],
+                    default=None)
     if entry is None:
         return None
     return int(entry["fetch_bytes"] + entry["write_bytes"])
