@@ -271,7 +271,6 @@ struct hf_plan {
   // Krylov workspace (5 vectors, solve_pcg slots 0..4) + adjoint scratch
   double* kx = nullptr;
   double* kr = nullptr;
-  double* kz = nullptr;
   double* kp = nullptr;
   double* kq = nullptr;
   double* adj_rhs = nullptr;
@@ -1266,7 +1265,6 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   guard(dalloc(p, &p->d_flag, 4));
   guard(dalloc(p, &p->kx, p->cells));
   guard(dalloc(p, &p->kr, p->cells));
-  guard(dalloc(p, &p->kz, p->cells));
   guard(dalloc(p, &p->kp, p->cells));
   guard(dalloc(p, &p->kq, p->cells));
   const int tail_levels = static_cast<int>(p->levels.size()) - p->tail_first;
@@ -1472,7 +1470,6 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   const int nbv = chunk_blocks(ch);
   double* x = p->kx;
   double* r = p->kr;
-  double* z = p->kz;
   double* pp = p->kp;
   double* q = p->kq;
   const uint64_t red0 = p->reductions;
@@ -1515,11 +1512,12 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, solution, x);
     return finish(ok(), HF_TERM_CONVERGED);
   }
+  // p = z = M r: the preconditioner writes z straight into p (nothing else
+  // reads this z)
   auto precondition_restart = [&]() -> hf_status {
-    HF_TRY(mg_apply(p, r, z, nullptr, true));
+    HF_TRY(mg_apply(p, r, pp, nullptr, true));
     res->preconditioner_applies++;
     finalize(p, p->levels[0].tred.blocks, 1, F_RHO, 0);
-    HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, pp, z);
     return ok();
   };
   HF_TRY(precondition_restart());
