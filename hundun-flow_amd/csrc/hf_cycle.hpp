@@ -841,8 +841,53 @@ __device__ __forceinline__ void tail_apply(const Lv<double>& L, const Bc& bc,
   __syncthreads();
 }
 
+// tail_smooth on a level of at most one cell per thread: the cell's
+// coefficients are loaded once and kept in registers across the passes (the
+// same values and arithmetic as the general form below).
+__device__ void tail_smooth_small(const Lv<double>& L, int fused, const Bc& bc,
+                                  int sweeps, int reverse, int retain) {
+  const double omega = 0.72;
+  const int c = threadIdx.x;
+  const bool mine = c < L.cells;
+  int i = 0, j = 0, k = 0;
+  StencilCoef<double> f{};
+  if (mine) {
+    tail_ijk(L, c, i, j, k);
+    f = stencil_coef<double>(L, j, k, c);
+  }
+  auto apply = [&](int defect) {
+    if (mine) {
+      const double v = stencil_with<double>(L, bc, L.x, i, j, k, c, L.x[c], f);
+      L.r[c] = defect ? L.b[c] - v : v;
+    }
+    __syncthreads();
+  };
+  if (!fused) apply(0);
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    const int first = reverse ? 1 : 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int color = (first + pass) & 1;
+      if (mine && ((i + j + k) & 1) == color) {
+        const double xc = L.x[c];
+        const double v =
+            fused ? stencil_with<double>(L, bc, L.x, i, j, k, c, xc, f) : L.r[c];
+        L.x[c] = xc + omega * (L.b[c] - v) / f.d;
+      }
+      __syncthreads();
+      const bool final_update = sweep + 1 == sweeps && pass == 1;
+      if (!final_update || retain) {
+        if (!fused || (final_update && retain)) apply(retain && final_update);
+      }
+    }
+  }
+}
+
 __device__ void tail_smooth(const Lv<double>& L, int fused, const Bc& bc,
                             int sweeps, int reverse, int retain) {
+  if (L.cells <= static_cast<int>(blockDim.x)) {
+    tail_smooth_small(L, fused, bc, sweeps, reverse, retain);
+    return;
+  }
   const double omega = 0.72;
   if (!fused) tail_apply(L, bc, 0);
   for (int sweep = 0; sweep < sweeps; ++sweep) {
@@ -946,12 +991,6 @@ k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
       T0.L.x[c] = L0.x[c];
   }
 }
-
-// ---------------------------------------------------------------------------
-// Krylov stencil kernels (FP64, chunked).  The operator arrays are the masked
-// level-0 coefficients, which equal the exact IBM operator.
-// ---------------------------------------------------------------------------
-
 
 // ---------------------------------------------------------------------------
 // z-marching stencil kernels.  A block owns one 2-D tile of the plane (the

@@ -83,16 +83,40 @@ __device__ __forceinline__ C neighbour_or_boundary(int mode, C self,
   return ld(x[wrapped]);
 }
 
+// The seven coefficients of the row of cell c = (i, j, k) of a level.
+template <class C>
+struct StencilCoef {
+  C d, xl, xh, yl, yh, zl, zh;
+};
+
+template <class C, class T>
+__device__ __forceinline__ StencilCoef<C> stencil_coef(const Lv<T>& L, int j,
+                                                       int k, int c) {
+  const int row = j + L.ny * k;
+  const int fyo = c + L.nx * k;
+  StencilCoef<C> f;
+  f.d = C(ld(L.D[c]));
+  f.xl = C(ld(L.fx[c + row]));
+  f.xh = C(ld(L.fx[c + row + 1]));
+  f.yl = C(ld(L.fy[fyo]));
+  f.yh = C(ld(L.fy[fyo + L.nx]));
+  f.zl = C(ld(L.fz[c]));
+  f.zh = C(ld(L.fz[c + L.nx * L.ny]));
+  return f;
+}
+
 // A x at cell (i,j,k) in the reference order: D*x, then x-lo, x-hi, y-lo,
 // y-hi, z-lo, z-hi, each product subtracted in turn (src/solver_mg.cpp:
-// 2405-2411).  Built with -ffp-contract=off so FP64 results are bitwise equal
-// to the sequential reference.
+// 2405-2411), with the row's coefficients given.  Built with
+// -ffp-contract=off so FP64 results are bitwise equal to the sequential
+// reference.
 template <class C, class T>
-__device__ __forceinline__ C stencil(const Lv<T>& L, const Bc& bc, const T* x,
-                                     int i, int j, int k, int c, C xc) {
+__device__ __forceinline__ C stencil_with(const Lv<T>& L, const Bc& bc,
+                                          const T* x, int i, int j, int k,
+                                          int c, C xc,
+                                          const StencilCoef<C>& f) {
   const int nx = L.nx;
   const int plane = nx * L.ny;
-  const int row = j + L.ny * k;
   const C xl = i > 0 ? ld(x[c - 1])
                      : neighbour_or_boundary<C>(bc.m[0], xc, x, c + (nx - 1));
   const C xh = i + 1 < nx
@@ -112,15 +136,21 @@ __device__ __forceinline__ C stencil(const Lv<T>& L, const Bc& bc, const T* x,
                    ? ld(x[c + plane])
                    : neighbour_or_boundary<C>(bc.m[5], xc, x,
                                               c - (L.nz - 1) * plane);
-  const int fyo = c + nx * k;
-  C v = C(ld(L.D[c])) * xc;
-  v -= C(ld(L.fx[c + row])) * xl;
-  v -= C(ld(L.fx[c + row + 1])) * xh;
-  v -= C(ld(L.fy[fyo])) * yl;
-  v -= C(ld(L.fy[fyo + nx])) * yh;
-  v -= C(ld(L.fz[c])) * zl;
-  v -= C(ld(L.fz[c + plane])) * zh;
+  C v = f.d * xc;
+  v -= f.xl * xl;
+  v -= f.xh * xh;
+  v -= f.yl * yl;
+  v -= f.yh * yh;
+  v -= f.zl * zl;
+  v -= f.zh * zh;
   return v;
+}
+
+// A x at cell (i,j,k), coefficients loaded from the level.
+template <class C, class T>
+__device__ __forceinline__ C stencil(const Lv<T>& L, const Bc& bc, const T* x,
+                                     int i, int j, int k, int c, C xc) {
+  return stencil_with<C>(L, bc, x, i, j, k, c, xc, stencil_coef<C>(L, j, k, c));
 }
 
 // Block index -> logical tile such that each XCD owns one contiguous slab of
