@@ -111,6 +111,16 @@ __global__ void k_update_xr(Chunk ch, double* x, const double* p, double* r,
 }
 
 
+// x += alpha p alone, for an iteration whose r the true residual replaces
+// (the x update of k_update_xr, same expression).
+__global__ void k_update_x(Chunk ch, double* x, const double* p,
+                           const double* alpha) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  const double a = *alpha;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) x[c] = x[c] + a * p[c];
+}
+
 __global__ void k_copy(Chunk ch, double* dst, const double* src) {
   int lb, lo, hi;
   chunk_range(ch, lb, lo, hi);

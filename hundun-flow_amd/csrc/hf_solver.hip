@@ -1536,7 +1536,17 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     }
     res->operator_applies++;
     finalize(p, tr.blocks, 1, F_ALPHA, 0);
-    if (fuse_entry) {
+    // an iteration that verifies on schedule replaces r with the true
+    // residual, so its update forms x alone and its step checks wait for the
+    // true residual's read
+    const bool scheduled =
+        (res->iterations + 1) % ctl->true_residual_interval == 0 ||
+        res->iterations + 1 == ctl->maximum_iterations;
+    if (scheduled) {
+      HF_PROF(p, P_UPDATE_XR);
+      HF_LAUNCH(k_update_x, nbv, kBlock, 0, p->stream, ch, x,
+                static_cast<const double*>(pp), p->scalars + S_ALPHA);
+    } else if (fuse_entry) {
       // the update also stores the preconditioner's input b (used unless
       // this iteration verifies the true residual, which replaces r)
       HF_PROF(p, P_UPDATE_XR);
@@ -1556,22 +1566,33 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       HF_LAUNCH(k_update_xr, nbv, kBlock, 0, p->stream, ch, x, pp, r, q,
                 p->scalars + S_ALPHA, p->partials);
     }
-    finalize(p, fuse_entry ? p->levels[0].tred.blocks : nbv, 1, F_STORE,
-             S_RNORM2);
+    if (!scheduled)
+      finalize(p, fuse_entry ? p->levels[0].tred.blocks : nbv, 1, F_STORE,
+               S_RNORM2);
     res->iterations++;
-    HF_TRY(read_scalars(p));
-    const double pap = p->host_scalars[S_PAP];
-    if (!(pap > 0.0) || !std::isfinite(pap))
-      return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown), HF_TERM_BREAKDOWN);
-    if (!std::isfinite(p->host_scalars[S_ALPHA]))
-      return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
-    const double recursive = std::sqrt(p->host_scalars[S_RNORM2]);
-    if (!std::isfinite(recursive))
-      return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
-    res->recursive_residual = recursive;
-    const bool verify = recursive <= tol ||
-                        res->iterations % ctl->true_residual_interval == 0 ||
-                        res->iterations == ctl->maximum_iterations;
+    // p.Ap and alpha (read with the recursive norm, or with the true one)
+    auto step_failure = [&]() -> int {
+      const double pap = p->host_scalars[S_PAP];
+      if (!(pap > 0.0) || !std::isfinite(pap)) return HF_TERM_BREAKDOWN;
+      if (!std::isfinite(p->host_scalars[S_ALPHA])) return HF_TERM_NON_FINITE;
+      return 0;
+    };
+    auto fail_step = [&](int term) {
+      return finish(fail(HF_NUMERICAL_FAILURE, term == HF_TERM_BREAKDOWN
+                                                   ? kDetailBreakdown
+                                                   : kDetailNonFinite),
+                    term);
+    };
+    bool verify = scheduled;
+    if (!scheduled) {
+      HF_TRY(read_scalars(p));
+      if (const int term = step_failure()) return fail_step(term);
+      const double recursive = std::sqrt(p->host_scalars[S_RNORM2]);
+      if (!std::isfinite(recursive))
+        return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
+      res->recursive_residual = recursive;
+      verify = recursive <= tol;
+    }
     if (verify) {
       {
         HF_PROF(p, P_TRUE_RESIDUAL);
@@ -1582,6 +1603,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
       res->operator_applies++;
       HF_TRY(read_scalars(p));
+      if (scheduled) {
+        if (const int term = step_failure()) return fail_step(term);
+      }
       res->final_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
       res->recursive_residual = res->final_true_residual;
       if (!std::isfinite(res->final_true_residual))
