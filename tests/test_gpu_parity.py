@@ -235,6 +235,64 @@ def test_pcg_fp64_matches_oracle(torch_cuda, name, flexible, scaling):
     plan.close()
 
 
+# the oracle's termination codes, in the order of the C ABI's
+_TERMS = ["converged", "zero_rhs", "maximum_iterations", "breakdown"]
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+@pytest.mark.parametrize("maxit", [5, 1000])
+@pytest.mark.parametrize("flexible", [False, True])
+def test_pcg_gates_match_oracle(torch_cuda, interval, maxit, flexible):
+    """The PCG gates (src/solver_krylov.cpp:1299-1555): the true-residual
+    interval, the iteration limit and its failure, counted as the oracle
+    counts them.  A solve stopped by the limit leaves the caller's solution
+    untouched; iterations that verify on schedule update x alone, so the
+    operator and preconditioner counts pin which iterations verified."""
+    torch = torch_cuda
+    _say(f"[pcg gates] interval={interval} maxit={maxit} flexible={flexible}")
+    case = CASES["step"]()
+    pol = _policy("step")
+    plan = _gpu_plan(torch, case, **pol)
+    (d, bx, by, bz), mg = O.from_case(case, O.policy(**pol))
+    b = case["rhs"]
+    cells = b.size
+    x0 = np.random.default_rng(5).standard_normal(cells) * 1e-3
+    x0[case["cell_active"] == 0] = 0.0
+    xg_t = _t(torch, x0.copy())
+    xg_t, res = plan.solve_pcg(_t(torch, b), solution=xg_t, check=False,
+                               rtol=1e-12, maximum_iterations=maxit,
+                               true_residual_interval=interval,
+                               flexible=flexible)
+    xo, ores = O.pcg(case["n"], case["bc"], d, bx, by, bz,
+                     case["cell_active"], mg, b, x0=x0, rtol=1e-12,
+                     maximum_iterations=maxit,
+                     true_residual_interval=interval, flexible=flexible)
+    want = _TERMS[ores.termination]
+    _say(f"  gpu {res.termination} it={res.iterations} "
+         f"ops={res.operator_applies} pcs={res.preconditioner_applies} | "
+         f"oracle {want} it={ores.iterations} ops={ores.operator_applies} "
+         f"pcs={ores.preconditioner_applies}")
+    assert res.termination == want
+    xg = xg_t.cpu().numpy()
+    if want == "maximum_iterations":
+        assert res.iterations == ores.iterations == maxit
+        assert res.operator_applies == ores.operator_applies
+        assert res.preconditioner_applies == ores.preconditioner_applies
+        # same stopping point: the true residuals agree to rounding
+        rel = abs(res.final_true_residual - ores.final_true_residual)
+        assert rel <= 1e-8 * ores.final_true_residual
+        # a failed solve publishes no solution
+        assert np.array_equal(xg, x0)
+    else:
+        assert abs(res.iterations - ores.iterations) <= 2
+        act = case["cell_active"]
+        Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
+        assert np.linalg.norm(b - Ax) <= 1.01e-12 * np.linalg.norm(b)
+        ga, go = _gauge(xg, act), _gauge(xo, act)
+        assert np.linalg.norm(ga - go) <= 1e-9 * np.linalg.norm(go)
+    plan.close()
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("name", ["sphere32", "step"])
 @pytest.mark.parametrize("scaling", [0, 1])
