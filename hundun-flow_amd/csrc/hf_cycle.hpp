@@ -60,16 +60,6 @@ __global__ void k_count(Chunk ch, const uint8_t* act, double* partials) {
   if (threadIdx.x == 0) partials[lb] = v;
 }
 
-__global__ void k_sumsq(Chunk ch, const double* v, double* partials) {
-  __shared__ double scratch[16];
-  int lb, lo, hi;
-  chunk_range(ch, lb, lo, hi);
-  double d = 0.0;
-  for (int c = lo + threadIdx.x; c < hi; c += kBlock) d += v[c] * v[c];
-  d = block_sum(d, scratch);
-  if (threadIdx.x == 0) partials[lb] = d;
-}
-
 __global__ void k_dot(Chunk ch, const double* a, const double* b,
                       double* partials) {
   __shared__ double scratch[16];
@@ -1012,14 +1002,15 @@ k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
 //   kZOpDot    q = A p and the partial sum of p.q (Krylov operator)
 //   kZResidual r = 1 b + (-1) A x and the partial sum of r.r (the true
 //              residual, axpby(r, 1, b, -1, Ax): src/solver_krylov.cpp:
-//              1082-1091)
-//   kZApply    r = b - A x (defect) or r = A x at level storage
+//              1082-1091); aux > 0: also the partial sum of b.b, aux
+//              partials after those of r.r
+//   kZApply    r = b - A x (aux != 0, defect) or r = A x at level storage
 // ---------------------------------------------------------------------------
 constexpr int kZOpDot = 0, kZResidual = 1, kZApply = 2;
 
 template <class T, int kMode>
 __global__ void k_stencil_z(Tiles t, Lv<T> L, Bc bc, const T* x, const T* b,
-                            T* out, double* partials, int defect) {
+                            T* out, double* partials, int aux) {
   using C = typename Compute<T>::type;
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
@@ -1030,7 +1021,7 @@ __global__ void k_stencil_z(Tiles t, Lv<T> L, Bc bc, const T* x, const T* b,
   const int nx = t.nx, ny = t.ny, nz = t.nz;
   const int k0 = zb * t.per;
   const int k1 = min(nz, k0 + t.per);
-  double d = 0.0;
+  double d = 0.0, bb = 0.0;
   if (i < nx && j < ny) {
     const int plane = nx * ny;
     int c = cell_index(nx, ny, i, j, k0);
@@ -1072,11 +1063,13 @@ __global__ void k_stencil_z(Tiles t, Lv<T> L, Bc bc, const T* x, const T* b,
         out[c] = v;
         d += xc * v;
       } else if constexpr (kMode == kZResidual) {
-        const double rc = 1.0 * b[c] + -1.0 * v;
+        const double bv = b[c];
+        const double rc = 1.0 * bv + -1.0 * v;
         out[c] = rc;
         d += rc * rc;
+        bb += bv * bv;
       } else {
-        out[c] = defect ? st<T>(C(ld(b[c])) - v) : st<T>(v);
+        out[c] = aux ? st<T>(C(ld(b[c])) - v) : st<T>(v);
       }
       zl = xc;
       xc = zh;
@@ -1087,6 +1080,10 @@ __global__ void k_stencil_z(Tiles t, Lv<T> L, Bc bc, const T* x, const T* b,
   if (kMode != kZApply && partials != nullptr) {
     d = block_sum(d, scratch);
     if (threadIdx.x == 0) partials[lb] = d;
+    if (kMode == kZResidual && aux > 0) {
+      bb = block_sum(bb, scratch);
+      if (threadIdx.x == 0) partials[lb + aux] = bb;
+    }
   }
 }
 

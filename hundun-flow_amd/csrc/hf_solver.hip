@@ -1479,9 +1479,18 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     res->reduction_calls = p->reductions - red0;
     return st;
   };
-  // ||b||
-  HF_LAUNCH(k_sumsq, nbv, kBlock, 0, p->stream, ch, rhs, p->partials);
-  finalize(p, nbv, 1, F_STORE, S_SUM);
+  // ||b|| and the initial true residual come from one pass and one read; the
+  // gates then run in the reference's order (a non-finite ||b||, a zero
+  // ||b||, whose residual goes unused, then the residual)
+  HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, x, solution);
+  {
+    HF_PROF(p, P_TRUE_RESIDUAL);
+    HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
+              p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs, r,
+              p->partials, static_cast<int>(p->partial_stride));
+  }
+  finalize(p, tr.blocks, 1, F_STORE, S_SUM, 0.0, 1, 1);
+  finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
   HF_TRY(read_scalars(p));
   const double bnorm = std::sqrt(p->host_scalars[S_SUM]);
   res->rhs_norm = bnorm;
@@ -1493,16 +1502,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     HF_TRY(hip_check(hipMemsetAsync(solution, 0, N * sizeof(double), p->stream)));
     return finish(ok(), HF_TERM_ZERO_RHS);
   }
-  HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, x, solution);
-  {
-    HF_PROF(p, P_TRUE_RESIDUAL);
-    HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
-              p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs, r,
-              p->partials, 0);
-  }
-  finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
   res->operator_applies++;
-  HF_TRY(read_scalars(p));
   res->initial_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
   res->final_true_residual = res->initial_true_residual;
   res->recursive_residual = res->initial_true_residual;

@@ -293,6 +293,64 @@ def test_pcg_gates_match_oracle(torch_cuda, interval, maxit, flexible):
     plan.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pcg_start_gates(torch_cuda, precision):
+    """The gates before the first iteration (src/solver_krylov.cpp:
+    1321-1395): a zero right-hand side zeroes the solution; a non-finite
+    right-hand side fails as non_finite and a non-finite initial guess as
+    operator_failure, both leaving the caller's solution untouched; an
+    initial guess that already meets the tolerance is returned as it is,
+    after no iteration."""
+    import hundun_flow as H
+    torch = torch_cuda
+    _say(f"[pcg start gates] precision={precision}")
+    case = CASES["step"]()
+    plan = _gpu_plan(torch, case, **_policy("step", precision=precision))
+    act = case["cell_active"]
+    b = case["rhs"]
+    fluid = np.flatnonzero(act)
+    x0 = np.random.default_rng(9).standard_normal(b.size)
+    x0[act == 0] = 0.0
+    numerical = H.STATUS_NAMES.index("numerical_failure")
+
+    x, res = plan.solve_pcg(_t(torch, np.zeros_like(b)),
+                            solution=_t(torch, x0), rtol=1e-10)
+    assert res.termination == "zero_rhs" and res.status == 0
+    assert res.iterations == 0 and res.rhs_norm == 0.0
+    assert not x.cpu().numpy().any()
+
+    bad = b.copy()
+    bad[fluid[len(fluid) // 2]] = np.nan
+    x, res = plan.solve_pcg(_t(torch, bad), solution=_t(torch, x0),
+                            check=False, rtol=1e-10)
+    assert res.termination == "non_finite" and res.status == numerical
+    assert np.array_equal(x.cpu().numpy(), x0)
+
+    xinf = x0.copy()
+    xinf[fluid[0]] = np.inf
+    x, res = plan.solve_pcg(_t(torch, b), solution=_t(torch, xinf),
+                            check=False, rtol=1e-10)
+    assert res.termination == "operator_failure" and res.status == numerical
+    assert res.iterations == 0
+    assert np.array_equal(x.cpu().numpy(), xinf)
+
+    xs, res = plan.solve_pcg(_t(torch, b), rtol=1e-10, flexible=True)
+    assert res.termination == "converged" and res.iterations > 0
+    want = xs.cpu().numpy().copy()
+    x, res = plan.solve_pcg(_t(torch, b), solution=xs, rtol=1e-8,
+                            flexible=True)
+    assert res.termination == "converged" and res.iterations == 0
+    assert res.operator_applies == 1 and res.preconditioner_applies == 0
+    assert np.array_equal(x.cpu().numpy(), want)
+    # the norms the gates use: ||b|| and the initial true residual
+    assert abs(res.rhs_norm - np.linalg.norm(b)) <= 1e-12 * np.linalg.norm(b)
+    (d, bx, by, bz), _ = O.from_case(case, O.policy(**_policy("step")))
+    r0 = np.linalg.norm(b - O.operator_apply(case["n"], case["bc"], d, bx, by,
+                                             bz, act, want))
+    assert abs(res.initial_true_residual - r0) <= 1e-6 * r0
+    plan.close()
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("name", ["sphere32", "step"])
 @pytest.mark.parametrize("scaling", [0, 1])
