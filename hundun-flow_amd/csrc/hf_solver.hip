@@ -1601,11 +1601,13 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                   r, p->partials, 0);
       }
       finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
-      res->operator_applies++;
       HF_TRY(read_scalars(p));
+      // a step that fails these checks stops before the true residual, as
+      // in the reference, so that residual is not counted
       if (scheduled) {
         if (const int term = step_failure()) return fail_step(term);
       }
+      res->operator_applies++;
       res->final_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
       res->recursive_residual = res->final_true_residual;
       if (!std::isfinite(res->final_true_residual))
