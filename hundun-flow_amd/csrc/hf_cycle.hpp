@@ -1087,6 +1087,92 @@ __global__ void k_stencil_z(Tiles t, Lv<T> L, Bc bc, const T* x, const T* b,
   }
 }
 
+// The true residual of k_stencil_z<double, kZResidual> (same operands and
+// order) fused with the preconditioner's entry of k_mg_entry, which the
+// restart after an unconverged check runs on that residual: b_mg = fluid ?
+// r : 0 at the level-0 storage precision.  Accumulators `stride` apart:
+// r.r, then (proj) vol * b_mg over the fluid cells, then (with_bb) b.b.
+template <class Te>
+__global__ void k_residual_entry(Tiles t, Lv<double> L, Bc bc, const double* x,
+                                 const double* rhs, double* r,
+                                 const uint8_t* act, Widths w, Te* bmg,
+                                 double* partials, int stride, int proj,
+                                 int with_bb) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  double d = 0.0, m = 0.0, bb = 0.0;
+  if (i < nx && j < ny) {
+    const int plane = nx * ny;
+    const double wxy = w.x[i] * w.y[j];
+    int c = cell_index(nx, ny, i, j, k0);
+    double xc = x[c];
+    double zl = k0 > 0 ? x[c - plane]
+                       : neighbour_or_boundary<double>(bc.m[4], xc, x,
+                                                       c + (nz - 1) * plane);
+    double fzl = L.fz[c];
+    for (int k = k0; k < k1; ++k) {
+      const double zh = k + 1 < nz
+                            ? x[c + plane]
+                            : neighbour_or_boundary<double>(bc.m[5], xc, x,
+                                                            c - (nz - 1) * plane);
+      const double fzh = L.fz[c + plane];
+      const int row = j + ny * k;
+      const double xl = i > 0 ? x[c - 1]
+                              : neighbour_or_boundary<double>(bc.m[0], xc, x,
+                                                              c + (nx - 1));
+      const double xh = i + 1 < nx
+                            ? x[c + 1]
+                            : neighbour_or_boundary<double>(bc.m[1], xc, x,
+                                                            c - (nx - 1));
+      const double yl = j > 0 ? x[c - nx]
+                              : neighbour_or_boundary<double>(bc.m[2], xc, x,
+                                                              c + (ny - 1) * nx);
+      const double yh = j + 1 < ny
+                            ? x[c + nx]
+                            : neighbour_or_boundary<double>(bc.m[3], xc, x,
+                                                            c - (ny - 1) * nx);
+      const int fyo = c + nx * k;
+      double v = L.D[c] * xc;
+      v -= L.fx[c + row] * xl;
+      v -= L.fx[c + row + 1] * xh;
+      v -= L.fy[fyo] * yl;
+      v -= L.fy[fyo + nx] * yh;
+      v -= fzl * zl;
+      v -= fzh * zh;
+      const double bv = rhs[c];
+      const double rc = 1.0 * bv + -1.0 * v;
+      r[c] = rc;
+      d += rc * rc;
+      bb += bv * bv;
+      const bool fluid = act[c] != 0;
+      const Te be = fluid ? st<Te>(rc) : st<Te>(0.0);
+      bmg[c] = be;
+      if (fluid) m += wxy * w.z[k] * static_cast<double>(ld(be));
+      zl = xc;
+      xc = zh;
+      fzl = fzh;
+      c += plane;
+    }
+  }
+  d = block_sum(d, scratch);
+  if (threadIdx.x == 0) partials[lb] = d;
+  if (proj) {
+    m = block_sum(m, scratch);
+    if (threadIdx.x == 0) partials[lb + stride] = m;
+  }
+  if (with_bb) {
+    bb = block_sum(bb, scratch);
+    if (threadIdx.x == 0) partials[lb + 2 * stride] = bb;
+  }
+}
+
 
 }  // namespace hf
 

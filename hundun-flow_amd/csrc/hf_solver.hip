@@ -932,7 +932,7 @@ struct ExitOperands {
 // receives the operands z is formed from.
 hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
                    bool dots, bool entered = false,
-                   ExitOperands* exit = nullptr) {
+                   ExitOperands* exit = nullptr, int64_t entry_blocks = 0) {
   Level& L0 = p->levels[0];
   const bool proj = p->policy.null_space_constant != 0;
   // whole cycle inside the tail kernel, or a line level 0: x = 0 and the
@@ -950,8 +950,10 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
                 proj ? p->partials : static_cast<double*>(nullptr));
     }
     if (proj) {
-      finalize(p, L0.tred.blocks, 1, F_MEAN, S_MEAN0, L0.volume, 1,
-               entered ? 1 : 0);
+      // an entry made by the caller leaves its partials in accumulator 1
+      // (entry_blocks of them when not in the reduction layout)
+      finalize(p, entered && entry_blocks > 0 ? entry_blocks : L0.tred.blocks,
+               1, F_MEAN, S_MEAN0, L0.volume, 1, entered ? 1 : 0);
       if (tail0) project_apply<T>(p, 0, b);
     }
   });
@@ -1479,17 +1481,36 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     res->reduction_calls = p->reductions - red0;
     return st;
   };
+  // r = b - A x with r.r in accumulator 0 and, with_bb, b.b in accumulator
+  // 2.  On a grid level 0 the pass also makes the preconditioner's entry
+  // from r (accumulator 1), which the restart that follows an unconverged
+  // check then skips.
+  const int stride = static_cast<int>(p->partial_stride);
+  auto true_residual = [&](bool with_bb) -> hf_status {
+    HF_PROF(p, P_TRUE_RESIDUAL);
+    if (!fuse_entry) {
+      HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
+                p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs,
+                r, p->partials, with_bb ? 2 * stride : 0);
+      return ok();
+    }
+    Level& L0 = p->levels[0];
+    dispatch(L0.prec, [&](auto t) {
+      using T = decltype(t);
+      HF_LAUNCH(k_residual_entry<T>, tr.blocks, kBlock, 0, p->stream, tr, A,
+                p->bc, static_cast<const double*>(x), rhs, r,
+                static_cast<const uint8_t*>(L0.act), widths(p),
+                static_cast<T*>(L0.b), p->partials, stride,
+                p->policy.null_space_constant != 0 ? 1 : 0, with_bb ? 1 : 0);
+    });
+    return ok();
+  };
   // ||b|| and the initial true residual come from one pass and one read; the
   // gates then run in the reference's order (a non-finite ||b||, a zero
   // ||b||, whose residual goes unused, then the residual)
   HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, x, solution);
-  {
-    HF_PROF(p, P_TRUE_RESIDUAL);
-    HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
-              p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs, r,
-              p->partials, static_cast<int>(p->partial_stride));
-  }
-  finalize(p, tr.blocks, 1, F_STORE, S_SUM, 0.0, 1, 1);
+  HF_TRY(true_residual(true));
+  finalize(p, tr.blocks, 1, F_STORE, S_SUM, 0.0, 1, 2);
   finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
   HF_TRY(read_scalars(p));
   const double bnorm = std::sqrt(p->host_scalars[S_SUM]);
@@ -1515,7 +1536,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   // p = z = M r: the preconditioner writes z straight into p (nothing else
   // reads this z)
   auto precondition_restart = [&]() -> hf_status {
-    HF_TRY(mg_apply(p, r, pp, nullptr, true));
+    HF_TRY(mg_apply(p, r, pp, nullptr, true, fuse_entry, nullptr, tr.blocks));
     res->preconditioner_applies++;
     finalize(p, p->levels[0].tred.blocks, 1, F_RHO, 0);
     return ok();
@@ -1594,12 +1615,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       verify = recursive <= tol;
     }
     if (verify) {
-      {
-        HF_PROF(p, P_TRUE_RESIDUAL);
-        HF_LAUNCH((k_stencil_z<double, kZResidual>), tr.blocks, kBlock, 0,
-                  p->stream, tr, A, p->bc, static_cast<const double*>(x), rhs,
-                  r, p->partials, 0);
-      }
+      HF_TRY(true_residual(false));
       finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
       HF_TRY(read_scalars(p));
       // a step that fails these checks stops before the true residual, as
