@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <vector>
 
@@ -62,14 +63,12 @@ int64_t blocks_for(int64_t work) {
   return std::max<int64_t>(1, (work + kBlock - 1) / kBlock);
 }
 
-struct Level {
-  int n[3];
-  int64_t cells = 0, nfx = 0, nfy = 0, nfz = 0;
-  int coarsen_mask = 0;
-  int line_mask = 0;
-  int prec = HF_PRECISION_FP64;
-  int fused = 1;
-  bool tail = false;
+// The numeric hierarchy of one level.  A plan holds the published set in
+// its Level and, once a second refresh arrives, a spare set that the refresh
+// builds into and that is swapped in only when the whole hierarchy passed
+// its checks (the active/inactive hierarchy storage of src/solver_mg.cpp:
+// 4700-4770).
+struct Coeffs {
   double volume = 0.0;  // sum of fluid volumes (constant projection)
   // FP64 hierarchy (level 0 doubles as the Krylov operator)
   double* D64 = nullptr;
@@ -83,6 +82,17 @@ struct Level {
   void* fxT = nullptr;
   void* fyT = nullptr;
   void* fzT = nullptr;
+};
+
+struct Level : Coeffs {
+  int n[3];
+  int64_t cells = 0, nfx = 0, nfy = 0, nfz = 0;
+  int coarsen_mask = 0;
+  int line_mask = 0;
+  int prec = HF_PRECISION_FP64;
+  int fused = 1;
+  bool tail = false;
+  Coeffs spare;  // inactive hierarchy (allocated at the first refresh)
   // workspace slots in storage precision; x and b are the current buffers of
   // a ping-pong pair on levels smoothed by the fused sweep kernel
   void* x = nullptr;
@@ -363,6 +373,30 @@ hf_status dalloc_raw(hf_plan* p, void** out, size_t bytes) {
 #define HF_LAUNCH(kernel, grid, block, shared, stream, ...)             \
   hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)),         \
                      dim3(block), shared, stream, __VA_ARGS__)
+
+// Device arrays of one numeric hierarchy set of level L.
+hf_status alloc_coeffs(hf_plan* p, const Level& L, Coeffs& k) {
+  HF_TRY(dalloc(p, &k.D64, L.cells));
+  HF_TRY(dalloc(p, &k.fx64, L.nfx));
+  HF_TRY(dalloc(p, &k.fy64, L.nfy));
+  HF_TRY(dalloc(p, &k.fz64, L.nfz));
+  HF_TRY(dalloc(p, &k.vol, L.cells));
+  HF_TRY(dalloc(p, &k.act, L.cells));
+  if (L.prec == HF_PRECISION_FP64) {
+    k.DT = k.D64;
+    k.fxT = k.fx64;
+    k.fyT = k.fy64;
+    k.fzT = k.fz64;
+    return ok();
+  }
+  const size_t es = elem_size(L.prec);
+  HF_TRY(dalloc_raw(p, &k.DT, es * L.cells));
+  HF_TRY(dalloc_raw(p, &k.fxT, es * L.nfx));
+  HF_TRY(dalloc_raw(p, &k.fyT, es * L.nfy));
+  HF_TRY(dalloc_raw(p, &k.fzT, es * L.nfz));
+  return ok();
+}
+
 
 
 // Level planner, src/solver_mg_workspace.cpp:113-143, :181-199, :317-453.
@@ -1090,6 +1124,14 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       !(policy->anisotropy_threshold > 1.0) || policy->precision > 2 ||
       policy->coarse_face_scaling > 1 || policy->line_order > 1)
     return fail(HF_INVALID_PLAN, kDetailPlan);
+  {
+    // every kernel indexes cells and faces with 32-bit integers
+    const int64_t nx = grid->nx, ny = grid->ny, nz = grid->nz;
+    const int64_t limit = std::numeric_limits<int32_t>::max();
+    if (nx * ny * nz > limit || (nx + 1) * ny * nz > limit ||
+        nx * (ny + 1) * nz > limit || nx * ny * (nz + 1) > limit)
+      return fail(HF_INVALID_PLAN, kDetailPlan);
+  }
   hf_plan* p = new (std::nothrow) hf_plan();
   if (p == nullptr) return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
   p->grid = *grid;
@@ -1217,24 +1259,8 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     guard(dalloc(p, &p->d_widths[a], p->n[a]));
   }
   for (Level& L : p->levels) {
-    guard(dalloc(p, &L.D64, L.cells));
-    guard(dalloc(p, &L.fx64, L.nfx));
-    guard(dalloc(p, &L.fy64, L.nfy));
-    guard(dalloc(p, &L.fz64, L.nfz));
-    guard(dalloc(p, &L.vol, L.cells));
-    guard(dalloc(p, &L.act, L.cells));
+    guard(alloc_coeffs(p, L, L));
     const size_t es = elem_size(L.prec);
-    if (L.prec == HF_PRECISION_FP64) {
-      L.DT = L.D64;
-      L.fxT = L.fx64;
-      L.fyT = L.fy64;
-      L.fzT = L.fz64;
-    } else {
-      guard(dalloc_raw(p, &L.DT, es * L.cells));
-      guard(dalloc_raw(p, &L.fxT, es * L.nfx));
-      guard(dalloc_raw(p, &L.fyT, es * L.nfy));
-      guard(dalloc_raw(p, &L.fzT, es * L.nfz));
-    }
     guard(dalloc_raw(p, &L.x, es * L.cells));
     guard(dalloc_raw(p, &L.b, es * L.cells));
     guard(dalloc_raw(p, &L.r, es * L.cells));
@@ -1342,22 +1368,25 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
   const bool any = ca || fxa || fya || fza;
   const bool all = ca && fxa && fya && fza;
   if (any && !all) return fail(HF_INVALID_PLAN, kDetailPlan);
-  p->has_activity = all;
-  Level& L0 = p->levels[0];
-  if (all) {
-    if (p->fxa == nullptr) {
-      HF_TRY(dalloc(p, &p->fxa, L0.nfx));
-      HF_TRY(dalloc(p, &p->fya, L0.nfy));
-      HF_TRY(dalloc(p, &p->fza, L0.nfz));
-    }
-    HF_TRY(hip_check(hipMemcpyAsync(p->fxa, fxa, L0.nfx, hipMemcpyDeviceToDevice, p->stream)));
-    HF_TRY(hip_check(hipMemcpyAsync(p->fya, fya, L0.nfy, hipMemcpyDeviceToDevice, p->stream)));
-    HF_TRY(hip_check(hipMemcpyAsync(p->fza, fza, L0.nfz, hipMemcpyDeviceToDevice, p->stream)));
+  // The first publication builds into the level arrays themselves; a
+  // refresh of a published hierarchy builds into the spare set and swaps it
+  // in only after every check passed, so a rejected refresh leaves the
+  // previous hierarchy solving exactly as before (src/solver_mg.cpp:
+  // 4700-4770).
+  const bool into_spare = p->coefficients_ready;
+  std::vector<Coeffs*> target(p->levels.size());
+  for (size_t l = 0; l < p->levels.size(); ++l) {
+    Level& L = p->levels[l];
+    if (into_spare && L.spare.D64 == nullptr)
+      HF_TRY(alloc_coeffs(p, L, L.spare));
+    target[l] = into_spare ? &L.spare : static_cast<Coeffs*>(&L);
   }
+  Level& L0 = p->levels[0];
+  Coeffs& K0 = *target[0];
   // level 0: masking
-  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfx), kBlock, 0, p->stream, bx, fxa, L0.fx64, L0.nfx);
-  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfy), kBlock, 0, p->stream, by, fya, L0.fy64, L0.nfy);
-  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfz), kBlock, 0, p->stream, bz, fza, L0.fz64, L0.nfz);
+  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfx), kBlock, 0, p->stream, bx, fxa, K0.fx64, L0.nfx);
+  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfy), kBlock, 0, p->stream, by, fya, K0.fy64, L0.nfy);
+  HF_LAUNCH(k_mask_faces, blocks_for(L0.nfz), kBlock, 0, p->stream, bz, fza, K0.fz64, L0.nfz);
   Finest f{};
   f.nx = p->n[0];
   f.ny = p->n[1];
@@ -1373,69 +1402,96 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
   f.wx = p->d_widths[0];
   f.wy = p->d_widths[1];
   f.wz = p->d_widths[2];
-  f.oD = L0.D64;
-  f.ofx = L0.fx64;
-  f.ofy = L0.fy64;
-  f.ofz = L0.fz64;
-  f.vol = L0.vol;
-  f.act = L0.act;
+  f.oD = K0.D64;
+  f.ofx = K0.fx64;
+  f.ofy = K0.fy64;
+  f.ofz = K0.fz64;
+  f.vol = K0.vol;
+  f.act = K0.act;
   HF_LAUNCH(k_mask_cells, blocks_for(L0.cells), kBlock, 0, p->stream, f);
   // coarse levels
   HF_TRY(hip_check(hipMemsetAsync(p->d_flag, 0, sizeof(int), p->stream)));
   for (size_t l = 1; l < p->levels.size(); ++l) {
-    Level& F = p->levels[l - 1];
-    Level& C = p->levels[l];
+    const Level& F = p->levels[l - 1];
+    const Level& C = p->levels[l];
+    const Coeffs& KF = *target[l - 1];
+    const Coeffs& KC = *target[l];
     Agg a;
     for (int ax = 0; ax < 3; ++ax) {
       a.fn[ax] = F.n[ax];
       a.cn[ax] = C.n[ax];
     }
     a.mask = F.coarsen_mask;
-    Fine64 fine{F.D64, F.fx64, F.fy64, F.fz64, F.vol};
-    Coarse64 coarse{C.D64, C.fx64, C.fy64, C.fz64, C.vol, C.act};
+    Fine64 fine{KF.D64, KF.fx64, KF.fy64, KF.fz64, KF.vol};
+    Coarse64 coarse{KC.D64, KC.fx64, KC.fy64, KC.fz64, KC.vol, KC.act};
     HF_LAUNCH(k_coarse_cells, blocks_for(C.cells), kBlock, 0, p->stream, a, fine, coarse);
     // distance-corrected aggregation halves faces whose normal was coarsened
     const bool dc = p->policy.coarse_face_scaling != 0;
     const double sx = dc && (F.coarsen_mask & 1) ? 0.5 : 1.0;
     const double sy = dc && (F.coarsen_mask & 2) ? 0.5 : 1.0;
     const double sz = dc && (F.coarsen_mask & 4) ? 0.5 : 1.0;
-    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfx), kBlock, 0, p->stream, a, 0, F.fx64, C.fx64, sx);
-    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfy), kBlock, 0, p->stream, a, 1, F.fy64, C.fy64, sy);
-    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfz), kBlock, 0, p->stream, a, 2, F.fz64, C.fz64, sz);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfx), kBlock, 0, p->stream, a, 0, KF.fx64, KC.fx64, sx);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfy), kBlock, 0, p->stream, a, 1, KF.fy64, KC.fy64, sy);
+    HF_LAUNCH(k_coarse_faces, blocks_for(C.nfz), kBlock, 0, p->stream, a, 2, KF.fz64, KC.fz64, sz);
     HF_LAUNCH(k_coarse_diag, blocks_for(C.cells), kBlock, 0, p->stream, C.n[0], C.n[1], C.n[2], coarse, p->d_flag);
   }
-  for (Level& L : p->levels)
-    HF_LAUNCH(k_validate, blocks_for(L.cells), kBlock, 0, p->stream, L.n[0], L.n[1], L.n[2], L.D64, L.fx64, L.fy64, L.fz64, L.vol, p->d_flag);
+  for (size_t l = 0; l < p->levels.size(); ++l) {
+    const Level& L = p->levels[l];
+    const Coeffs& K = *target[l];
+    HF_LAUNCH(k_validate, blocks_for(L.cells), kBlock, 0, p->stream, L.n[0], L.n[1], L.n[2], K.D64, K.fx64, K.fy64, K.fz64, K.vol, p->d_flag);
+  }
   // storage-precision copies
-  for (Level& L : p->levels) {
+  for (size_t l = 0; l < p->levels.size(); ++l) {
+    const Level& L = p->levels[l];
+    const Coeffs& K = *target[l];
     if (L.prec == HF_PRECISION_FP64) continue;
     dispatch(L.prec, [&](auto t) {
       using T = decltype(t);
-      HF_LAUNCH(k_convert<T>, blocks_for(L.cells), kBlock, 0, p->stream, L.D64, static_cast<T*>(L.DT), L.cells);
-      HF_LAUNCH(k_convert<T>, blocks_for(L.nfx), kBlock, 0, p->stream, L.fx64, static_cast<T*>(L.fxT), L.nfx);
-      HF_LAUNCH(k_convert<T>, blocks_for(L.nfy), kBlock, 0, p->stream, L.fy64, static_cast<T*>(L.fyT), L.nfy);
-      HF_LAUNCH(k_convert<T>, blocks_for(L.nfz), kBlock, 0, p->stream, L.fz64, static_cast<T*>(L.fzT), L.nfz);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.cells), kBlock, 0, p->stream, K.D64, static_cast<T*>(K.DT), L.cells);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.nfx), kBlock, 0, p->stream, K.fx64, static_cast<T*>(K.fxT), L.nfx);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.nfy), kBlock, 0, p->stream, K.fy64, static_cast<T*>(K.fyT), L.nfy);
+      HF_LAUNCH(k_convert<T>, blocks_for(L.nfz), kBlock, 0, p->stream, K.fz64, static_cast<T*>(K.fzT), L.nfz);
     });
   }
   // projection volumes (sum of fluid volumes per level)
   for (size_t l = 0; l < p->levels.size(); ++l) {
-    Level& L = p->levels[l];
+    const Level& L = p->levels[l];
+    Coeffs& K = *target[l];
     const Chunk ch = make_chunk(static_cast<int>(L.cells));
     const int nb = chunk_blocks(ch);
-    HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, ch, L.vol,
+    HF_LAUNCH(k_sum, nb, kBlock, 0, p->stream, ch, K.vol,
               static_cast<const uint8_t*>(nullptr), p->partials);
     finalize(p, nb, 1, F_STORE, S_SUM);
     HF_TRY(read_scalars(p));
-    L.volume = p->host_scalars[S_SUM];
+    K.volume = p->host_scalars[S_SUM];
   }
   int flag = 0;
   HF_TRY(hip_check(hipMemcpyAsync(&flag, p->d_flag, sizeof(int), hipMemcpyDeviceToHost, p->stream)));
   HF_TRY(hip_check(hipStreamSynchronize(p->stream)));
+  HF_TRY(hip_check(hipGetLastError()));
   if (flag != 0) return fail(HF_NUMERICAL_FAILURE, kDetailCoefficients);
-  if (p->policy.null_space_constant && !(p->levels[0].volume > 0.0))
+  if (p->policy.null_space_constant && !(target[0]->volume > 0.0))
     return fail(HF_NUMERICAL_FAILURE, kDetailCoefficients);
-  HF_TRY(upload_tail(p));
-  p->coefficients_ready = true;
+  // every check passed: keep the raw activity masks for the adjoint, then
+  // publish
+  if (all) {
+    if (p->fxa == nullptr) {
+      HF_TRY(dalloc(p, &p->fxa, L0.nfx));
+      HF_TRY(dalloc(p, &p->fya, L0.nfy));
+      HF_TRY(dalloc(p, &p->fza, L0.nfz));
+    }
+    HF_TRY(hip_check(hipMemcpyAsync(p->fxa, fxa, L0.nfx, hipMemcpyDeviceToDevice, p->stream)));
+    HF_TRY(hip_check(hipMemcpyAsync(p->fya, fya, L0.nfy, hipMemcpyDeviceToDevice, p->stream)));
+    HF_TRY(hip_check(hipMemcpyAsync(p->fza, fza, L0.nfz, hipMemcpyDeviceToDevice, p->stream)));
+  }
+  p->has_activity = all;
+  if (into_spare)
+    for (Level& L : p->levels) std::swap(static_cast<Coeffs&>(L), L.spare);
+  // the tail kernel reads its level descriptors from device memory; a plan
+  // whose descriptors could not follow the swap accepts no further solves
+  const hf_status uploaded = upload_tail(p);
+  p->coefficients_ready = uploaded.code == HF_OK;
+  HF_TRY(uploaded);
   return hip_check(hipGetLastError());
 }
 
@@ -1548,7 +1604,28 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     if (!(rho > 0.0) || !std::isfinite(rho))
       return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown), HF_TERM_BREAKDOWN);
   }
+  // r.z of the latest preconditioner application is tested at the next
+  // host read (the reference tests it before the next operator apply,
+  // src/solver_krylov.cpp:1520-1544); a failing value reports the breakdown
+  // with the counts of the reference, which stopped before the operator
+  // apply and the update issued since.
+  bool rho_pending = false;
   while (res->iterations < ctl->maximum_iterations) {
+    const uint64_t operator_applies0 = res->operator_applies;
+    const uint32_t iterations0 = res->iterations;
+    auto rho_failed = [&]() -> bool {
+      if (!rho_pending) return false;
+      rho_pending = false;
+      const double rho = p->host_scalars[S_RHO];
+      if (rho > 0.0 && std::isfinite(rho)) return false;
+      res->operator_applies = operator_applies0;
+      res->iterations = iterations0;
+      return true;
+    };
+    auto fail_rho = [&]() {
+      return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown),
+                    HF_TERM_BREAKDOWN);
+    };
     {
       HF_PROF(p, P_OPERATOR);
       HF_LAUNCH((k_stencil_z<double, kZOpDot>), tr.blocks, kBlock, 0,
@@ -1607,6 +1684,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     bool verify = scheduled;
     if (!scheduled) {
       HF_TRY(read_scalars(p));
+      if (rho_failed()) return fail_rho();
       if (const int term = step_failure()) return fail_step(term);
       const double recursive = std::sqrt(p->host_scalars[S_RNORM2]);
       if (!std::isfinite(recursive))
@@ -1618,6 +1696,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       HF_TRY(true_residual(false));
       finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
       HF_TRY(read_scalars(p));
+      if (rho_failed()) return fail_rho();
       // a step that fails these checks stops before the true residual, as
       // in the reference, so that residual is not counted
       if (scheduled) {
@@ -1635,6 +1714,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       if (res->iterations == ctl->maximum_iterations)
         return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
       HF_TRY(precondition_restart());
+      rho_pending = true;
       continue;
     }
     // z is formed again by the direction update instead of being stored
@@ -1642,6 +1722,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
     HF_TRY(mg_apply(p, r, nullptr, ctl->flexible ? q : nullptr, true,
                     fuse_entry, &ex));
     res->preconditioner_applies++;
+    rho_pending = true;
     finalize(p, p->levels[0].tred.blocks, ctl->flexible ? 2 : 1,
              ctl->flexible ? F_BETA_PR : F_BETA_FR, 0);
     {

@@ -1000,14 +1000,25 @@ double or_norm2(const double* v, size_t n) {
   return scale * sqrt(squares);
 }
 
+/* Operator and preconditioner of one Krylov solve. */
+typedef struct or_krylov_ops {
+  size_t n;
+  void (*apply_a)(void* ctx, const double* x, double* y);
+  void* a_ctx;
+  /* NULL: identity preconditioner */
+  int (*apply_m)(void* ctx, const double* r, double* z);
+  void* m_ctx;
+  /* optional: x after every iteration, row-major [iteration][n] */
+  double* trace;
+  uint32_t trace_capacity;
+} or_krylov_ops;
+
 /* solve_pcg, src/solver_krylov.cpp:1299-1555: true residual every
  * true_residual_interval iterations (and at the recursive gate); an
  * unconverged verification restarts the direction p = M r. */
-int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
-                 const double* bx, const double* by, const double* bz,
-                 const uint8_t* act, or_mg* mg, const double* rhs,
-                 double* solution, const or_control* ctl, or_result* res) {
-  const size_t N = (size_t)n[0] * n[1] * n[2];
+static int pcg_core(const or_krylov_ops* ops, const double* rhs,
+                    double* solution, const or_control* ctl, or_result* res) {
+  const size_t N = ops->n;
   memset(res, 0, sizeof *res);
   double* x = (double*)malloc(N * sizeof(double));
   double* r = (double*)malloc(N * sizeof(double));
@@ -1031,7 +1042,7 @@ int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
     goto done;
   }
   memcpy(x, solution, N * sizeof(double));
-  or_operator_apply(n, bc, diag, bx, by, bz, act, x, ap);
+  ops->apply_a(ops->a_ctx, x, ap);
   res->operator_applies++;
   for (size_t i = 0; i < N; ++i) r[i] = 1.0 * rhs[i] + -1.0 * ap[i];
   res->initial_true_residual = or_norm2(r, N);
@@ -1043,8 +1054,8 @@ int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
   }
 #define OR_PRECONDITION()                                             \
   do {                                                                \
-    if (mg != NULL) {                                                 \
-      const int pst = or_mg_apply(mg, r, z);                          \
+    if (ops->apply_m != NULL) {                                       \
+      const int pst = ops->apply_m(ops->m_ctx, r, z);                 \
       if (pst != ST_OK) OR_FAIL(pst, TERM_NON_FINITE);                \
     } else {                                                          \
       memcpy(z, r, N * sizeof(double));                               \
@@ -1053,25 +1064,29 @@ int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
   } while (0)
   OR_PRECONDITION();
   double rho = or_dot(r, z, N);
+  res->breakdown_value = rho;
   if (!(rho > 0.0) || !isfinite(rho)) OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
   memcpy(p, z, N * sizeof(double));
   while (res->iterations < ctl->maximum_iterations) {
-    or_operator_apply(n, bc, diag, bx, by, bz, act, p, ap);
+    ops->apply_a(ops->a_ctx, p, ap);
     res->operator_applies++;
     const double denominator = or_dot(p, ap, N);
+    res->breakdown_value = denominator;
     if (!(denominator > 0.0) || !isfinite(denominator))
       OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
     const double alpha = rho / denominator;
     if (!isfinite(alpha)) OR_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
     for (size_t i = 0; i < N; ++i) x[i] = x[i] + alpha * p[i];
     for (size_t i = 0; i < N; ++i) r[i] = r[i] + -alpha * ap[i];
+    if (ops->trace != NULL && res->iterations < ops->trace_capacity)
+      memcpy(ops->trace + (size_t)res->iterations * N, x, N * sizeof(double));
     res->iterations++;
     const double recursive = or_norm2(r, N);
     const int verify = recursive <= tol ||
                        res->iterations % ctl->true_residual_interval == 0 ||
                        res->iterations == ctl->maximum_iterations;
     if (verify) {
-      or_operator_apply(n, bc, diag, bx, by, bz, act, x, ap);
+      ops->apply_a(ops->a_ctx, x, ap);
       res->operator_applies++;
       for (size_t i = 0; i < N; ++i) r[i] = 1.0 * rhs[i] + -1.0 * ap[i];
       res->final_true_residual = or_norm2(r, N);
@@ -1084,12 +1099,14 @@ int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
         OR_FAIL(ST_REJECTED, TERM_MAXIMUM_ITERATIONS);
       OR_PRECONDITION();
       rho = or_dot(r, z, N);
+      res->breakdown_value = rho;
       if (!(rho > 0.0) || !isfinite(rho)) OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
       memcpy(p, z, N * sizeof(double));
       continue;
     }
     OR_PRECONDITION();
     const double rho_next = or_dot(r, z, N);
+    res->breakdown_value = rho_next;
     if (!(rho_next > 0.0) || !isfinite(rho_next))
       OR_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
     double beta;
@@ -1113,4 +1130,77 @@ done:
   return st;
 #undef OR_FAIL
 #undef OR_PRECONDITION
+}
+
+typedef struct ibm_operator_ctx {
+  const int* n;
+  const int* bc;
+  const double *diag, *bx, *by, *bz;
+  const uint8_t* act;
+} ibm_operator_ctx;
+
+static void ibm_operator_cb(void* ctx, const double* x, double* y) {
+  const ibm_operator_ctx* c = (const ibm_operator_ctx*)ctx;
+  or_operator_apply(c->n, c->bc, c->diag, c->bx, c->by, c->bz, c->act, x, y);
+}
+
+static int mg_precondition_cb(void* ctx, const double* r, double* z) {
+  return or_mg_apply((or_mg*)ctx, r, z);
+}
+
+int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
+                 const double* bx, const double* by, const double* bz,
+                 const uint8_t* act, or_mg* mg, const double* rhs,
+                 double* solution, const or_control* ctl, or_result* res) {
+  ibm_operator_ctx a = {n, bc, diag, bx, by, bz, act};
+  or_krylov_ops ops;
+  memset(&ops, 0, sizeof ops);
+  ops.n = (size_t)n[0] * n[1] * n[2];
+  ops.apply_a = ibm_operator_cb;
+  ops.a_ctx = &a;
+  if (mg != NULL) {
+    ops.apply_m = mg_precondition_cb;
+    ops.m_ctx = mg;
+  }
+  return pcg_core(&ops, rhs, solution, ctl, res);
+}
+
+/* Dense test form of the same recurrence: y = A x and z = M r with row-major
+ * n x n matrices (M NULL: identity). */
+typedef struct dense_ctx {
+  size_t n;
+  const double* m;
+} dense_ctx;
+
+static void dense_apply_cb(void* ctx, const double* x, double* y) {
+  const dense_ctx* d = (const dense_ctx*)ctx;
+  for (size_t i = 0; i < d->n; ++i) {
+    double v = 0.0;
+    for (size_t j = 0; j < d->n; ++j) v += d->m[i * d->n + j] * x[j];
+    y[i] = v;
+  }
+}
+
+static int dense_precondition_cb(void* ctx, const double* r, double* z) {
+  dense_apply_cb(ctx, r, z);
+  return ST_OK;
+}
+
+int or_pcg_dense(int n, const double* a, const double* m, const double* rhs,
+                 double* solution, const or_control* ctl, or_result* res,
+                 double* trace, uint32_t trace_capacity) {
+  dense_ctx ac = {(size_t)n, a};
+  dense_ctx mc = {(size_t)n, m};
+  or_krylov_ops ops;
+  memset(&ops, 0, sizeof ops);
+  ops.n = (size_t)n;
+  ops.apply_a = dense_apply_cb;
+  ops.a_ctx = &ac;
+  if (m != NULL) {
+    ops.apply_m = dense_precondition_cb;
+    ops.m_ctx = &mc;
+  }
+  ops.trace = trace;
+  ops.trace_capacity = trace_capacity;
+  return pcg_core(&ops, rhs, solution, ctl, res);
 }

@@ -65,6 +65,7 @@ typedef struct or_result {
   double rhs_norm;
   uint64_t operator_applies;
   uint64_t preconditioner_applies;
+  double breakdown_value; /* the last gated scalar (r.z or p.Ap) */
 } or_result;
 
 typedef struct or_mg or_mg;
@@ -118,6 +119,13 @@ int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
                  const uint8_t* cell_active, or_mg* mg, const double* rhs,
                  double* solution, const or_control* control,
                  or_result* result);
+
+/* The same PCG recurrence over dense row-major n x n matrices (test form):
+ * operator a, preconditioner m (NULL: identity).  trace, when not NULL,
+ * receives x after each of the first trace_capacity iterations. */
+int or_pcg_dense(int n, const double* a, const double* m, const double* rhs,
+                 double* solution, const or_control* control,
+                 or_result* result, double* trace, uint32_t trace_capacity);
 
 /* Vector kernels, src/solver_krylov.cpp:176-204 and :650-711. */
 double or_dot(const double* a, const double* b, size_t n);

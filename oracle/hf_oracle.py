@@ -42,7 +42,8 @@ class or_result(ctypes.Structure):
                 ("final_true_residual", ctypes.c_double),
                 ("rhs_norm", ctypes.c_double),
                 ("operator_applies", ctypes.c_uint64),
-                ("preconditioner_applies", ctypes.c_uint64)]
+                ("preconditioner_applies", ctypes.c_uint64),
+                ("breakdown_value", ctypes.c_double)]
 
 
 _lib = None
@@ -90,6 +91,11 @@ def lib():
                                    _D, _D, ctypes.POINTER(or_control),
                                    ctypes.POINTER(or_result)]
         L.or_pcg_solve.restype = ctypes.c_int
+        L.or_pcg_dense.argtypes = [ctypes.c_int, _D, _D, _D, _D,
+                                   ctypes.POINTER(or_control),
+                                   ctypes.POINTER(or_result), _D,
+                                   ctypes.c_uint32]
+        L.or_pcg_dense.restype = ctypes.c_int
         L.or_dot.argtypes = [_D, _D, ctypes.c_size_t]
         L.or_dot.restype = ctypes.c_double
         L.or_norm2.argtypes = [_D, ctypes.c_size_t]
@@ -240,6 +246,26 @@ def pcg(n, bc, diag, bx, by, bz, cell_active, mg, rhs, x0=None, rtol=1e-10,
                        _u8(cell_active), None if mg is None else mg.h,
                        _d(rhs), _d(x), ctypes.byref(ctl), ctypes.byref(res))
     return x, res
+
+
+def pcg_dense(a, m, rhs, x0=None, rtol=1e-12, atol=0.0,
+              maximum_iterations=200, true_residual_interval=1000,
+              flexible=False, trace=0):
+    """The oracle PCG recurrence on a dense operator `a` with dense
+    preconditioner `m` (None: identity).  Returns (x, result, iterates)."""
+    n = a.shape[0]
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    mm = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    ctl = or_control(atol, rtol, maximum_iterations, true_residual_interval,
+                     1 if flexible else 0)
+    res = or_result()
+    tr = np.zeros((max(trace, 1), n))
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().or_pcg_dense(n, _d(a), None if mm is None else _d(mm), _d(rhs),
+                       _d(x), ctypes.byref(ctl), ctypes.byref(res), _d(tr),
+                       trace)
+    return x, res, tr[:min(trace, res.iterations)]
 
 
 def dot(a, b):

@@ -57,3 +57,31 @@ def test_invalid_arguments_are_rejected_without_gpu():
     st = lib.hf_plan_create(None, None, None, None)
     assert st.code == 2
     assert lib.hf_status_message(H.hf_status(5, 0)) == b"numerical failure"
+
+
+@pytest.mark.parametrize("n", [(2048, 1024, 1025), (46341, 46341, 1),
+                               (2147483647, 1, 1)])
+def test_grid_beyond_32bit_indexing_is_refused(n):
+    """A grid whose cell or face count exceeds 2^31 - 1 is refused with
+    invalid_plan before any allocation (the kernels index with int32)."""
+    import numpy as np
+    import hundun_flow as H
+    lib = H.load_library()
+    nx, ny, nz = n
+    # face coordinates are only read after the size check; keep them short
+    # for the huge extents by pointing at a small valid array
+    f = [np.linspace(0.0, 1.0, min(v, 4096) + 1) for v in n]
+    desc = H.hf_grid_desc()
+    desc.nx, desc.ny, desc.nz = nx, ny, nz
+    keep = [np.ascontiguousarray(v) for v in f]
+    desc.faces_x = keep[0].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    desc.faces_y = keep[1].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    desc.faces_z = keep[2].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for i, b in enumerate([1, 1, 1, 1, 1, 1]):
+        desc.boundary[i] = b
+    pol = H.default_policy()
+    out = ctypes.c_void_p()
+    st = lib.hf_plan_create(ctypes.byref(desc), ctypes.byref(pol), None,
+                            ctypes.byref(out))
+    assert st.code == 2 and st.detail == 9101
+    assert not out.value

@@ -229,8 +229,10 @@ def test_pcg_fp64_matches_oracle(torch_cuda, name, flexible, scaling):
     _say(f"  it gpu={res.iterations} oracle={ores.iterations} "
          f"true_rel={true_rel:.2e} diff={diff:.2e}")
     assert true_rel <= 1.01e-12
-    assert abs(res.iterations - ores.iterations) <= 2
-    assert diff <= 1e-9
+    # the Krylov dots are summed in another order than the oracle's; the
+    # iterates agree to about 1e-15, well inside the 1e-10 parity bar
+    assert res.iterations == ores.iterations
+    assert diff <= 1e-10
     assert np.all(xg[act == 0] == 0.0)
     plan.close()
 
@@ -284,12 +286,14 @@ def test_pcg_gates_match_oracle(torch_cuda, interval, maxit, flexible):
         # a failed solve publishes no solution
         assert np.array_equal(xg, x0)
     else:
-        assert abs(res.iterations - ores.iterations) <= 2
+        assert res.iterations == ores.iterations
+        assert res.operator_applies == ores.operator_applies
+        assert res.preconditioner_applies == ores.preconditioner_applies
         act = case["cell_active"]
         Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
         assert np.linalg.norm(b - Ax) <= 1.01e-12 * np.linalg.norm(b)
         ga, go = _gauge(xg, act), _gauge(xo, act)
-        assert np.linalg.norm(ga - go) <= 1e-9 * np.linalg.norm(go)
+        assert np.linalg.norm(ga - go) <= 1e-10 * np.linalg.norm(go)
     plan.close()
 
 
