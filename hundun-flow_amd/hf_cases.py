@@ -161,9 +161,24 @@ def lognormal_sphere(n=(64, 64, 64), seed: int = 1, radius: float = 0.2,
 
 
 def divergence_rhs(faces, bc, rng, noise: float = 0.1) -> np.ndarray:
-    """Discrete divergence of a random solenoidal-plus-noise face velocity:
-    u = curl(psi) sampled on faces (psi random, box-filtered) plus white
-    noise; wall-normal velocities are zero, periodic faces wrap."""
+    """Discrete divergence of a random solenoidal-plus-noise face velocity
+    (face_velocity), integrated over each cell."""
+    u, v, w = face_velocity(faces, bc, rng, noise)
+    dx, dy, dz = (np.diff(f) for f in faces)
+    ax = dy[None, :, None] * dz[:, None, None]
+    ay = dx[None, None, :] * dz[:, None, None]
+    az = dx[None, None, :] * dy[None, :, None]
+    div = ((u[:, :, 1:] - u[:, :, :-1]) * ax
+           + (v[:, 1:, :] - v[:, :-1, :]) * ay
+           + (w[1:, :, :] - w[:-1, :, :]) * az)
+    return div
+
+
+def face_velocity(faces, bc, rng, noise: float = 0.1):
+    """Random solenoidal-plus-noise face velocity: u = curl(psi) sampled on
+    faces (psi random, box-filtered) plus white noise; wall-normal
+    velocities are zero, periodic faces wrap.  Arrays (nz, ny, nx+1),
+    (nz, ny+1, nx), (nz+1, ny, nx)."""
     nx, ny, nz = (len(f) - 1 for f in faces)
     psi = [box_filter(rng.standard_normal((nz + 1, ny + 1, nx + 1)),
                       (False, False, False), 2) for _ in range(3)]
@@ -192,13 +207,7 @@ def divergence_rhs(faces, bc, rng, noise: float = 0.1) -> np.ndarray:
             last[npax] = -1
             arr[tuple(first)] = 0.0
             arr[tuple(last)] = 0.0
-    ax = dy[None, :, None] * dz[:, None, None]
-    ay = dx[None, None, :] * dz[:, None, None]
-    az = dx[None, None, :] * dy[None, :, None]
-    div = ((u[:, :, 1:] - u[:, :, :-1]) * ax
-           + (v[:, 1:, :] - v[:, :-1, :]) * ay
-           + (w[1:, :, :] - w[:-1, :, :]) * az)
-    return div
+    return u, v, w
 
 
 def flame_cylinder(n=(256, 256, 256), seed: int = 2, ratio: float = 8.0,
@@ -287,12 +296,77 @@ def random_spheres(n=(1024, 512, 512), seed: int = 5, count: int = 200,
     return _pack(n, faces, bc, rho, ~solid, rhs)
 
 
+def compressible_flame(n=(64, 64, 64), seed: int = 6,
+                       acoustic_cfl: float = 4.0, bdf2: bool = True,
+                       solid: bool = True, source: bool = True,
+                       density_ratio: float = 7.0,
+                       gamma: float = 1.4) -> dict:
+    """Inputs of the low-Mach pressure-correction system
+    (src/solver_pressure.cpp:110-332) on a flame-sheet density field: trial
+    density rho, accepted rho_n (and rho_nm1 for BDF2) perturbed by 1%,
+    drho/dp = rho / (gamma p) at p = 1, BDF coefficients for the step dt
+    chosen so that c_max dt / h_min = acoustic_cfl, face coefficients by the
+    face rule from the mobility dt / rho, face fluxes phi = u . A of a
+    random solenoidal-plus-noise velocity, an optional small mass source,
+    an optional cylinder mask.  Neumann x/y walls, periodic z.  The
+    diagonal carries a0 V drho/dp > 0, so the system has no null space;
+    small acoustic CFL numbers make it strongly diagonally dominant."""
+    nx, ny, nz = n
+    rng = _rng(seed)
+    faces = [uniform_faces(nx), uniform_faces(ny), uniform_faces(nz)]
+    bc = [NEUMANN, NEUMANN, NEUMANN, NEUMANN, PERIODIC, PERIODIC]
+    X, Y, Z = _grid(faces)
+    rho = flame_sheet(Y, density_ratio) * np.exp(
+        0.02 * rng.standard_normal((nz, ny, nx)))
+    h = min(float(np.min(np.diff(f))) for f in faces)
+    c_max = math.sqrt(gamma / float(rho.min()))
+    dt = acoustic_cfl * h / c_max
+    if bdf2:
+        a = (1.5 / dt, -2.0 / dt, 0.5 / dt)
+    else:
+        a = (1.0 / dt, -1.0 / dt, 0.0)
+    rho_n = rho * (1.0 + 0.01 * rng.standard_normal(rho.shape))
+    rho_nm1 = rho * (1.0 + 0.01 * rng.standard_normal(rho.shape))
+    u, v, w = face_velocity(faces, bc, rng, noise=0.1)
+    dx, dy, dz = (np.diff(f) for f in faces)
+    phi = [u * (dy[None, :, None] * dz[:, None, None]),
+           v * (dx[None, None, :] * dz[:, None, None]),
+           w * (dx[None, None, :] * dy[None, :, None])]
+    active = np.ones((nz, ny, nx), dtype=bool)
+    if solid:
+        active = ~((X - 0.5) ** 2 + (Y - 0.5) ** 2 < 0.1 * 0.1)
+    act = active.astype(np.uint8)
+    fx, fy, fz = face_activity(act, bc)
+    flat = lambda arr: np.ascontiguousarray(arr.ravel(), dtype=np.float64)
+    case = {
+        "n": tuple(int(v) for v in n),
+        "faces": [np.ascontiguousarray(f, dtype=np.float64) for f in faces],
+        "bc": list(bc),
+        "mobility": flat(dt / rho),
+        "rho": flat(rho),
+        "rho_n": flat(rho_n),
+        "rho_nm1": flat(rho_nm1) if bdf2 else None,
+        "drho_dp": flat(rho / gamma),
+        "source": flat(0.01 * rng.standard_normal(rho.shape)) if source
+        else None,
+        "bdf": a,
+        "phi": [flat(phi[0]), flat(phi[1]), flat(phi[2])],
+        "cell_active": np.ascontiguousarray(act.ravel()),
+        "x_active": fx,
+        "y_active": fy,
+        "z_active": fz,
+        "dt": dt,
+    }
+    return case
+
+
 CASES = {
     "lognormal_sphere": lognormal_sphere,
     "flame_cylinder": flame_cylinder,
     "flame_step": flame_step,
     "flame_probe": flame_probe,
     "random_spheres": random_spheres,
+    "compressible_flame": compressible_flame,
 }
 
 

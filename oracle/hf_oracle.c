@@ -35,6 +35,7 @@ typedef struct or_level {
   size_t cells, nfx, nfy, nfz;
   double *D, *fx, *fy, *fz, *vol;
   double *x, *b, *r, *t;
+  int chebyshev; /* Chebyshev-Jacobi point smoother on this level */
 } or_level;
 
 struct or_mg {
@@ -44,6 +45,7 @@ struct or_mg {
   or_level level[OR_MAX_LEVELS];
   uint8_t* cell_active; /* copy, level-0 cells (NULL = all active) */
   int line_failed;      /* a Thomas pivot vanished during the last apply */
+  int shortcut;         /* diagonal shortcut selected: z = D^-1 r */
 };
 
 static size_t cidx(const int n[3], int i, int j, int k) {
@@ -421,6 +423,28 @@ static int build_coarse(or_mg* mg) {
   return status;
 }
 
+/* select_diagonal_shortcut, src/solver_mg.cpp:1573-1611: with no null
+ * space, use D^-1 when the largest finest-row face sum over the diagonal
+ * (masked coefficients, physical boundary faces included) is at most the
+ * policy ratio. */
+static int select_diagonal_shortcut(const or_mg* mg) {
+  if (mg->policy.diagonal_shortcut_ratio == 0.0 ||
+      mg->policy.null_space_constant)
+    return 0;
+  const or_level* L = &mg->level[0];
+  double maximum = 0.0;
+  for (int k = 0; k < L->n[2]; ++k)
+    for (int j = 0; j < L->n[1]; ++j)
+      for (int i = 0; i < L->n[0]; ++i) {
+        const double sum = faces_sum(L, L->fx, L->fy, L->fz, i, j, k);
+        const double ratio = sum / L->D[cidx(L->n, i, j, k)];
+        if (ratio > maximum) maximum = ratio; /* std::max(maximum, ratio) */
+      }
+  return maximum <= mg->policy.diagonal_shortcut_ratio;
+}
+
+int or_mg_diagonal_shortcut(const or_mg* mg) { return mg->shortcut; }
+
 or_mg* or_mg_create(const int n[3], const double* wx, const double* wy,
                     const double* wz, const int bc[6], const or_policy* policy,
                     const double* diag, const double* bx, const double* by,
@@ -434,7 +458,12 @@ or_mg* or_mg_create(const int n[3], const double* wx, const double* wy,
   if (policy->pre_sweeps <= 0 || policy->post_sweeps <= 0 ||
       policy->coarse_sweeps <= 0 || policy->maximum_levels < 2 ||
       policy->maximum_levels > OR_MAX_LEVELS ||
-      policy->minimum_coarse_extent < 2 || !(policy->anisotropy_threshold > 1.0))
+      policy->minimum_coarse_extent < 2 || !(policy->anisotropy_threshold > 1.0) ||
+      policy->point_smoother < 0 || policy->point_smoother > 1 ||
+      !(policy->chebyshev_lower_fraction > 0.0) ||
+      !(policy->chebyshev_lower_fraction < 1.0) ||
+      !(policy->diagonal_shortcut_ratio >= 0.0) ||
+      !(policy->diagonal_shortcut_ratio < 1.0))
     return NULL;
   or_mg* mg = (or_mg*)calloc(1, sizeof(or_mg));
   memcpy(mg->bc, bc, sizeof mg->bc);
@@ -486,6 +515,20 @@ or_mg* or_mg_create(const int n[3], const double* wx, const double* wy,
     memcpy(mg->cell_active, ca, mg->level[0].cells);
   }
   copy_finest(mg, w, diag, bx, by, bz, ca, fxa, fya, fza);
+  /* the Chebyshev-Jacobi smoother on every point level but the coarsest
+   * (src/solver_mg.cpp:1048-1062) */
+  for (int l = 0; l < mg->levels; ++l) {
+    or_level* L = &mg->level[l];
+    L->chebyshev = l + 1 < mg->levels && L->line_mask == 0 &&
+                   policy->point_smoother == 1;
+  }
+  mg->shortcut = select_diagonal_shortcut(mg);
+  if (mg->shortcut) {
+    /* the coarse hierarchy of a diagonal plan is never read
+     * (src/solver_mg.cpp:4556-4566) */
+    *status = ST_OK;
+    return mg;
+  }
   *status = build_coarse(mg);
   if (*status != ST_OK) {
     or_mg_destroy(mg);
@@ -752,10 +795,58 @@ static void line_smooth_axis(or_mg* mg, int level, int ax, int sweeps,
   if (!ok) mg->line_failed = 1;
 }
 
-/* smooth(), src/solver_mg.cpp:3557-3590: point levels use red-black, line
- * levels relax each line axis in x, y, z order (reverse: z, y, x). */
+/* Chebyshev-Jacobi smoothing, src/solver_mg.cpp:2916-3090 with the
+ * streamed stage of :2440-2560: the Jacobi-preconditioned spectrum is
+ * bounded by [fraction * 2, 2]; stage s updates the direction
+ * d = f_prev * d + f_res * (b - A x) / D and x += d from the previous
+ * stage's x (Jacobi: all neighbours from the previous stage).  The retained
+ * defect is b - A x after the last stage. */
+static void chebyshev_smooth(or_mg* mg, int level, int stages, int retain) {
+  or_level* L = &mg->level[level];
+  const int* n = L->n;
+  const double beta = 2.0;
+  const double alpha = mg->policy.chebyshev_lower_fraction * beta;
+  const double theta = (beta + alpha) * 0.5;
+  const double delta = (beta - alpha) * 0.5;
+  const double sigma = theta / delta;
+  double previous_rho = 1.0 / sigma;
+  double* current = L->x;
+  double* next = L->t;
+  for (int stage = 0; stage < stages; ++stage) {
+    const double rho = stage == 0 ? 0.0 : 1.0 / (2.0 * sigma - previous_rho);
+    const double previous_factor = stage == 0 ? 0.0 : rho * previous_rho;
+    const double residual_factor = stage == 0 ? 1.0 / theta : 2.0 * rho / delta;
+    if (stage != 0) previous_rho = rho;
+    for (int k = 0; k < n[2]; ++k)
+      for (int j = 0; j < n[1]; ++j)
+        for (int i = 0; i < n[0]; ++i) {
+          const size_t c = cidx(n, i, j, k);
+          const double value = row_value(n, mg->bc, L->D, L->fx, L->fy, L->fz,
+                                         current, i, j, k);
+          const double defect = L->b[c] - value;
+          const double scaled = residual_factor * defect / L->D[c];
+          const double direction =
+              stage == 0 ? scaled : previous_factor * L->r[c] + scaled;
+          L->r[c] = direction;
+          next[c] = current[c] + direction;
+        }
+    double* swap = current;
+    current = next;
+    next = swap;
+  }
+  if (current != L->x) memcpy(L->x, current, L->cells * sizeof(double));
+  if (retain) level_defect(mg, L, L->x, L->b, L->r);
+}
+
+/* smooth(), src/solver_mg.cpp:3557-3590: point levels use red-black (or
+ * Chebyshev-Jacobi, :3240-3244), line levels relax each line axis in x, y,
+ * z order (reverse: z, y, x). */
 static void smooth(or_mg* mg, int level, int sweeps, int reverse, int retain) {
   const int mask = mg->level[level].line_mask;
+  if (mg->level[level].chebyshev) {
+    chebyshev_smooth(mg, level, sweeps, retain);
+    return;
+  }
   if (mask == 0) {
     point_smooth(mg, level, sweeps, reverse, retain);
     return;
@@ -931,8 +1022,13 @@ int or_mg_apply(or_mg* mg, const double* residual, double* correction) {
     if (st != ST_OK) return st;
   }
   mg->line_failed = 0;
-  st = mg_cycle(mg, 0, mg->policy.f_cycle ? 2 : 1);
-  if (st != ST_OK) return st;
+  if (mg->shortcut) {
+    /* src/solver_mg.cpp:4926-4936 */
+    for (size_t c = 0; c < L->cells; ++c) L->x[c] = L->b[c] / L->D[c];
+  } else {
+    st = mg_cycle(mg, 0, mg->policy.f_cycle ? 2 : 1);
+    if (st != ST_OK) return st;
+  }
   if (mg->line_failed) return ST_NUMERICAL;
   if (mg->policy.null_space_constant) {
     st = project_constant(L, L->x);
@@ -1203,4 +1299,463 @@ int or_pcg_dense(int n, const double* a, const double* m, const double* rhs,
   ops.trace = trace;
   ops.trace_capacity = trace_capacity;
   return pcg_core(&ops, rhs, solution, ctl, res);
+}
+
+/* ------------------------------------------------------------------ */
+/* FGMRES, src/solver_krylov.cpp:1557-2650 (single rank, no recycle, no */
+/* convergence audit).                                                  */
+/* ------------------------------------------------------------------ */
+/* kSingleReductionFgmresShift, src/solver_krylov.cpp:35 */
+#define OR_FGMRES_SHIFT 1.0
+
+/* single_reduction_fgmres_norm, src/solver_krylov.cpp:565-604: returns 0 ok,
+ * 1 unsafe recurrence (t < 0), -1 non-finite. */
+static int fgmres_norm(const double* h, size_t count, double bar, double* t,
+                       double* next) {
+  if (!isfinite(bar)) return -1;
+  *t = bar;
+  for (size_t row = 0; row < count; ++row) {
+    const double projection = h[row];
+    if (!isfinite(projection)) return -1;
+    const double square = projection * projection;
+    if (!isfinite(square)) return -1;
+    *t = *t - square;
+    if (!isfinite(*t)) return -1;
+  }
+  if (*t < 0.0) {
+    *next = 0.0;
+    return 1;
+  }
+  *next = sqrt(*t);
+  return isfinite(*next) ? 0 : -1;
+}
+
+/* add_scaled_basis, src/solver_krylov.cpp:510-555: destination += sum of
+ * scales[i] * sources[i], added in row order per element. */
+static void add_scaled_basis(double* dst, double* const* src,
+                             const double* scales, size_t count, size_t n) {
+  for (size_t e = 0; e < n; ++e) {
+    double v = dst[e];
+    for (size_t row = 0; row < count; ++row) v = v + scales[row] * src[row][e];
+    dst[e] = v;
+  }
+}
+
+/* Upper-triangular back substitution of the rotated Hessenberg columns
+ * (column-major, `rows` per column). */
+static int fgmres_back_substitute(const double* h, size_t rows,
+                                  const double* g, size_t count, double* y) {
+  for (size_t reverse = count; reverse > 0; --reverse) {
+    const size_t row = reverse - 1;
+    double value = g[row];
+    for (size_t upper = row + 1; upper < count; ++upper)
+      value -= h[upper * rows + row] * y[upper];
+    const double divisor = h[row * rows + row];
+    if (divisor == 0.0 || !isfinite(divisor)) return TERM_BREAKDOWN;
+    y[row] = value / divisor;
+    if (!isfinite(y[row])) return TERM_NON_FINITE;
+  }
+  return 0;
+}
+
+static int fgmres_core(const or_krylov_ops* ops, const double* rhs,
+                       double* solution, const or_fgmres_control* ctl,
+                       or_result* res) {
+  const size_t N = ops->n;
+  const size_t R = ctl->restart;
+  memset(res, 0, sizeof *res);
+  if (R == 0 || R > 64 || ctl->maximum_iterations == 0 ||
+      ctl->true_residual_interval == 0) {
+    res->status = ST_INVALID_PLAN;
+    res->termination = 8;
+    return ST_INVALID_PLAN;
+  }
+  const size_t rows = R + 1;
+  double* x = (double*)malloc(N * sizeof(double));
+  double* ax = (double*)malloc(N * sizeof(double));
+  double** v = (double**)malloc((R + 1) * sizeof(double*));
+  double** z = (double**)malloc(R * sizeof(double*));
+  for (size_t i = 0; i <= R; ++i) v[i] = (double*)malloc(N * sizeof(double));
+  for (size_t i = 0; i < R; ++i) z[i] = (double*)malloc(N * sizeof(double));
+  double* h = (double*)calloc(rows * R, sizeof(double));
+  double* cs = (double*)calloc(R, sizeof(double));
+  double* sn = (double*)calloc(R, sizeof(double));
+  double* g = (double*)calloc(rows, sizeof(double));
+  double* y = (double*)calloc(R, sizeof(double));
+  double* scales = (double*)calloc(R + 1, sizeof(double));
+  double restarts = 0.0;
+  int st = ST_OK;
+  const double eps = 2.220446049250313080847e-16;
+#define FG_FAIL(code, term) \
+  do {                      \
+    st = (code);            \
+    res->termination = (term); \
+    goto done;              \
+  } while (0)
+#define FG_TRUE_RESIDUAL(vec, out, norm)                                  \
+  do {                                                                    \
+    ops->apply_a(ops->a_ctx, (vec), ax);                                  \
+    res->operator_applies++;                                              \
+    for (size_t e = 0; e < N; ++e) (out)[e] = 1.0 * rhs[e] + -1.0 * ax[e]; \
+    (norm) = or_norm2((out), N);                                          \
+    if (!isfinite(norm)) FG_FAIL(ST_NUMERICAL, 5);                        \
+  } while (0)
+  const double rhs_norm = or_norm2(rhs, N);
+  res->rhs_norm = rhs_norm;
+  if (!isfinite(rhs_norm)) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+  const double tol = fmax(ctl->absolute_tolerance,
+                          ctl->relative_tolerance * rhs_norm);
+  if (rhs_norm == 0.0) {
+    memset(solution, 0, N * sizeof(double));
+    res->termination = TERM_ZERO_RHS;
+    goto done;
+  }
+  memcpy(x, solution, N * sizeof(double));
+  FG_TRUE_RESIDUAL(x, v[0], res->initial_true_residual);
+  res->final_true_residual = res->initial_true_residual;
+  if (!isfinite(tol)) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+  if (res->initial_true_residual <= tol) {
+    memcpy(solution, x, N * sizeof(double));
+    res->termination = TERM_CONVERGED;
+    goto done;
+  }
+  while (res->iterations < ctl->maximum_iterations) {
+    const double beta = res->final_true_residual;
+    const double inv = 1.0 / beta;
+    for (size_t e = 0; e < N; ++e) {
+      v[0][e] = inv * v[0][e];
+      if (!isfinite(v[0][e])) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+    }
+    for (size_t i = 0; i < rows; ++i) g[i] = 0.0;
+    g[0] = beta;
+    int restarted = 0;
+    for (size_t column = 0;
+         column < R && res->iterations < ctl->maximum_iterations; ++column) {
+      if (ops->apply_m != NULL) {
+        const int pst = ops->apply_m(ops->m_ctx, v[column], z[column]);
+        res->preconditioner_applies++;
+        if (pst != ST_OK) FG_FAIL(pst, 6);
+      } else {
+        memcpy(z[column], v[column], N * sizeof(double));
+        res->preconditioner_applies++;
+      }
+      ops->apply_a(ops->a_ctx, z[column], ax);
+      res->operator_applies++;
+      for (size_t e = 0; e < N; ++e) ax[e] = ax[e] + -OR_FGMRES_SHIFT * v[column][e];
+      const size_t dots = column + 1;
+      double* hc = h + column * rows;
+      for (size_t row = 0; row < dots; ++row) hc[row] = or_dot(v[row], ax, N);
+      hc[dots] = or_dot(ax, ax, N);
+      for (size_t row = 0; row <= dots; ++row)
+        if (!isfinite(hc[row])) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+      const double bar = hc[dots];
+      for (size_t row = 0; row < dots; ++row) scales[row] = -hc[row];
+      double t = 0.0, next_norm = 0.0;
+      const int nst = fgmres_norm(hc, dots, bar, &t, &next_norm);
+      if (nst < 0) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+      const int unsafe = nst == 1;
+      if (unsafe && column != 0) {
+        /* discard the column, publish the shorter cycle, restart
+         * (src/solver_krylov.cpp:2232-2340) */
+        res->iterations++;
+        const int term = fgmres_back_substitute(h, rows, g, column, y);
+        if (term) FG_FAIL(ST_NUMERICAL, term);
+        add_scaled_basis(x, z, y, column, N);
+        FG_TRUE_RESIDUAL(x, v[0], res->final_true_residual);
+        if (res->final_true_residual <= tol) {
+          memcpy(solution, x, N * sizeof(double));
+          res->termination = TERM_CONVERGED;
+          goto done;
+        }
+        if (res->iterations == ctl->maximum_iterations)
+          FG_FAIL(ST_REJECTED, TERM_MAXIMUM_ITERATIONS);
+        restarts += 1.0;
+        memset(h, 0, rows * R * sizeof(double));
+        memset(cs, 0, R * sizeof(double));
+        memset(sn, 0, R * sizeof(double));
+        memset(g, 0, rows * sizeof(double));
+        memset(y, 0, R * sizeof(double));
+        restarted = 1;
+        break;
+      }
+      int explicit_pass = 0;
+      if (unsafe || next_norm == 0.0) {
+        /* second orthogonalisation pass (src/solver_krylov.cpp:2342-2400) */
+        add_scaled_basis(ax, v, scales, dots, N);
+        for (size_t row = 0; row < dots; ++row) y[row] = or_dot(v[row], ax, N);
+        for (size_t row = 0; row < dots; ++row) {
+          hc[row] += y[row];
+          if (!isfinite(hc[row])) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+          scales[row] = -y[row];
+        }
+        add_scaled_basis(ax, v, scales, dots, N);
+        next_norm = or_norm2(ax, N);
+        if (!isfinite(next_norm)) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+        explicit_pass = 1;
+      }
+      hc[column] += OR_FGMRES_SHIFT;
+      if (!isfinite(hc[column])) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+      hc[column + 1] = next_norm;
+      for (size_t row = 0; row < column; ++row) {
+        const double upper = hc[row], lower = hc[row + 1];
+        const double rotated = cs[row] * upper + sn[row] * lower;
+        hc[row + 1] = -sn[row] * upper + cs[row] * lower;
+        hc[row] = rotated;
+      }
+      const double rn = hypot(hc[column], hc[column + 1]);
+      if (!(rn > 0.0) || !isfinite(rn)) FG_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
+      cs[column] = hc[column] / rn;
+      sn[column] = hc[column + 1] / rn;
+      hc[column] = rn;
+      hc[column + 1] = 0.0;
+      g[column + 1] = -sn[column] * g[column];
+      g[column] = cs[column] * g[column];
+      const double recursive = fabs(g[column + 1]);
+      res->iterations++;
+      const int happy = next_norm == 0.0;
+      const int cycle_end = happy || column + 1 == R ||
+                            res->iterations == ctl->maximum_iterations;
+      const int verify = happy || recursive <= tol ||
+                         res->iterations % ctl->true_residual_interval == 0 ||
+                         cycle_end;
+      if (!cycle_end) {
+        if (!explicit_pass) add_scaled_basis(ax, v, scales, dots, N);
+        const double inv_norm = 1.0 / next_norm;
+        for (size_t e = 0; e < N; ++e) {
+          v[column + 1][e] = inv_norm * ax[e];
+          if (!isfinite(v[column + 1][e])) FG_FAIL(ST_NUMERICAL, TERM_NON_FINITE);
+        }
+      }
+      if (ops->trace != NULL && res->iterations <= ops->trace_capacity &&
+          restarts == 0.0 && res->iterations == column + 1) {
+        double* tr = ops->trace + (size_t)(res->iterations - 1) * N;
+        double* yt = (double*)calloc(dots, sizeof(double));
+        if (fgmres_back_substitute(h, rows, g, dots, yt) == 0) {
+          memcpy(tr, x, N * sizeof(double));
+          add_scaled_basis(tr, z, yt, dots, N);
+        }
+        free(yt);
+      }
+      if (!verify) continue;
+      const int term = fgmres_back_substitute(h, rows, g, dots, y);
+      if (term) FG_FAIL(ST_NUMERICAL, term);
+      if (!cycle_end) {
+        /* audit a trial solution in the unused last basis slot */
+        double* trial = v[R];
+        memcpy(trial, x, N * sizeof(double));
+        add_scaled_basis(trial, z, y, dots, N);
+        double audited = 0.0;
+        FG_TRUE_RESIDUAL(trial, ax, audited);
+        res->final_true_residual = audited;
+        if (audited <= tol) {
+          memcpy(solution, trial, N * sizeof(double));
+          res->termination = TERM_CONVERGED;
+          goto done;
+        }
+        continue;
+      }
+      add_scaled_basis(x, z, y, dots, N);
+      FG_TRUE_RESIDUAL(x, v[0], res->final_true_residual);
+      if (res->final_true_residual <= tol) {
+        memcpy(solution, x, N * sizeof(double));
+        res->termination = TERM_CONVERGED;
+        goto done;
+      }
+      if (res->iterations == ctl->maximum_iterations)
+        FG_FAIL(ST_REJECTED, TERM_MAXIMUM_ITERATIONS);
+      if (happy) {
+        const double guard = 64.0 * eps * fmax(beta, tol);
+        if (!(res->final_true_residual + guard < beta))
+          FG_FAIL(ST_NUMERICAL, TERM_BREAKDOWN);
+        restarts += 1.0;
+      }
+      restarted = 1;
+      break;
+    }
+    if (!restarted) break;
+  }
+  FG_FAIL(ST_REJECTED, TERM_MAXIMUM_ITERATIONS);
+done:
+  res->status = st;
+  res->breakdown_value = restarts;
+  free(x);
+  free(ax);
+  for (size_t i = 0; i <= R; ++i) free(v[i]);
+  for (size_t i = 0; i < R; ++i) free(z[i]);
+  free(v);
+  free(z);
+  free(h);
+  free(cs);
+  free(sn);
+  free(g);
+  free(y);
+  free(scales);
+  return st;
+#undef FG_FAIL
+#undef FG_TRUE_RESIDUAL
+}
+
+/* Operator rows per unit volume and the volume-scaled preconditioner
+ * (VolumeScaledPreconditioner, src/solver_cold.hpp:378-415): the rows are
+ * the integrated operator divided by the cell volume (w_x w_y) w_z. */
+typedef struct volume_ctx {
+  ibm_operator_ctx a;
+  or_mg* mg;
+  const double *wx, *wy, *wz;
+  double* scratch;
+} volume_ctx;
+
+static double cell_volume(const volume_ctx* v, size_t c) {
+  const int* n = v->a.n;
+  const size_t i = c % (size_t)n[0];
+  const size_t j = (c / (size_t)n[0]) % (size_t)n[1];
+  const size_t k = c / ((size_t)n[0] * (size_t)n[1]);
+  return v->wx[i] * v->wy[j] * v->wz[k];
+}
+
+static void volume_operator_cb(void* ctx, const double* x, double* y) {
+  const volume_ctx* v = (const volume_ctx*)ctx;
+  ibm_operator_cb((void*)&v->a, x, y);
+  const size_t N = (size_t)v->a.n[0] * v->a.n[1] * v->a.n[2];
+  for (size_t c = 0; c < N; ++c) y[c] = y[c] / cell_volume(v, c);
+}
+
+static int volume_precondition_cb(void* ctx, const double* r, double* z) {
+  volume_ctx* v = (volume_ctx*)ctx;
+  const size_t N = (size_t)v->a.n[0] * v->a.n[1] * v->a.n[2];
+  for (size_t c = 0; c < N; ++c) v->scratch[c] = r[c] * cell_volume(v, c);
+  if (v->mg == NULL) {
+    memcpy(z, v->scratch, N * sizeof(double));
+    return ST_OK;
+  }
+  return or_mg_apply(v->mg, v->scratch, z);
+}
+
+int or_fgmres_solve(const int n[3], const double* wx, const double* wy,
+                    const double* wz, const int bc[6], const double* diag,
+                    const double* bx, const double* by, const double* bz,
+                    const uint8_t* act, or_mg* mg, const double* rhs,
+                    double* solution, const or_fgmres_control* ctl,
+                    or_result* res) {
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  volume_ctx v;
+  memset(&v, 0, sizeof v);
+  v.a.n = n;
+  v.a.bc = bc;
+  v.a.diag = diag;
+  v.a.bx = bx;
+  v.a.by = by;
+  v.a.bz = bz;
+  v.a.act = act;
+  v.mg = mg;
+  v.wx = wx;
+  v.wy = wy;
+  v.wz = wz;
+  or_krylov_ops ops;
+  memset(&ops, 0, sizeof ops);
+  ops.n = N;
+  if (ctl->volume_rows) {
+    v.scratch = (double*)malloc(N * sizeof(double));
+    ops.apply_a = volume_operator_cb;
+    ops.a_ctx = &v;
+    ops.apply_m = volume_precondition_cb;
+    ops.m_ctx = &v;
+  } else {
+    ops.apply_a = ibm_operator_cb;
+    ops.a_ctx = &v.a;
+    if (mg != NULL) {
+      ops.apply_m = mg_precondition_cb;
+      ops.m_ctx = mg;
+    }
+  }
+  const int st = fgmres_core(&ops, rhs, solution, ctl, res);
+  free(v.scratch);
+  return st;
+}
+
+int or_fgmres_dense(int n, const double* a, const double* m,
+                    const double* rhs, double* solution,
+                    const or_fgmres_control* ctl, or_result* res,
+                    double* trace, uint32_t trace_capacity) {
+  dense_ctx ac = {(size_t)n, a};
+  dense_ctx mc = {(size_t)n, m};
+  or_krylov_ops ops;
+  memset(&ops, 0, sizeof ops);
+  ops.n = (size_t)n;
+  ops.apply_a = dense_apply_cb;
+  ops.a_ctx = &ac;
+  if (m != NULL) {
+    ops.apply_m = dense_precondition_cb;
+    ops.m_ctx = &mc;
+  }
+  ops.trace = trace;
+  ops.trace_capacity = trace_capacity;
+  return fgmres_core(&ops, rhs, solution, ctl, res);
+}
+
+/* ------------------------------------------------------------------ */
+/* Pressure-correction system, src/solver_pressure.cpp:110-332          */
+/* (assemble_pressure_system_impl) with face_sum / outward_flux of      */
+/* :80-94: D = a0 V drho/dp + sum of the six face coefficients,         */
+/* b = -(V (a0 rho + a1 rho_n + a2 rho_nm1 - S) + outward flux).        */
+/* ------------------------------------------------------------------ */
+int or_assemble_pressure_system(const int n[3], const double* wx,
+                                const double* wy, const double* wz,
+                                const double* bx, const double* by,
+                                const double* bz, const double* phix,
+                                const double* phiy, const double* phiz,
+                                const double* rho, const double* rho_n,
+                                const double* rho_nm1, const double* drho_dp,
+                                const double* source, double a0, double a1,
+                                double a2, double* diag, double* rhs) {
+  const int bdf2 = rho_nm1 != NULL;
+  if (!(a0 > 0.0) || !isfinite(a0) || !isfinite(a1) || !isfinite(a2))
+    return ST_INVALID_PLAN;
+  const size_t nfx = (size_t)(n[0] + 1) * n[1] * n[2];
+  const size_t nfy = (size_t)n[0] * (n[1] + 1) * n[2];
+  const size_t nfz = (size_t)n[0] * n[1] * (n[2] + 1);
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  /* input finiteness (finite_field_box / finite_nonnegative_faces) */
+  for (size_t c = 0; c < N; ++c)
+    if (!isfinite(rho[c]) || !isfinite(rho_n[c]) ||
+        (bdf2 && !isfinite(rho_nm1[c])) || !isfinite(drho_dp[c]) ||
+        (source != NULL && !isfinite(source[c])))
+      return ST_NUMERICAL;
+  for (size_t f = 0; f < nfx; ++f)
+    if (!isfinite(bx[f]) || bx[f] < 0.0 || !isfinite(phix[f])) return ST_NUMERICAL;
+  for (size_t f = 0; f < nfy; ++f)
+    if (!isfinite(by[f]) || by[f] < 0.0 || !isfinite(phiy[f])) return ST_NUMERICAL;
+  for (size_t f = 0; f < nfz; ++f)
+    if (!isfinite(bz[f]) || bz[f] < 0.0 || !isfinite(phiz[f])) return ST_NUMERICAL;
+  /* two passes: the preflight publishes nothing on failure */
+  for (int pass = 0; pass < 2; ++pass)
+    for (int k = 0; k < n[2]; ++k)
+      for (int j = 0; j < n[1]; ++j)
+        for (int i = 0; i < n[0]; ++i) {
+          const size_t c = cidx(n, i, j, k);
+          const double volume = wx[i] * wy[j] * wz[k];
+          const double r = rho[c], rn = rho_n[c];
+          const double rnm1 = bdf2 ? rho_nm1[c] : 0.0;
+          const double s = source != NULL ? source[c] : 0.0;
+          const double defect = volume * (a0 * r + a1 * rn + a2 * rnm1 - s);
+          const double faces =
+              bx[xfidx(n, i, j, k)] + bx[xfidx(n, i + 1, j, k)] +
+              by[yfidx(n, i, j, k)] + by[yfidx(n, i, j + 1, k)] +
+              bz[zfidx(n, i, j, k)] + bz[zfidx(n, i, j, k + 1)];
+          const double d = a0 * volume * drho_dp[c] + faces;
+          const double flux = phix[xfidx(n, i + 1, j, k)] - phix[xfidx(n, i, j, k)] +
+                              phiy[yfidx(n, i, j + 1, k)] - phiy[yfidx(n, i, j, k)] +
+                              phiz[zfidx(n, i, j, k + 1)] - phiz[zfidx(n, i, j, k)];
+          const double b = -(defect + flux);
+          if (pass == 0) {
+            if (!(r > 0.0) || !(rn > 0.0) || (bdf2 && !(rnm1 > 0.0)) ||
+                !(drho_dp[c] > 0.0) || !(volume > 0.0) || !(d > 0.0) ||
+                !isfinite(defect) || !isfinite(d) || !isfinite(b))
+              return ST_NUMERICAL;
+          } else {
+            diag[c] = d;
+            rhs[c] = b;
+          }
+        }
+  return ST_OK;
 }

@@ -46,7 +46,23 @@ typedef struct or_policy {
                                    lines (the reference); 1 zebra: lines with
                                    (a+b) even, then odd (reverse: odd, even),
                                    periodic ghosts refreshed per colour pass */
+  int point_smoother;           /* 0 red-black, 1 Chebyshev-Jacobi
+                                   (MgPointSmootherKind) */
+  double chebyshev_lower_fraction;  /* 0.3 */
+  double diagonal_shortcut_ratio;   /* 0 off; else D^-1 when every finest
+                                       row has face sum / D <= this */
 } or_policy;
+
+typedef struct or_fgmres_control {
+  double absolute_tolerance;
+  double relative_tolerance;
+  uint32_t maximum_iterations;
+  uint32_t true_residual_interval;
+  uint32_t restart;
+  /* 1: operator rows per unit volume (V^-1 A) with the multigrid applied to
+   * V r (VolumeScaledPreconditioner) */
+  int volume_rows;
+} or_fgmres_control;
 
 typedef struct or_control {
   double absolute_tolerance;
@@ -126,6 +142,46 @@ int or_pcg_solve(const int n[3], const int bc[6], const double* diag,
 int or_pcg_dense(int n, const double* a, const double* m, const double* rhs,
                  double* solution, const or_control* control,
                  or_result* result, double* trace, uint32_t trace_capacity);
+
+/* solve_fgmres restatement (single-reduction shifted Arnoldi, restarted),
+ * src/solver_krylov.cpp:1557-2650, without the recycle and convergence-audit
+ * options.  Operator: the exact IBM operator (rows divided by the cell
+ * volume with volume_rows); preconditioner: the MG plan (NULL: identity),
+ * applied to V r with volume_rows.  norm_breakdown_restarts is reported in
+ * result->breakdown_value. */
+int or_fgmres_solve(const int n[3], const double* widths_x,
+                    const double* widths_y, const double* widths_z,
+                    const int bc[6], const double* diag, const double* bx,
+                    const double* by, const double* bz,
+                    const uint8_t* cell_active, or_mg* mg, const double* rhs,
+                    double* solution, const or_fgmres_control* control,
+                    or_result* result);
+
+/* The same FGMRES over dense row-major matrices (test form); trace receives
+ * the candidate x0 + Z y after each of the first trace_capacity iterations
+ * of the first Arnoldi cycle. */
+int or_fgmres_dense(int n, const double* a, const double* m,
+                    const double* rhs, double* solution,
+                    const or_fgmres_control* control, or_result* result,
+                    double* trace, uint32_t trace_capacity);
+
+/* 1 when the plan selected the diagonal shortcut (z = D^-1 r). */
+int or_mg_diagonal_shortcut(const or_mg* mg);
+
+/* Pressure-correction system (src/solver_pressure.cpp:110-332):
+ * D = a0 V drho_dp + face sum, b = -(V (a0 rho + a1 rho_n + a2 rho_nm1 - S)
+ * + outward flux of phi).  rho_nm1 NULL selects BDF1 (a2 unused), source
+ * NULL no mass source.  Nothing is written unless every input and result
+ * passes the reference's checks. */
+int or_assemble_pressure_system(const int n[3], const double* widths_x,
+                                const double* widths_y, const double* widths_z,
+                                const double* bx, const double* by,
+                                const double* bz, const double* phi_x,
+                                const double* phi_y, const double* phi_z,
+                                const double* rho, const double* rho_n,
+                                const double* rho_nm1, const double* drho_dp,
+                                const double* source, double a0, double a1,
+                                double a2, double* diag, double* rhs);
 
 /* Vector kernels, src/solver_krylov.cpp:176-204 and :650-711. */
 double or_dot(const double* a, const double* b, size_t n);

@@ -24,7 +24,19 @@ class or_policy(ctypes.Structure):
                 ("null_space_constant", ctypes.c_int),
                 ("unit_linear", ctypes.c_int),
                 ("coarse_face_scaling", ctypes.c_int),
-                ("line_order", ctypes.c_int)]
+                ("line_order", ctypes.c_int),
+                ("point_smoother", ctypes.c_int),
+                ("chebyshev_lower_fraction", ctypes.c_double),
+                ("diagonal_shortcut_ratio", ctypes.c_double)]
+
+
+class or_fgmres_control(ctypes.Structure):
+    _fields_ = [("absolute_tolerance", ctypes.c_double),
+                ("relative_tolerance", ctypes.c_double),
+                ("maximum_iterations", ctypes.c_uint32),
+                ("true_residual_interval", ctypes.c_uint32),
+                ("restart", ctypes.c_uint32),
+                ("volume_rows", ctypes.c_int)]
 
 
 class or_control(ctypes.Structure):
@@ -96,6 +108,21 @@ def lib():
                                    ctypes.POINTER(or_result), _D,
                                    ctypes.c_uint32]
         L.or_pcg_dense.restype = ctypes.c_int
+        L.or_fgmres_solve.argtypes = [_I, _D, _D, _D, _I, _D, _D, _D, _D,
+                                      _U8, ctypes.c_void_p, _D, _D,
+                                      ctypes.POINTER(or_fgmres_control),
+                                      ctypes.POINTER(or_result)]
+        L.or_fgmres_solve.restype = ctypes.c_int
+        L.or_fgmres_dense.argtypes = [ctypes.c_int, _D, _D, _D, _D,
+                                      ctypes.POINTER(or_fgmres_control),
+                                      ctypes.POINTER(or_result), _D,
+                                      ctypes.c_uint32]
+        L.or_fgmres_dense.restype = ctypes.c_int
+        L.or_assemble_pressure_system.argtypes = (
+            [_I] + [_D] * 14 + [ctypes.c_double] * 3 + [_D, _D])
+        L.or_assemble_pressure_system.restype = ctypes.c_int
+        L.or_mg_diagonal_shortcut.argtypes = [ctypes.c_void_p]
+        L.or_mg_diagonal_shortcut.restype = ctypes.c_int
         L.or_dot.argtypes = [_D, _D, ctypes.c_size_t]
         L.or_dot.restype = ctypes.c_double
         L.or_norm2.argtypes = [_D, ctypes.c_size_t]
@@ -152,7 +179,9 @@ def policy(**kw) -> or_policy:
     p = or_policy(anisotropy_threshold=4.0, pre_sweeps=2, post_sweeps=2,
                   maximum_levels=32, coarse_sweeps=24,
                   minimum_coarse_extent=3, f_cycle=0, null_space_constant=1,
-                  unit_linear=1, coarse_face_scaling=0, line_order=0)
+                  unit_linear=1, coarse_face_scaling=0, line_order=0,
+                  point_smoother=0, chebyshev_lower_fraction=0.3,
+                  diagonal_shortcut_ratio=0.0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -220,6 +249,9 @@ class MG:
         ptr = lib().or_mg_level_array(self.h, level, which)
         return np.ctypeslib.as_array(ptr, shape=(counts[which],)).copy()
 
+    def diagonal_shortcut(self):
+        return bool(lib().or_mg_diagonal_shortcut(self.h))
+
     def smooth(self, level, sweeps, reverse=0, retain=0):
         st = lib().or_mg_smooth(self.h, level, sweeps, reverse, retain)
         if st != 0:
@@ -265,6 +297,80 @@ def pcg_dense(a, m, rhs, x0=None, rtol=1e-12, atol=0.0,
     lib().or_pcg_dense(n, _d(a), None if mm is None else _d(mm), _d(rhs),
                        _d(x), ctypes.byref(ctl), ctypes.byref(res), _d(tr),
                        trace)
+    return x, res, tr[:min(trace, res.iterations)]
+
+
+def assemble_pressure_system(n, faces, bx, by, bz, phi, rho, rho_n,
+                             drho_dp, a, rho_nm1=None, source=None):
+    """(status, D, b) of the pressure-correction system; phi = (phi_x,
+    phi_y, phi_z) face fluxes, a = (a0, a1, a2) BDF coefficients."""
+    cells = n[0] * n[1] * n[2]
+    w = [np.ascontiguousarray(np.diff(f), dtype=np.float64) for f in faces]
+    diag = np.full(cells, np.nan)
+    rhs = np.full(cells, np.nan)
+    c = lambda v: None if v is None else _d(np.ascontiguousarray(
+        v, dtype=np.float64))
+    keep = [np.ascontiguousarray(v, dtype=np.float64) for v in
+            (bx, by, bz, phi[0], phi[1], phi[2], rho, rho_n, drho_dp)]
+    nm1 = None if rho_nm1 is None else np.ascontiguousarray(rho_nm1,
+                                                            dtype=np.float64)
+    src = None if source is None else np.ascontiguousarray(source,
+                                                           dtype=np.float64)
+    pn, _k1 = _i(n)
+    st = lib().or_assemble_pressure_system(
+        pn, _d(w[0]), _d(w[1]), _d(w[2]), *[_d(k) for k in keep[:8]],
+        c(nm1), _d(keep[8]), c(src), float(a[0]), float(a[1]), float(a[2]),
+        _d(diag), _d(rhs))
+    return st, diag, rhs
+
+
+def production_policy(**kw) -> or_policy:
+    """The reference's production pressure policy
+    (src/solver_piso_detail.hpp:10-19): Chebyshev-Jacobi F(1,2) cycle,
+    lower spectrum fraction 0.3, diagonal shortcut at ratio 0.1; no null
+    space (the compressible pressure rows carry a diagonal reaction)."""
+    base = dict(pre_sweeps=1, post_sweeps=2, point_smoother=1, f_cycle=1,
+                chebyshev_lower_fraction=0.3, diagonal_shortcut_ratio=0.1,
+                null_space_constant=0)
+    base.update(kw)
+    return policy(**base)
+
+
+def fgmres(n, faces, bc, diag, bx, by, bz, cell_active, mg, rhs, x0=None,
+           rtol=1e-10, atol=0.0, maximum_iterations=400,
+           true_residual_interval=4, restart=12, volume_rows=False):
+    cells = n[0] * n[1] * n[2]
+    x = np.zeros(cells) if x0 is None else np.array(x0, dtype=np.float64)
+    ctl = or_fgmres_control(atol, rtol, maximum_iterations,
+                            true_residual_interval, restart,
+                            1 if volume_rows else 0)
+    res = or_result()
+    w = [np.ascontiguousarray(np.diff(f), dtype=np.float64) for f in faces]
+    pn, _k1 = _i(n)
+    pb, _k2 = _i(bc)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().or_fgmres_solve(pn, _d(w[0]), _d(w[1]), _d(w[2]), pb, _d(diag),
+                          _d(bx), _d(by), _d(bz), _u8(cell_active),
+                          None if mg is None else mg.h, _d(rhs), _d(x),
+                          ctypes.byref(ctl), ctypes.byref(res))
+    return x, res
+
+
+def fgmres_dense(a, m, rhs, x0=None, rtol=1e-12, atol=0.0,
+                 maximum_iterations=200, true_residual_interval=1000,
+                 restart=12, trace=0):
+    n = a.shape[0]
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    mm = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    ctl = or_fgmres_control(atol, rtol, maximum_iterations,
+                            true_residual_interval, restart, 0)
+    res = or_result()
+    tr = np.zeros((max(trace, 1), n))
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().or_fgmres_dense(n, _d(a), None if mm is None else _d(mm), _d(rhs),
+                          _d(x), ctypes.byref(ctl), ctypes.byref(res),
+                          _d(tr), trace)
     return x, res, tr[:min(trace, res.iterations)]
 
 

@@ -5,8 +5,12 @@
 // in the product.  It reads a case file written by tests/golden/make_golden.py,
 // compiles the reference NativeCartesianMgPlan on MPI_COMM_SELF, and writes:
 // the level plan, every level's diagonal, one preconditioner application with
-// all level-0 workspace slots, and a solve_pcg solution (the MG wrapped as a
-// per-call fixed_spd preconditioner, which the reference's PCG requires).
+// all level-0 workspace slots, and one Krylov solve: solve_pcg (the MG
+// wrapped as a per-call fixed_spd preconditioner, which the reference's PCG
+// requires) or solve_fgmres (the MG as given, or behind a volume-scaling
+// wrapper with per-volume operator rows, the cold pressure pairing).  With a
+// repeat count in the case file it also times the solve (wall clock of each
+// repetition), for the CPU baseline.
 //
 // Reference interfaces used: include/hundun/v04_linear.hpp (MG plan, solve_pcg,
 // workspaces, reduction engine), include/hundun/v04_parallel.hpp (HaloEngine),
@@ -200,6 +204,70 @@ class PcgAdapter final : public LinearPreconditioner {
   }
 };
 
+// Operator rows per unit volume: the harness operator divided by the cell
+// volume (w_x w_y) w_z (the cold pressure rows, src/solver_cold.hpp:417-470,
+// are volume-normalised).
+class VolumeRowsOperator final : public LinearOperator {
+ public:
+  const HarnessOperator* base{};
+  const std::vector<double>* volume{};
+  LinearOperatorCertificate certificate() const noexcept override {
+    LinearOperatorCertificate c = base->certificate();
+    c.operator_class = LinearOperatorClass::nonsymmetric;
+    return c;
+  }
+  Status apply(FieldView x, FieldView y) const noexcept override {
+    const Status st = base->apply(x, y);
+    if (!st) return st;
+    const Int3 n = base->n;
+    for (int k = 0; k < n.z; ++k)
+      for (int j = 0; j < n.y; ++j)
+        for (int i = 0; i < n.x; ++i)
+          y.unchecked({i, j, k}, 0) =
+              y.unchecked({i, j, k}, 0) / (*volume)[cidx(n, i, j, k)];
+    return {};
+  }
+};
+
+// The MG applied to V r: the arithmetic of VolumeScaledPreconditioner
+// (src/solver_cold.hpp:378-415), whose CartesianKernelPlan lies outside the
+// compiled closure.
+class VolumeScaledAdapter final : public LinearPreconditioner {
+ public:
+  NativeCartesianMgPlan* plan{};
+  const std::vector<double>* volume{};
+  FieldView scratch{};
+  Int3 n{};
+  LinearPreconditionerCertificate certificate() const noexcept override {
+    return plan->certificate();
+  }
+  Status prepare_batch(const LinearPreconditionerBatchDescriptor& descriptor,
+                       LinearPreconditionerBatchTicket& ticket) noexcept override {
+    return plan->prepare_batch(descriptor, ticket);
+  }
+  Status apply_prepared(ConstFieldView residual, FieldView correction,
+                        std::uint32_t iteration,
+                        const LinearPreconditionerBatchTicket& ticket) noexcept override {
+    scale(residual);
+    return plan->apply_prepared(as_const(scratch), correction, iteration,
+                                ticket);
+  }
+  Status apply(ConstFieldView residual, FieldView correction,
+               std::uint32_t iteration) noexcept override {
+    scale(residual);
+    return plan->apply(as_const(scratch), correction, iteration);
+  }
+
+ private:
+  void scale(ConstFieldView residual) {
+    for (int k = 0; k < n.z; ++k)
+      for (int j = 0; j < n.y; ++j)
+        for (int i = 0; i < n.x; ++i)
+          scratch.unchecked({i, j, k}, 0) =
+              residual.unchecked({i, j, k}, 0) * (*volume)[cidx(n, i, j, k)];
+  }
+};
+
 std::uint8_t mask_of(CoarseningKind k) {
   switch (k) {
     case CoarseningKind::full_xyz: return 7;
@@ -232,7 +300,7 @@ int main(int argc, char** argv) {
     Reader in(argv[1]);
     char magic[8];
     in.in.read(magic, 8);
-    if (std::memcmp(magic, "HFREF001", 8) != 0) {
+    if (std::memcmp(magic, "HFREF002", 8) != 0) {
       std::cerr << "bad case file\n";
       MPI_Finalize();
       return 2;
@@ -257,6 +325,15 @@ int main(int argc, char** argv) {
     const double pcg_rtol = in.get<double>();
     const std::int32_t pcg_maxit = in.get<std::int32_t>();
     const std::int32_t pcg_interval = in.get<std::int32_t>();
+    policy.point_smoother = in.get<std::int32_t>()
+                                ? MgPointSmootherKind::chebyshev_jacobi
+                                : MgPointSmootherKind::red_black;
+    policy.chebyshev_lower_spectrum_fraction = in.get<double>();
+    policy.diagonal_shortcut_maximum_ratio = in.get<double>();
+    const std::int32_t solver = in.get<std::int32_t>();  // 0 pcg, 1 fgmres
+    const std::int32_t restart = in.get<std::int32_t>();
+    const bool volume_rows = in.get<std::int32_t>() != 0;
+    const std::int32_t repeats = in.get<std::int32_t>();
     const std::size_t cells = static_cast<std::size_t>(n.x) * n.y * n.z;
     const Int3 xe{n.x + 1, n.y, n.z}, ye{n.x, n.y + 1, n.z}, ze{n.x, n.y, n.z + 1};
     const std::size_t nfx = static_cast<std::size_t>(xe.x) * xe.y * xe.z;
@@ -336,7 +413,7 @@ int main(int argc, char** argv) {
     st = MgWorkspace::bind(wreq, vectors.view, workspace);
     if (!st) return fail("mg workspace bind", st);
     st = ReductionEngine::compile(MPI_COMM_SELF, ReductionMode::mpi_allreduce,
-                                  4U, reductions);
+                                  64U, reductions);
     if (!st) return fail("reductions", st);
     const HaloFieldSpec halo_fields[1] = {{10U, 1U, 1U}};
     const HaloTopology topology{bc[0] == 2, bc[2] == 2, bc[4] == 2};
@@ -360,14 +437,34 @@ int main(int argc, char** argv) {
         as_const(diag.view), face_view(bx, xe, CartesianAxis::x, 710U),
         face_view(by, ye, CartesianAxis::y, 711U),
         face_view(bz, ze, CartesianAxis::z, 712U)};
+    // the Krylov workspace comes first: its fingerprint is part of the
+    // identity that the solver, the operator and the preconditioner share
+    LinearWorkspaceRequirements lreq;
+    const bool fgmres = solver == 1;
+    st = make_linear_workspace_requirements(
+        fgmres ? LinearAlgorithm::fgmres : LinearAlgorithm::pcg, n, 1U,
+        fgmres ? static_cast<std::uint32_t>(restart) : 0U,
+        ReductionMode::mpi_allreduce, 91U, lreq);
+    if (!st) return fail("linear workspace requirements", st);
+    Field lvec(n, lreq.vector_slots, 40U, 740U, 1U);
+    Field lscalar({static_cast<std::int32_t>(lreq.scalar_doubles), 1, 1}, 1U,
+                  41U, 740U, 0U);
+    SolverWorkspace lws;
+    st = SolverWorkspace::bind(lreq, lvec.view, lscalar.view, lws);
+    if (!st) return fail("solver workspace bind", st);
+    LinearIdentity identity{301U, 302U, 303U, lws.fingerprint(), 305U};
+    spec.operator_class =
+        MgOperatorClass::symmetric_diagonally_dominant_m_matrix;
+    spec.identity = identity;
     NativeCartesianMgPlan plan;
     st = NativeCartesianMgPlan::compile(spec, services, coeffs, plan);
     if (!st) return fail("mg compile", st);
 
     Writer out(argv[2]);
-    out.out.write("HFOUT001", 8);
+    out.out.write("HFOUT002", 8);
     const std::size_t levels = plan.level_count();
     out.put<std::int32_t>(static_cast<std::int32_t>(levels));
+    out.put<std::int32_t>(plan.uses_diagonal_preconditioner() ? 1 : 0);
     std::vector<std::size_t> level_cells(levels);
     for (std::size_t l = 0; l < levels; ++l) {
       MgLevelView v;
@@ -408,19 +505,6 @@ int main(int argc, char** argv) {
         out.vec(values);
       }
     }
-    // solve_pcg with the MG preconditioner
-    LinearWorkspaceRequirements lreq;
-    st = make_linear_workspace_requirements(LinearAlgorithm::pcg, n, 1U, 0U,
-                                            ReductionMode::mpi_allreduce, 91U,
-                                            lreq);
-    if (!st) return fail("linear workspace requirements", st);
-    Field lvec(n, lreq.vector_slots, 40U, 740U, 1U);
-    Field lscalar({static_cast<std::int32_t>(lreq.scalar_doubles), 1, 1}, 1U,
-                  41U, 740U, 0U);
-    SolverWorkspace lws;
-    st = SolverWorkspace::bind(lreq, lvec.view, lscalar.view, lws);
-    if (!st) return fail("solver workspace bind", st);
-    LinearIdentity identity{301U, 302U, 303U, lws.fingerprint(), 305U};
     HarnessOperator op;
     op.n = n;
     std::memcpy(op.bc, bc, sizeof bc);
@@ -433,6 +517,22 @@ int main(int argc, char** argv) {
     PcgAdapter pc;
     pc.plan = &plan;
     pc.identity = identity;
+    std::vector<double> volume(cells);
+    for (int k = 0; k < n.z; ++k)
+      for (int j = 0; j < n.y; ++j)
+        for (int i = 0; i < n.x; ++i)
+          volume[cidx(n, i, j, k)] =
+              (faces[0][i + 1] - faces[0][i]) * (faces[1][j + 1] - faces[1][j]) *
+              (faces[2][k + 1] - faces[2][k]);
+    VolumeRowsOperator vop;
+    vop.base = &op;
+    vop.volume = &volume;
+    Field vscratch(n, 1U, 52U, 752U, 1U);
+    VolumeScaledAdapter vpc;
+    vpc.plan = &plan;
+    vpc.volume = &volume;
+    vpc.scratch = vscratch.view;
+    vpc.n = n;
     Field rhs(n, 1U, 50U, 750U, 0U), sol(n, 1U, 51U, 751U, 0U);
     for (std::size_t c = 0; c < cells; ++c) rhs.view.base[c] = b[c];
     LinearSolveInvocation inv;
@@ -443,18 +543,38 @@ int main(int argc, char** argv) {
     inv.control.relative_tolerance = pcg_rtol;
     inv.control.maximum_iterations = static_cast<std::uint32_t>(pcg_maxit);
     inv.control.true_residual_interval = static_cast<std::uint32_t>(pcg_interval);
-    const LinearSolveResult res = solve_pcg(op, pc, inv, lws, reductions);
+    if (fgmres) inv.control.restart = static_cast<std::uint32_t>(restart);
+    auto solve = [&]() {
+      for (std::size_t c = 0; c < cells; ++c) sol.view.base[c] = 0.0;
+      if (!fgmres) return solve_pcg(op, pc, inv, lws, reductions);
+      if (volume_rows) return solve_fgmres(vop, vpc, inv, lws, reductions);
+      return solve_fgmres(op, plan, inv, lws, reductions);
+    };
+    const LinearSolveResult res = solve();
+    std::vector<double> seconds;
+    for (std::int32_t rep = 0; rep < repeats; ++rep) {
+      const double t0 = MPI_Wtime();
+      const LinearSolveResult again = solve();
+      seconds.push_back(MPI_Wtime() - t0);
+      if (again.iterations != res.iterations) rc = 4;
+    }
     out.put<std::int32_t>(static_cast<std::int32_t>(res.status.code));
     out.put<std::int32_t>(static_cast<std::int32_t>(res.termination));
     out.put<std::int32_t>(static_cast<std::int32_t>(res.iterations));
     out.put<double>(res.initial_true_residual);
     out.put<double>(res.final_true_residual);
     out.put<double>(res.rhs_norm);
+    out.put<std::int64_t>(static_cast<std::int64_t>(res.operator_applies));
+    out.put<std::int64_t>(static_cast<std::int64_t>(res.preconditioner_applies));
+    out.put<std::int64_t>(static_cast<std::int64_t>(res.norm_breakdown_restarts));
     std::vector<double> x(sol.view.base, sol.view.base + cells);
     out.vec(x);
+    out.put<std::int32_t>(static_cast<std::int32_t>(seconds.size()));
+    out.vec(seconds);
     if (!out.out) rc = 3;
-    std::cerr << "ref_driver: levels=" << levels << " pcg status="
-              << static_cast<int>(res.status.code) << " iterations="
+    std::cerr << "ref_driver: levels=" << levels << " solver="
+              << (fgmres ? "fgmres" : "pcg") << " status="
+              << static_cast<int>(res.status.code) << "/" << res.status.detail << " iterations="
               << res.iterations << " final=" << res.final_true_residual
               << "\n";
   }

@@ -1,13 +1,15 @@
 """Writes golden fixtures from the reference's own compiled hot path.
 
-Runs oracle/_ref/hf_ref_driver (the reference NativeCartesianMgPlan and
-solve_pcg, compiled from the reference sources by oracle/ref.mk) on small
-seeded cases and stores inputs and outputs as .npz fixtures next to this
-script.  Inputs: grid, boundary kinds, policy, coefficients (D, beta_x,
-beta_y, beta_z from the face rule), IBM activity, an MG input residual r and
-a PCG right-hand side b.  Outputs: the level plan, every level's diagonal,
-z = M r with the solution/rhs/residual slot of every level after that
-application, and the PCG solution with its iteration count.
+Runs oracle/_ref/hf_ref_driver (the reference NativeCartesianMgPlan,
+solve_pcg and solve_fgmres, compiled from the reference sources by
+oracle/ref.mk) on small seeded cases and stores inputs and outputs as .npz
+fixtures next to this script.  Inputs: grid, boundary kinds, policy,
+coefficients (D, beta_x, beta_y, beta_z from the face rule, or the
+compressible pressure system's D), IBM activity, an MG input residual r and
+a Krylov right-hand side b.  Outputs: the level plan, whether the plan chose
+the diagonal shortcut, every level's diagonal, z = M r with the
+solution/rhs/residual slot of every level after that application, and the
+Krylov solution with its iteration and application counts.
 
 Usage: python tests/golden/make_golden.py   (needs the reference tree)
 """
@@ -91,12 +93,52 @@ def cases():
     yield "line_xy_deep", line_case(
         (12, 10, 6), 31, -1, [1, 1, 2, 2, 1, 1], depth=6.0), dict(
             coarse_sweeps=6)
+    # the production pairing: Chebyshev-Jacobi F(1,2) cycles, diagonal
+    # shortcut, FGMRES(12), the volume-scaled cold pairing
+    yield "cheb_mixed", with_bc(
+        hf_cases.lognormal_sphere(n=(16, 12, 10), seed=32, radius=0.25),
+        mixed), dict(null_space_constant=0, point_smoother=1, pre_sweeps=1,
+                     post_sweeps=2)
+    yield "cheb_fcycle_null_space", hf_cases.lognormal_sphere(
+        n=(16, 16, 12), seed=33), dict(point_smoother=1, f_cycle=1,
+                                        chebyshev_lower_fraction=0.25)
+    yield "fgmres_rb_sphere", with_bc(
+        hf_cases.lognormal_sphere(n=(16, 12, 10), seed=34, radius=0.25),
+        mixed), dict(null_space_constant=0, solver=1, restart=12)
+    yield "fgmres_production", compressible(
+        (20, 16, 12), 35, 6.0), dict(production=True, solver=1, restart=12)
+    yield "fgmres_restart3_resmin", compressible(
+        (16, 12, 12), 36, 12.0), dict(production=True, solver=1, restart=3,
+                                      unit_linear=0)
+    yield "fgmres_volume_rows", compressible(
+        (16, 16, 10), 37, 6.0), dict(production=True, solver=1, restart=12,
+                                     volume_rows=1)
+    yield "shortcut_fgmres", compressible(
+        (16, 12, 10), 38, 0.05), dict(production=True, solver=1, restart=12)
 
 
-def write_case(path, case, pol, r, b, rtol, maxit, interval):
+def compressible(n, seed, cfl):
+    """The compressible pressure-correction system of
+    hf_cases.compressible_flame (D = a0 V drho/dp + face sum) assembled by
+    the oracle's restatement of src/solver_pressure.cpp:110-332."""
+    case = hf_cases.compressible_flame(n=n, seed=seed, acoustic_cfl=cfl)
+    d0, bx, by, bz = O.assemble(case["n"], case["faces"], case["bc"],
+                                case["mobility"])
+    st, D, b = O.assemble_pressure_system(
+        case["n"], case["faces"], bx, by, bz, case["phi"], case["rho"],
+        case["rho_n"], case["drho_dp"], case["bdf"], case["rho_nm1"],
+        case["source"])
+    assert st == 0
+    case["coeffs"] = (D, bx, by, bz)
+    case["rhs"] = np.where(case["cell_active"] != 0, b, 0.0)
+    return case
+
+
+def write_case(path, case, pol, r, b, rtol, maxit, interval, solver=0,
+               restart=0, volume_rows=0, repeats=0):
     n = case["n"]
     with open(path, "wb") as f:
-        f.write(b"HFREF001")
+        f.write(b"HFREF002")
         f.write(struct.pack("<3i", *n))
         f.write(struct.pack("<6i", *case["bc"]))
         f.write(struct.pack("<d", pol.anisotropy_threshold))
@@ -106,6 +148,10 @@ def write_case(path, case, pol, r, b, rtol, maxit, interval):
                             pol.null_space_constant, pol.unit_linear))
         f.write(struct.pack("<i", 1))
         f.write(struct.pack("<dii", rtol, maxit, interval))
+        f.write(struct.pack("<i", pol.point_smoother))
+        f.write(struct.pack("<dd", pol.chebyshev_lower_fraction,
+                            pol.diagonal_shortcut_ratio))
+        f.write(struct.pack("<4i", solver, restart, volume_rows, repeats))
         for a in case["faces"]:
             f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
         for a in case["coeffs"]:
@@ -118,7 +164,7 @@ def write_case(path, case, pol, r, b, rtol, maxit, interval):
 
 def read_output(path):
     data = open(path, "rb").read()
-    assert data[:8] == b"HFOUT001"
+    assert data[:8] == b"HFOUT002"
     off = 8
 
     def ints(k):
@@ -133,10 +179,17 @@ def read_output(path):
         off += 8 * k
         return v
 
+    def int64s(k):
+        nonlocal off
+        v = struct.unpack_from(f"<{k}q", data, off)
+        off += 8 * k
+        return v
+
     (levels,) = ints(1)
+    (shortcut,) = ints(1)
     plan = np.array([ints(5) for _ in range(levels)], dtype=np.int32)
     cells = [int(p[0] * p[1] * p[2]) for p in plan]
-    out = {"plan": plan}
+    out = {"plan": plan, "shortcut": np.int32(shortcut)}
     for l in range(levels):
         out[f"diag_{l}"] = doubles(cells[l])
     out["z"] = doubles(cells[0])
@@ -144,13 +197,29 @@ def read_output(path):
         for s in ("x", "b", "r"):
             out[f"slot_{s}_{l}"] = doubles(cells[l])
     status, termination, iterations = ints(3)
-    out["pcg_status"] = np.int32(status)
-    out["pcg_termination"] = np.int32(termination)
-    out["pcg_iterations"] = np.int32(iterations)
-    out["pcg_initial"], out["pcg_final"], out["pcg_rhs_norm"] = doubles(3)
-    out["pcg_x"] = doubles(cells[0])
+    out["krylov_status"] = np.int32(status)
+    out["krylov_termination"] = np.int32(termination)
+    out["krylov_iterations"] = np.int32(iterations)
+    (out["krylov_initial"], out["krylov_final"],
+     out["krylov_rhs_norm"]) = doubles(3)
+    applies = int64s(3)
+    out["krylov_applies"] = np.array(applies, dtype=np.int64)
+    out["krylov_x"] = doubles(cells[0])
+    (repeats,) = ints(1)
+    out["seconds"] = doubles(repeats)
     assert off == len(data)
     return out
+
+
+def run_driver(case, pol, r, b, rtol=1e-12, maxit=400, interval=4, solver=0,
+               restart=0, volume_rows=0, repeats=0):
+    with tempfile.TemporaryDirectory() as tmp:
+        cin = os.path.join(tmp, "case.bin")
+        cout = os.path.join(tmp, "out.bin")
+        write_case(cin, case, pol, r, b, rtol, maxit, interval, solver,
+                   restart, volume_rows, repeats)
+        subprocess.run([DRIVER, cin, cout], check=True)
+        return read_output(cout)
 
 
 def main():
@@ -158,23 +227,26 @@ def main():
         raise SystemExit("build the reference closure first: "
                          "make -C oracle -f ref.mk")
     for name, case, overrides in cases():
-        pol = O.policy(**overrides)
-        case["coeffs"] = O.assemble(case["n"], case["faces"], case["bc"],
-                                    case["mobility"])
+        overrides = dict(overrides)
+        solver = overrides.pop("solver", 0)
+        restart = overrides.pop("restart", 0)
+        volume_rows = overrides.pop("volume_rows", 0)
+        pol = (O.production_policy(**{k: v for k, v in overrides.items()
+                                      if k != "production"})
+               if overrides.get("production") else O.policy(**overrides))
+        if "coeffs" not in case:
+            case["coeffs"] = O.assemble(case["n"], case["faces"], case["bc"],
+                                        case["mobility"])
         cells = case["n"][0] * case["n"][1] * case["n"][2]
         rng = np.random.Generator(np.random.Philox(1000 + len(name)))
         r = rng.standard_normal(cells)
         b = case["rhs"]
-        if not pol.null_space_constant:
+        if not pol.null_space_constant and "phi" not in case:
             b = np.where(case["cell_active"] != 0,
                          rng.standard_normal(cells), 0.0)
         rtol, maxit, interval = 1e-12, 400, 4
-        with tempfile.TemporaryDirectory() as tmp:
-            cin = os.path.join(tmp, "case.bin")
-            cout = os.path.join(tmp, "out.bin")
-            write_case(cin, case, pol, r, b, rtol, maxit, interval)
-            subprocess.run([DRIVER, cin, cout], check=True)
-            out = read_output(cout)
+        out = run_driver(case, pol, r, b, rtol, maxit, interval, solver,
+                         restart, volume_rows)
         fixture = {
             "n": np.array(case["n"], dtype=np.int32),
             "bc": np.array(case["bc"], dtype=np.int32),
@@ -182,7 +254,10 @@ def main():
                                 pol.post_sweeps, pol.maximum_levels,
                                 pol.coarse_sweeps, pol.minimum_coarse_extent,
                                 pol.f_cycle, pol.null_space_constant,
-                                pol.unit_linear], dtype=np.float64),
+                                pol.unit_linear, pol.point_smoother,
+                                pol.chebyshev_lower_fraction,
+                                pol.diagonal_shortcut_ratio],
+                               dtype=np.float64),
             "faces_x": case["faces"][0], "faces_y": case["faces"][1],
             "faces_z": case["faces"][2],
             "diag": case["coeffs"][0], "bx": case["coeffs"][1],
@@ -190,12 +265,17 @@ def main():
             "cell_active": case["cell_active"],
             "x_active": case["x_active"], "y_active": case["y_active"],
             "z_active": case["z_active"], "r": r, "b": b,
-            "pcg_control": np.array([rtol, maxit, interval]),
+            "krylov_control": np.array([rtol, maxit, interval, solver,
+                                        restart, volume_rows]),
         }
+        out.pop("seconds")
         fixture.update(out)
         np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **fixture)
-        print(f"{name}: levels={len(out['plan'])} pcg_it="
-              f"{int(out['pcg_iterations'])} status={int(out['pcg_status'])}")
+        print(f"{name}: levels={len(out['plan'])} "
+              f"{'fgmres' if solver else 'pcg'} "
+              f"it={int(out['krylov_iterations'])} "
+              f"status={int(out['krylov_status'])} "
+              f"shortcut={int(out['shortcut'])}")
 
 
 if __name__ == "__main__":
