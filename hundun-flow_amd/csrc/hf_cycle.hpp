@@ -117,6 +117,14 @@ __global__ void k_copy(Chunk ch, double* dst, const double* src) {
   for (int c = lo + threadIdx.x; c < hi; c += kBlock) dst[c] = src[c];
 }
 
+// dst[0..n) = value: clears memory with vector stores, where a solve may run
+// inside a caller's captured stream work (no memset nodes).
+__global__ void k_fill(Chunk ch, double* dst, double value) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) dst[c] = value;
+}
+
 // ---------------------------------------------------------------------------
 // Projection helpers (constant null space, src/solver_mg.cpp:1412-1454).
 // Level 0 recomputes its fluid volumes from the axis widths and the activity

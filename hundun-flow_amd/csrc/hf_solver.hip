@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -55,6 +56,7 @@ constexpr uint32_t kDetailBreakdown = 9106U;
 constexpr uint32_t kDetailNonFinite = 9107U;
 constexpr uint32_t kDetailLine = 9108U;  // line relaxation pivot vanished
 constexpr uint32_t kDetailLineOff = 9109U;  // line levels not enabled
+constexpr uint32_t kDetailScratch = 9110U;  // beyond the static scratch array
 
 hf_status ok() { return {HF_OK, 0U}; }
 hf_status fail(int code, uint32_t detail) { return {code, detail}; }
@@ -246,6 +248,7 @@ struct ProfScope {
 
 struct hf_plan {
   hf_grid_desc grid{};
+  int device = 0;  // HIP device current at creation (owner of the stream)
   hf_policy policy{};
   hipStream_t stream = nullptr;
   int n[3] = {0, 0, 0};
@@ -1082,6 +1085,52 @@ Lv<double> krylov_view(const hf_plan* p) { return op_view(p->levels[0]); }
 
 }  // namespace
 
+// Reduction scratch of hf_pcg_solve_static: the per-block partials, the
+// folded partials and the FP64 scalar block of one solve, in one array that
+// each GPU holds for the life of the process.  In host code `scratch` is a
+// stand-in for the symbol, not a device address: kernels get the address
+// that hipGetSymbolAddress returned for the device (static_scratch below).
+constexpr size_t kStaticScratchBytes = size_t{8} << 20;
+static __device__ __align__(256) double
+    scratch[kStaticScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+hf_status static_scratch(int device, double** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices)
+    return fail(HF_INVALID_PLAN, kDetailScratch);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch[device] == nullptr) {
+    int current = 0;
+    if (hipGetDevice(&current) != hipSuccess)
+      return fail(HF_INVALID_PLAN, kDetailHip);
+    if (current != device && hipSetDevice(device) != hipSuccess)
+      return fail(HF_INVALID_PLAN, kDetailHip);
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device) (void)hipSetDevice(current);
+    if (e != hipSuccess || ptr == nullptr)
+      return fail(HF_INVALID_PLAN, kDetailHip);
+    g_scratch[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[device];
+  return ok();
+}
+
+// Doubles of reduction scratch one solve of plan p uses.
+int64_t reduction_doubles(const hf_plan* p) {
+  return 3 * p->partial_stride + 3 * int64_t{kFoldStride} + S_COUNT;
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1137,6 +1186,10 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   p->grid = *grid;
   p->policy = *policy;
   p->stream = static_cast<hipStream_t>(stream);
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  }
   p->n[0] = grid->nx;
   p->n[1] = grid->ny;
   p->n[2] = grid->nz;
@@ -1517,7 +1570,6 @@ hf_status hf_mg_apply(hf_plan* p, const double* r, double* z) {
 
 static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                           const hf_solve_control* ctl, hf_solve_result* res) {
-  const int64_t N = p->cells;
   const Lv<double> A = krylov_view(p);
   const Tiles& tr = p->levels[0].tz;  // z-marching Krylov stencils
   // the PCG update stores the preconditioner's entry when level 0 is a grid
@@ -1576,7 +1628,7 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   const double tol = std::max(ctl->absolute_tolerance, ctl->relative_tolerance * bnorm);
   res->true_residual_limit = tol;
   if (bnorm == 0.0) {
-    HF_TRY(hip_check(hipMemsetAsync(solution, 0, N * sizeof(double), p->stream)));
+    HF_LAUNCH(k_fill, nbv, kBlock, 0, p->stream, ch, solution, 0.0);
     return finish(ok(), HF_TERM_ZERO_RHS);
   }
   res->operator_applies++;
@@ -1776,6 +1828,75 @@ hf_status hf_pcg_solve(hf_plan* p, const double* rhs, double* solution,
   (void)hipEventDestroy(e1);
   const hipError_t last = hipGetLastError();
   if (st.code == HF_OK && last != hipSuccess) {
+    res->status = fail(HF_INVALID_PLAN, kDetailHip);
+    return res->status;
+  }
+  return st;
+}
+
+hf_status hf_pcg_solve_static(hf_plan* p, const double* rhs, double* solution,
+                              const hf_solve_control* ctl,
+                              hf_solve_result* res) {
+  if (res != nullptr) std::memset(res, 0, sizeof *res);
+  if (p == nullptr || rhs == nullptr || solution == nullptr || ctl == nullptr ||
+      res == nullptr || !p->coefficients_ready)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  auto refuse = [&](hf_status st) {
+    res->status = st;
+    res->termination = HF_TERM_INVALID_PLAN;
+    return st;
+  };
+  if (!std::isfinite(ctl->absolute_tolerance) ||
+      !std::isfinite(ctl->relative_tolerance) || ctl->absolute_tolerance < 0.0 ||
+      ctl->relative_tolerance < 0.0 ||
+      (ctl->absolute_tolerance == 0.0 && ctl->relative_tolerance == 0.0) ||
+      ctl->maximum_iterations == 0 || ctl->true_residual_interval == 0)
+    return refuse(fail(HF_INVALID_PLAN, kDetailPlan));
+  // a grid whose reductions need more than the static array is refused
+  if (reduction_doubles(p) * static_cast<int64_t>(sizeof(double)) >
+      static_cast<int64_t>(kStaticScratchBytes)) {
+    std::fprintf(stderr,
+                 "hundun_flow: hf_pcg_solve_static needs %lld bytes of "
+                 "reduction scratch, the static array holds %zu\n",
+                 static_cast<long long>(reduction_doubles(p) * 8),
+                 kStaticScratchBytes);
+    return refuse(fail(HF_INVALID_PLAN, kDetailScratch));
+  }
+  double* base = nullptr;
+  {
+    const hf_status st = static_scratch(p->device, &base);
+    if (st.code != HF_OK) return refuse(st);
+  }
+  // the solve's kernels take the static array in place of the plan's
+  // reduction buffers; event timing (which creates events) is off
+  double* const own[3] = {p->partials, p->folded, p->scalars};
+  const bool profiling = p->profiling;
+  p->partials = base;
+  p->folded = base + 3 * p->partial_stride;
+  p->scalars = p->folded + 3 * kFoldStride;
+  p->profiling = false;
+  // the array keeps the previous call's values: the scalar block (the line
+  // failure flag among them) starts from zero
+  const Chunk sc = make_chunk(S_COUNT);
+  HF_LAUNCH(k_fill, chunk_blocks(sc), kBlock, 0, p->stream, sc, p->scalars,
+            0.0);
+  const auto t0 = std::chrono::steady_clock::now();
+  hf_status st = pcg_impl(p, rhs, solution, ctl, res);
+  if (st.code != HF_OK && res->termination == HF_TERM_CONVERGED) {
+    res->status = st;
+    res->termination = st.detail == kDetailLine ? HF_TERM_PRECONDITIONER_FAILURE
+                                                : HF_TERM_INVALID_PLAN;
+  }
+  const hipError_t synced = hipStreamSynchronize(p->stream);
+  const auto t1 = std::chrono::steady_clock::now();
+  res->solve_milliseconds =
+      std::chrono::duration<double, std::milli>(t1 - t0).count();
+  p->partials = own[0];
+  p->folded = own[1];
+  p->scalars = own[2];
+  p->profiling = profiling;
+  const hipError_t last = hipGetLastError();
+  if (st.code == HF_OK && (synced != hipSuccess || last != hipSuccess)) {
     res->status = fail(HF_INVALID_PLAN, kDetailHip);
     return res->status;
   }

@@ -38,7 +38,7 @@ ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",
     "hf_assemble_coefficients", "hf_plan_set_coefficients",
     "hf_operator_apply", "hf_mg_apply", "hf_pcg_solve",
-    "hf_adjoint_gradients", "hf_project_mean", "hf_plan_level_count",
+    "hf_pcg_solve_static", "hf_adjoint_gradients", "hf_project_mean", "hf_plan_level_count",
     "hf_plan_level_info", "hf_plan_level_array", "hf_plan_device_bytes",
     "hf_plan_set_profiling", "hf_plan_profile", "hf_status_message",
 ]
@@ -142,6 +142,8 @@ def load_library(path: str = LIBRARY_PATH):
         "hf_mg_apply": [P, D, D],
         "hf_pcg_solve": [P, D, D, ctypes.POINTER(hf_solve_control),
                          ctypes.POINTER(hf_solve_result)],
+        "hf_pcg_solve_static": [P, D, D, ctypes.POINTER(hf_solve_control),
+                                ctypes.POINTER(hf_solve_result)],
         "hf_adjoint_gradients": [P, D, D, D, ctypes.POINTER(hf_solve_control),
                                  D, D, ctypes.POINTER(hf_solve_result)],
         "hf_project_mean": [P, D],
@@ -310,15 +312,19 @@ class Plan:
         c.flexible = 1 if flexible else 0
         return c
 
-    def solve_pcg(self, rhs, solution=None, check=True, **control):
+    def solve_pcg(self, rhs, solution=None, check=True, static=False,
+                  **control):
+        """PCG solve; static=True calls hf_pcg_solve_static, whose reduction
+        scratch is the library's static per-GPU array (nothing allocated)."""
         x = self._torch.zeros_like(rhs) if solution is None else solution
         ctl = self.control(**control)
         res = hf_solve_result()
-        status = self.lib.hf_pcg_solve(self.handle, _ptr(rhs), _ptr(x),
-                                       ctypes.byref(ctl), ctypes.byref(res))
+        name = "hf_pcg_solve_static" if static else "hf_pcg_solve"
+        status = getattr(self.lib, name)(self.handle, _ptr(rhs), _ptr(x),
+                                         ctypes.byref(ctl), ctypes.byref(res))
         result = SolveResult.from_c(res)
         if check:
-            _check("hf_pcg_solve", status)
+            _check(name, status)
         return x, result
 
     def project_mean(self, field):

@@ -205,6 +205,23 @@ hf_status hf_pcg_solve(hf_plan* plan, const double* rhs, double* solution,
                        const hf_solve_control* control,
                        hf_solve_result* result);
 
+/* hf_pcg_solve that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory or create HIP objects
+ * inside their loop (tight serving loops, harnesses that forbid hipMalloc,
+ * stream work a caller captures around the solve).  The plan's own buffers
+ * are used as by hf_pcg_solve, but the reduction scratch (per-block
+ * partials, folded partials, the FP64 scalar block) lives in one static
+ * device array of 8 MiB per GPU, held until the process ends; a grid whose
+ * reductions need more (about 88M cells and up) is refused with
+ * HF_INVALID_PLAN, detail 9110.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (two streams, two host threads) race.  No
+ * HIP events are created: solve_milliseconds is host wall time up to the
+ * final stream synchronisation, and hf_plan_profile records nothing. */
+hf_status hf_pcg_solve_static(hf_plan* plan, const double* rhs,
+                              double* solution,
+                              const hf_solve_control* control,
+                              hf_solve_result* result);
+
 /* Adjoint gradient of a scalar loss L(p) with p the gauge-fixed pressure
  * solution of A(rho) p = b (zero fluid mean).  Given dL/dp (grad_p) and the
  * forward solution p, solves A lambda = P grad_p (A is symmetric, so the
