@@ -1,0 +1,326 @@
+// Kernels of the reference's production pressure pairing on gfx950:
+// Chebyshev-Jacobi smoothing, the diagonal shortcut of the preconditioner,
+// and the single-reduction FGMRES (Arnoldi step, basis combinations and the
+// volume-scaled rows of the cold pressure system).
+#pragma once
+
+#include "hf_kernels.hpp"
+
+namespace hf {
+
+// Geometric cell volume w_x w_y w_z, multiplied left to right as the
+// reference's cell_volume (src/solver_cold.hpp:378-415 scales by it).
+__device__ __forceinline__ double geom_volume(const Widths& w, int i, int j,
+                                              int k) {
+  return w.x[i] * w.y[j] * w.z[k];
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev-Jacobi smoothing (src/solver_mg.cpp:2916-3090).  One launch is
+// one stage: every neighbour comes from the stage's input x (Jacobi), the
+// direction lives in the level's r slot, and
+//   d = rf (b - A x) / D            (first stage)
+//   d = pf d + rf (b - A x) / D     (later stages)
+//   x_next = x + d.
+// z-marching as k_stencil_z; x and x_next are distinct buffers.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_cheb_stage(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
+                             typename Compute<T>::type rf,
+                             typename Compute<T>::type pf, int first) {
+  using C = typename Compute<T>::type;
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  if (i >= nx || j >= ny) return;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  const int plane = nx * ny;
+  int c = cell_index(nx, ny, i, j, k0);
+  C xc = C(ld(x[c]));
+  C zl = k0 > 0 ? C(ld(x[c - plane]))
+                : neighbour_or_boundary<C>(bc.m[4], xc, x, c + (nz - 1) * plane);
+  C fzl = C(ld(L.fz[c]));
+  for (int k = k0; k < k1; ++k) {
+    const C zh = k + 1 < nz ? C(ld(x[c + plane]))
+                            : neighbour_or_boundary<C>(bc.m[5], xc, x,
+                                                       c - (nz - 1) * plane);
+    const C fzh = C(ld(L.fz[c + plane]));
+    const int row = j + ny * k;
+    const C xl = i > 0 ? C(ld(x[c - 1]))
+                       : neighbour_or_boundary<C>(bc.m[0], xc, x, c + (nx - 1));
+    const C xh = i + 1 < nx ? C(ld(x[c + 1]))
+                            : neighbour_or_boundary<C>(bc.m[1], xc, x,
+                                                       c - (nx - 1));
+    const C yl = j > 0 ? C(ld(x[c - nx]))
+                       : neighbour_or_boundary<C>(bc.m[2], xc, x,
+                                                  c + (ny - 1) * nx);
+    const C yh = j + 1 < ny ? C(ld(x[c + nx]))
+                            : neighbour_or_boundary<C>(bc.m[3], xc, x,
+                                                       c - (ny - 1) * nx);
+    const int fyo = c + nx * k;
+    const C d = C(ld(L.D[c]));
+    C v = d * xc;
+    v -= C(ld(L.fx[c + row])) * xl;
+    v -= C(ld(L.fx[c + row + 1])) * xh;
+    v -= C(ld(L.fy[fyo])) * yl;
+    v -= C(ld(L.fy[fyo + nx])) * yh;
+    v -= fzl * zl;
+    v -= fzh * zh;
+    const C defect = C(ld(L.b[c])) - v;
+    const C scaled = rf * defect / d;
+    const C dir = first ? scaled : pf * C(ld(L.r[c])) + scaled;
+    L.r[c] = st<T>(dir);
+    xn[c] = st<T>(xc + dir);
+    zl = xc;
+    xc = zh;
+    fzl = fzh;
+    c += plane;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal shortcut (src/solver_mg.cpp:1573-1611 selects it, :4926-4936
+// applies it): x = b / D on level 0.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_diag_solve(Chunk ch, Lv<T> L) {
+  using C = typename Compute<T>::type;
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock)
+    L.x[c] = st<T>(C(ld(L.b[c])) / C(ld(L.D[c])));
+}
+
+// Largest finest-row ratio (sum of the six masked faces) / D, physical
+// boundary faces included, as the non-negative bit pattern max into *out
+// (ratios are >= 0 on a validated hierarchy, so integer order is float
+// order; the maximum does not depend on the order of the updates).
+__global__ void k_row_ratio_max(int nx, int ny, int nz, const double* D,
+                                const double* fx, const double* fy,
+                                const double* fz,
+                                unsigned long long* out) {
+  const int64_t c =
+      xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t cells = static_cast<int64_t>(nx) * ny * nz;
+  double ratio = 0.0;
+  if (c < cells) {
+    const int i = static_cast<int>(c % nx);
+    const int j = static_cast<int>((c / nx) % ny);
+    const int k = static_cast<int>(c / (static_cast<int64_t>(nx) * ny));
+    const int64_t xf = i + static_cast<int64_t>(nx + 1) * (j + static_cast<int64_t>(ny) * k);
+    const int64_t yf = i + static_cast<int64_t>(nx) * (j + static_cast<int64_t>(ny + 1) * k);
+    const int64_t zf = c;
+    const double sum = fx[xf] + fx[xf + 1] + fy[yf] + fy[yf + nx] + fz[zf] +
+                       fz[zf + static_cast<int64_t>(nx) * ny];
+    ratio = sum / D[c];
+    if (!(ratio >= 0.0)) ratio = 0.0;  // max(0, NaN) keeps 0 (std::max)
+  }
+  // wave maximum, then one atomic per wave
+  for (int off = kWave / 2; off > 0; off >>= 1)
+    ratio = fmax(ratio, __shfl_down(ratio, off, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(ratio)));
+}
+
+// ---------------------------------------------------------------------------
+// Single-reduction FGMRES (src/solver_krylov.cpp:1557-2650).
+// ---------------------------------------------------------------------------
+constexpr int kFgDots = 16;       // accumulators of one Arnoldi pass
+constexpr int kFgMaxRestart = 64;
+
+enum FgMode : int { kFgArnoldi = 0, kFgResidual = 1 };
+
+// Level-0 FP64 action y = A' x, A' = A or A divided by the geometric cell
+// volume (the per-volume rows of the cold system), z-marching as
+// k_stencil_z, with the FGMRES epilogues:
+//  kFgArnoldi:  out = A' x - shift * aux (aux = v_col); accumulator a < nd
+//               holds basis_a . out, accumulator wacc holds out . out;
+//  kFgResidual: out = 1 b + (-1) A' x (b = aux); accumulator 0 = out . out.
+// Partials of accumulator a sit at partials + a * pstride.
+template <int kMode>
+__global__ void k_fg_stencil(Tiles t, Lv<double> A, Bc bc, const double* x,
+                             const double* aux, double* out,
+                             const double* basis, int64_t vstride, int nd,
+                             int wacc, double shift, int volume_rows,
+                             Widths w, double* partials, int64_t pstride) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  double acc[kFgDots];
+#pragma unroll
+  for (int a = 0; a < kFgDots; ++a) acc[a] = 0.0;
+  double ww = 0.0;
+  if (i < nx && j < ny) {
+    const int plane = nx * ny;
+    int c = cell_index(nx, ny, i, j, k0);
+    double xc = x[c];
+    double zl = k0 > 0 ? x[c - plane]
+                       : neighbour_or_boundary<double>(bc.m[4], xc, x,
+                                                       c + (nz - 1) * plane);
+    double fzl = A.fz[c];
+    const double wxy = volume_rows ? w.x[i] * w.y[j] : 1.0;
+    for (int k = k0; k < k1; ++k) {
+      const double zh = k + 1 < nz
+                            ? x[c + plane]
+                            : neighbour_or_boundary<double>(
+                                  bc.m[5], xc, x, c - (nz - 1) * plane);
+      const double fzh = A.fz[c + plane];
+      const int row = j + ny * k;
+      const double xl = i > 0 ? x[c - 1]
+                              : neighbour_or_boundary<double>(bc.m[0], xc, x,
+                                                              c + (nx - 1));
+      const double xh = i + 1 < nx
+                            ? x[c + 1]
+                            : neighbour_or_boundary<double>(bc.m[1], xc, x,
+                                                            c - (nx - 1));
+      const double yl = j > 0 ? x[c - nx]
+                              : neighbour_or_boundary<double>(
+                                    bc.m[2], xc, x, c + (ny - 1) * nx);
+      const double yh = j + 1 < ny
+                            ? x[c + nx]
+                            : neighbour_or_boundary<double>(
+                                  bc.m[3], xc, x, c - (ny - 1) * nx);
+      const int fyo = c + nx * k;
+      double v = A.D[c] * xc;
+      v -= A.fx[c + row] * xl;
+      v -= A.fx[c + row + 1] * xh;
+      v -= A.fy[fyo] * yl;
+      v -= A.fy[fyo + nx] * yh;
+      v -= fzl * zl;
+      v -= fzh * zh;
+      if (volume_rows) v = v / (wxy * w.z[k]);
+      if constexpr (kMode == kFgArnoldi) {
+        const double o = v + -shift * aux[c];
+        out[c] = o;
+#pragma unroll
+        for (int a = 0; a < kFgDots; ++a)
+          if (a < nd) acc[a] += basis[a * vstride + c] * o;
+        ww += o * o;
+      } else {
+        const double o = 1.0 * aux[c] + -1.0 * v;
+        out[c] = o;
+        ww += o * o;
+      }
+      zl = xc;
+      xc = zh;
+      fzl = fzh;
+      c += plane;
+    }
+  }
+  if constexpr (kMode == kFgArnoldi) {
+#pragma unroll
+    for (int a = 0; a < kFgDots; ++a) {
+      if (a < nd) {
+        const double s = block_sum(acc[a], scratch);
+        if (threadIdx.x == 0) partials[a * pstride + lb] = s;
+      }
+    }
+  }
+  const double s = block_sum(ww, scratch);
+  if (threadIdx.x == 0) partials[wacc * pstride + lb] = s;
+}
+
+// More dots of the Arnoldi column when it has more rows than one pass
+// holds: accumulator acc0 + a = basis_(row0 + a) . wv, a < nd, in the
+// z-marching tile layout of k_fg_stencil (same partial count).
+__global__ void k_fg_dots(Tiles t, const double* wv, const double* basis,
+                          int64_t vstride, int row0, int nd, int acc0,
+                          double* partials, int64_t pstride) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int i, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, i, j, k_unused);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  double acc[kFgDots];
+#pragma unroll
+  for (int a = 0; a < kFgDots; ++a) acc[a] = 0.0;
+  if (i < nx && j < ny) {
+    const int plane = nx * ny;
+    int c = cell_index(nx, ny, i, j, k0);
+    for (int k = k0; k < k1; ++k) {
+      const double o = wv[c];
+#pragma unroll
+      for (int a = 0; a < kFgDots; ++a)
+        if (a < nd) acc[a] += basis[(row0 + a) * vstride + c] * o;
+      c += plane;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < kFgDots; ++a) {
+    if (a < nd) {
+      const double s = block_sum(acc[a], scratch);
+      if (threadIdx.x == 0) partials[(acc0 + a) * pstride + lb] = s;
+    }
+  }
+}
+
+// Scales of one basis combination, passed by value.
+struct FgScales {
+  double s[kFgMaxRestart + 1];
+};
+
+// dst = src + sum_{row < count} s[row] * basis_row, the terms added in row
+// order (add_scaled_basis, src/solver_krylov.cpp:510-555); with scale != 0
+// the sum is then multiplied by post (the normalisation of a basis vector,
+// which must stay finite: a non-finite element sets *flag to 1).
+__global__ void k_fg_combine(Chunk ch, double* dst, const double* src,
+                             const double* basis, int64_t vstride,
+                             FgScales s, int count, double post, int scale,
+                             double* flag) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  bool bad = false;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
+    double v = src[c];
+    for (int row = 0; row < count; ++row) v = v + s.s[row] * basis[row * vstride + c];
+    if (scale) {
+      v = post * v;
+      bad = bad || !isfinite(v);
+    }
+    dst[c] = v;
+  }
+  if (bad) *flag = 1.0;
+}
+
+// dst = src * (w_x w_y w_z): the residual handed to the multigrid by the
+// volume-scaled preconditioner (src/solver_cold.hpp:378-415).
+__global__ void k_scale_volume(Tiles t, double* dst, const double* src,
+                               Widths w) {
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  int i, j, k;
+  tile_xyz(t, lb, i, j, k);
+  if (i >= t.nx || j >= t.ny) return;
+  const int c = cell_index(t.nx, t.ny, i, j, k);
+  dst[c] = src[c] * geom_volume(w, i, j, k);
+}
+
+// out[a] = sum of accumulator a's partials, a < nacc (one block; partials
+// are summed in the fixed shuffle-tree order of block_sum).
+__global__ void __launch_bounds__(1024)
+k_finalize_many(const double* partials, int64_t blocks, int64_t stride,
+                int nacc, double* out) {
+  __shared__ double scratch[16];
+  for (int a = 0; a < nacc; ++a) {
+    double v = 0.0;
+    for (int64_t b = threadIdx.x; b < blocks; b += blockDim.x)
+      v += partials[a * stride + b];
+    v = block_sum(v, scratch);
+    if (threadIdx.x == 0) out[a] = v;
+  }
+}
+
+}  // namespace hf

@@ -15,7 +15,9 @@
 #include <new>
 #include <vector>
 
+#include "hf_assembly.hpp"
 #include "hf_kernels.hpp"
+#include "hf_krylov.hpp"
 #include "hf_line.hpp"
 #include "hundun_flow.h"
 
@@ -57,6 +59,7 @@ constexpr uint32_t kDetailNonFinite = 9107U;
 constexpr uint32_t kDetailLine = 9108U;  // line relaxation pivot vanished
 constexpr uint32_t kDetailLineOff = 9109U;  // line levels not enabled
 constexpr uint32_t kDetailScratch = 9110U;  // beyond the static scratch array
+constexpr uint32_t kDetailAssembly = 9111U;  // pressure system inputs refused
 
 hf_status ok() { return {HF_OK, 0U}; }
 hf_status fail(int code, uint32_t detail) { return {code, detail}; }
@@ -107,6 +110,7 @@ struct Level : Coeffs {
   void* lcw = nullptr;
   void* lcd = nullptr;
   void* lghost = nullptr;
+  bool cheb = false;   // Chebyshev-Jacobi smoothing (x2 holds x_next)
   bool sweep = false;  // fused two-colour sweep kernel
   int sweep_zc = 0;    // planes per block of the sweep kernel
   int sweep_gx = 0, sweep_gy = 0, sweep_blocks = 0;
@@ -269,6 +273,17 @@ struct hf_plan {
   size_t tail_lds_bytes = 0;
   bool coefficients_ready = false;
   bool has_activity = false;
+  // the published hierarchy applies z = D^-1 r (diagonal shortcut)
+  bool shortcut = false;
+  unsigned long long* d_ratio = nullptr;  // shortcut selection maximum
+  // FGMRES basis and reduction buffers (allocated by the first solve)
+  int fg_capacity = 0;
+  double* fg_v = nullptr;        // restart + 1 basis vectors
+  double* fg_z = nullptr;        // restart preconditioned vectors
+  double* fg_partials = nullptr;  // (restart + 2) accumulators
+  double* fg_folded = nullptr;
+  double* fg_scalars = nullptr;
+  double* fg_host = nullptr;      // pinned mirror of fg_scalars
   // raw activity masks (copies) for the adjoint
   uint8_t* fxa = nullptr;
   uint8_t* fya = nullptr;
@@ -298,6 +313,7 @@ struct hf_plan {
   double class_ms[kProfileClasses] = {};
   uint64_t class_launches[kProfileClasses] = {};
   std::vector<void*> allocations;
+  std::vector<size_t> allocation_bytes;
 };
 
 namespace {
@@ -355,9 +371,24 @@ hf_status dalloc(hf_plan* p, T** out, size_t count) {
     return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
   }
   p->allocations.push_back(ptr);
+  p->allocation_bytes.push_back(bytes);
   p->device_bytes += bytes;
   *out = static_cast<T*>(ptr);
   return ok();
+}
+
+// Frees one allocation of the plan (null is a no-op).
+void dfree(hf_plan* p, void* ptr) {
+  if (ptr == nullptr) return;
+  for (size_t i = 0; i < p->allocations.size(); ++i) {
+    if (p->allocations[i] != ptr) continue;
+    (void)hipFree(ptr);
+    p->device_bytes -= p->allocation_bytes[i];
+    p->allocations.erase(p->allocations.begin() + static_cast<long>(i));
+    p->allocation_bytes.erase(p->allocation_bytes.begin() +
+                              static_cast<long>(i));
+    return;
+  }
 }
 
 hf_status dalloc_raw(hf_plan* p, void** out, size_t bytes) {
@@ -785,10 +816,49 @@ void line_smooth(hf_plan* p, int l, int sweeps, int reverse) {
   }
 }
 
+// Chebyshev-Jacobi smoothing of level l (src/solver_mg.cpp:2916-3090):
+// `stages` Jacobi stages with the three-term recurrence over the spectrum
+// [2 fraction, 2] of D^-1 A, from the x the level holds (zero on a first
+// visit: the entry or the restriction stored it), then the retained defect.
+template <class T>
+void cheb_smooth(hf_plan* p, int l, int stages, int retain) {
+  using C = typename Compute<T>::type;
+  Level& L = p->levels[l];
+  const double upper = 2.0;
+  const double lower = p->policy.chebyshev_lower_fraction * upper;
+  const double theta = (upper + lower) * 0.5;
+  const double delta = (upper - lower) * 0.5;
+  const double sigma = theta / delta;
+  double previous_rho = 1.0 / sigma;
+  const int rb_class = l == 0 ? P_RB_FINEST : P_RB_COARSE;
+  for (int stage = 0; stage < stages; ++stage) {
+    const double rho = stage == 0 ? 0.0 : 1.0 / (2.0 * sigma - previous_rho);
+    const double pf = stage == 0 ? 0.0 : rho * previous_rho;
+    const double rf = stage == 0 ? 1.0 / theta : 2.0 * rho / delta;
+    if (stage != 0) previous_rho = rho;
+    {
+      HF_PROF(p, rb_class);
+      HF_LAUNCH(k_cheb_stage<T>, L.tz.blocks, kBlock, 0, p->stream, L.tz,
+                view<T>(L), p->bc, static_cast<const T*>(L.x),
+                static_cast<T*>(L.x2), static_cast<C>(rf), static_cast<C>(pf),
+                stage == 0 ? 1 : 0);
+    }
+    std::swap(L.x, L.x2);
+  }
+  if (retain) {
+    HF_PROF(p, P_DEFECT);
+    level_apply<T>(p, L, view<T>(L), 1);
+  }
+}
+
 template <class T>
 void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
                  int from_zero, double* mean_partials = nullptr) {
   Level& L = p->levels[l];
+  if (L.cheb) {
+    cheb_smooth<T>(p, l, sweeps, retain);
+    return;
+  }
   if (L.line_mask != 0) {
     // x starts from an explicit zero on line levels; the retained defect is
     // recomputed after line smoothing (compute_residual, :4256-4262)
@@ -863,7 +933,7 @@ void exec_grid_op(hf_plan* p, const CycleOp& op,
     // the retained defect and the restriction of the next op, in one pass
     Level& Cd = p->levels[op.level + 1];
     const bool proj = p->policy.null_space_constant != 0;
-    const bool into_tail = Cd.tail || Cd.line_mask != 0;
+    const bool into_tail = Cd.tail || Cd.line_mask != 0 || Cd.cheb;
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       dispatch(Cd.prec, [&](auto tc) {
@@ -891,7 +961,7 @@ void exec_grid_op(hf_plan* p, const CycleOp& op,
     // a point-smoothed grid level starts from zero and applies the
     // projection in its first pass; the tail and line levels need x = 0 and
     // a projected b up front
-    const bool into_tail = Cn.tail || Cn.line_mask != 0;
+    const bool into_tail = Cn.tail || Cn.line_mask != 0 || Cn.cheb;
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       dispatch(Cn.prec, [&](auto tc) {
@@ -974,7 +1044,7 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
   const bool proj = p->policy.null_space_constant != 0;
   // whole cycle inside the tail kernel, or a line level 0: x = 0 and the
   // projected b are stored before the cycle
-  const bool tail0 = L0.tail || L0.line_mask != 0;
+  const bool tail0 = L0.tail || L0.line_mask != 0 || L0.cheb;
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
@@ -995,8 +1065,20 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
     }
   });
   // the cycle's last fused sweep also sums the level-0 mean of its output
-  const bool sweep_mean = proj && mean_in_sweep(p);
-  run_cycle(p, sweep_mean ? p->partials : nullptr);
+  const bool sweep_mean = proj && !p->shortcut && mean_in_sweep(p);
+  if (p->shortcut) {
+    // z = D^-1 r (src/solver_mg.cpp:4926-4936); the shortcut is only
+    // selected without a null space
+    const Chunk ch = make_chunk(static_cast<int>(L0.cells));
+    dispatch(L0.prec, [&](auto t) {
+      using T = decltype(t);
+      HF_PROF(p, P_OTHER);
+      HF_LAUNCH(k_diag_solve<T>, chunk_blocks(ch), kBlock, 0, p->stream, ch,
+                view<T>(L0));
+    });
+  } else {
+    run_cycle(p, sweep_mean ? p->partials : nullptr);
+  }
   const double* alpha = nullptr;
   const double* mean = nullptr;
   dispatch(L0.prec, [&](auto t) {
@@ -1131,6 +1213,426 @@ int64_t reduction_doubles(const hf_plan* p) {
 
 }  // namespace
 
+// ---- FGMRES (src/solver_krylov.cpp:1557-2650) ------------------------------
+namespace {
+
+// Slot of fg_scalars that a non-finite basis normalisation sets.
+constexpr int kFgFlag = kFgMaxRestart + 2;
+constexpr int kFgSlots = kFgMaxRestart + 4;
+
+// Basis, preconditioned vectors and reduction buffers for restart R, kept
+// by the plan; a longer restart replaces them.
+hf_status fg_reserve(hf_plan* p, int restart) {
+  if (p->fg_capacity >= restart) return ok();
+  dfree(p, p->fg_v);
+  dfree(p, p->fg_z);
+  dfree(p, p->fg_partials);
+  dfree(p, p->fg_folded);
+  p->fg_v = p->fg_z = p->fg_partials = p->fg_folded = nullptr;
+  p->fg_capacity = 0;
+  const int64_t N = p->cells;
+  const int64_t blocks = p->levels[0].tz.blocks;
+  HF_TRY(dalloc(p, &p->fg_v, static_cast<size_t>((restart + 1) * N)));
+  HF_TRY(dalloc(p, &p->fg_z, static_cast<size_t>(restart * N)));
+  HF_TRY(dalloc(p, &p->fg_partials,
+                static_cast<size_t>((restart + 2) * blocks)));
+  HF_TRY(dalloc(p, &p->fg_folded,
+                static_cast<size_t>((restart + 2) * int64_t{kFoldStride})));
+  if (p->fg_scalars == nullptr) HF_TRY(dalloc(p, &p->fg_scalars, kFgSlots));
+  if (p->fg_host == nullptr &&
+      hipHostMalloc(reinterpret_cast<void**>(&p->fg_host),
+                    kFgSlots * sizeof(double), hipHostMallocDefault) !=
+          hipSuccess) {
+    p->fg_host = nullptr;
+    return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  }
+  p->fg_capacity = restart;
+  return ok();
+}
+
+// single_reduction_fgmres_norm (src/solver_krylov.cpp:565-604): the new
+// vector's norm from bar = w.w less the squared projections; 0 ok, 1 the
+// recurrence went negative (unsafe), -1 non-finite.
+int fg_norm_recurrence(const double* h, size_t count, double bar, double* t,
+                       double* next) {
+  if (!std::isfinite(bar)) return -1;
+  double v = bar;
+  for (size_t row = 0; row < count; ++row) {
+    const double projection = h[row];
+    if (!std::isfinite(projection)) return -1;
+    const double square = projection * projection;
+    if (!std::isfinite(square)) return -1;
+    v = v - square;
+    if (!std::isfinite(v)) return -1;
+  }
+  *t = v;
+  if (v < 0.0) {
+    *next = 0.0;
+    return 1;
+  }
+  *next = std::sqrt(v);
+  return std::isfinite(*next) ? 0 : -1;
+}
+
+// Back substitution of the rotated (upper triangular) Hessenberg columns,
+// column-major with `rows` entries per column; 0, or the termination.
+int fg_back_substitute(const std::vector<double>& h, size_t rows,
+                       const std::vector<double>& g, size_t count,
+                       std::vector<double>& y) {
+  for (size_t r = count; r > 0; --r) {
+    const size_t row = r - 1;
+    double value = g[row];
+    for (size_t upper = row + 1; upper < count; ++upper)
+      value -= h[upper * rows + row] * y[upper];
+    const double divisor = h[row * rows + row];
+    if (divisor == 0.0 || !std::isfinite(divisor)) return HF_TERM_BREAKDOWN;
+    y[row] = value / divisor;
+    if (!std::isfinite(y[row])) return HF_TERM_NON_FINITE;
+  }
+  return 0;
+}
+
+hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
+                      const hf_fgmres_control* ctl, hf_solve_result* res) {
+  const size_t R = ctl->restart;
+  const size_t rows = R + 1;
+  const int64_t N = p->cells;
+  const Lv<double> A = krylov_view(p);
+  const Level& L0 = p->levels[0];
+  const Tiles& tz = L0.tz;
+  const Chunk ch = p->chunk0;
+  const int nbv = chunk_blocks(ch);
+  const int vrows = ctl->volume_rows ? 1 : 0;
+  const Widths wd = widths(p);
+  const int64_t pst = tz.blocks;
+  double* x = p->kx;
+  double* ax = p->kq;
+  double* scaled = p->kr;
+  auto V = [&](size_t r) { return p->fg_v + static_cast<int64_t>(r) * N; };
+  auto Z = [&](size_t r) { return p->fg_z + static_cast<int64_t>(r) * N; };
+  std::vector<double> h(rows * R, 0.0), cs(R, 0.0), sn(R, 0.0), g(rows, 0.0),
+      y(R, 0.0);
+  FgScales scales{};
+  uint64_t restarts = 0;
+  const double eps = std::numeric_limits<double>::epsilon();
+  // a basis normalisation is checked for finiteness at the next read; a
+  // failing one reports the counts of the moment it was made
+  bool flag_pending = false;
+  uint64_t ops_at = 0, pcs_at = 0;
+  uint32_t it_at = 0;
+  auto finish = [&](hf_status st, int term) {
+    res->status = st;
+    res->termination = term;
+    res->norm_breakdown_restarts = restarts;
+    return st;
+  };
+  auto fail_term = [&](int code, int term) {
+    return finish(fail(code, term == HF_TERM_BREAKDOWN ? kDetailBreakdown
+                                                       : kDetailNonFinite),
+                  term);
+  };
+  // sums of `nacc` accumulators of fg_partials into fg_host; `bad` reports a
+  // pending non-finite normalisation
+  auto reduce = [&](int nacc, bool* bad) -> hf_status {
+    const double* src = p->fg_partials;
+    int64_t count = pst;
+    int64_t stride = pst;
+    if (pst > kFoldSpan) {
+      const int64_t spans = (pst + kFoldSpan - 1) / kFoldSpan;
+      hipLaunchKernelGGL(k_fold_partials,
+                         dim3(static_cast<uint32_t>(spans),
+                              static_cast<uint32_t>(nacc)),
+                         dim3(kBlock), 0, p->stream, p->fg_partials, pst, pst,
+                         p->fg_folded, static_cast<int64_t>(kFoldStride));
+      src = p->fg_folded;
+      count = spans;
+      stride = kFoldStride;
+    }
+    {
+      HF_PROF(p, P_FINALIZE);
+      HF_LAUNCH(k_finalize_many, 1, 1024, 0, p->stream, src, count, stride,
+                nacc, p->fg_scalars);
+    }
+    ++p->reductions;
+    if (hipMemcpyAsync(p->fg_host, p->fg_scalars, kFgSlots * sizeof(double),
+                       hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+      return fail(HF_INVALID_PLAN, kDetailHip);
+    *bad = false;
+    if (flag_pending) {
+      flag_pending = false;
+      if (p->fg_host[kFgFlag] != 0.0) {
+        res->operator_applies = ops_at;
+        res->preconditioner_applies = pcs_at;
+        res->iterations = it_at;
+        *bad = true;
+      }
+    }
+    return ok();
+  };
+  // out = b - A' vec and its norm (FG_TRUE_RESIDUAL); term != 0 on failure
+  auto true_residual = [&](const double* vec, double* out, double* norm,
+                           int* term) -> hf_status {
+    {
+      HF_PROF(p, P_TRUE_RESIDUAL);
+      HF_LAUNCH(k_fg_stencil<kFgResidual>, tz.blocks, kBlock, 0, p->stream,
+                tz, A, p->bc, vec, rhs, out, static_cast<const double*>(nullptr),
+                int64_t{0}, 0, 0, 0.0, vrows, wd, p->fg_partials, pst);
+    }
+    bool bad = false;
+    HF_TRY(reduce(1, &bad));
+    if (bad) {
+      *term = HF_TERM_NON_FINITE;
+      return ok();
+    }
+    res->operator_applies++;
+    *norm = std::sqrt(p->fg_host[0]);
+    *term = std::isfinite(*norm) ? 0 : HF_TERM_OPERATOR_FAILURE;
+    return ok();
+  };
+  // dst = src + sum s[row] basis_row (scale: then times post, checked)
+  auto combine = [&](double* dst, const double* src, const double* basis,
+                     const double* s, size_t count, double post, int scale) {
+    FgScales k{};
+    for (size_t r = 0; r < count; ++r) k.s[r] = s[r];
+    HF_PROF(p, P_OTHER);
+    HF_LAUNCH(k_fg_combine, nbv, kBlock, 0, p->stream, ch, dst, src, basis, N,
+              k, static_cast<int>(count), post, scale,
+              p->fg_scalars + kFgFlag);
+    if (scale) {
+      flag_pending = true;
+      ops_at = res->operator_applies;
+      pcs_at = res->preconditioner_applies;
+      it_at = res->iterations;
+    }
+  };
+  // dots basis_row . wv for rows [0, count) into accumulators [acc0, ...)
+  auto basis_dots = [&](const double* wv, size_t row0, size_t count,
+                        int acc0) {
+    for (size_t r = row0; r < row0 + count; r += kFgDots) {
+      const int nd = static_cast<int>(std::min<size_t>(kFgDots, row0 + count - r));
+      HF_LAUNCH(k_fg_dots, tz.blocks, kBlock, 0, p->stream, tz, wv, p->fg_v, N,
+                static_cast<int>(r), nd, acc0 + static_cast<int>(r - row0),
+                p->fg_partials, pst);
+    }
+  };
+  auto norm_of = [&](const double* wv, double* norm, bool* bad) -> hf_status {
+    HF_LAUNCH(k_fg_dots, tz.blocks, kBlock, 0, p->stream, tz, wv, wv,
+              int64_t{0}, 0, 1, 0, p->fg_partials, pst);
+    HF_TRY(reduce(1, bad));
+    *norm = std::sqrt(p->fg_host[0]);
+    return ok();
+  };
+  auto publish = [&](const double* from) {
+    HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, solution, from);
+  };
+
+  // ||b||
+  HF_LAUNCH(k_dot, nbv, kBlock, 0, p->stream, ch, rhs, rhs, p->partials);
+  finalize(p, nbv, 1, F_STORE, S_SUM);
+  HF_TRY(read_scalars(p));
+  const double rhs_norm = std::sqrt(p->host_scalars[S_SUM]);
+  res->rhs_norm = rhs_norm;
+  if (!std::isfinite(rhs_norm))
+    return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+  const double tol =
+      std::max(ctl->absolute_tolerance, ctl->relative_tolerance * rhs_norm);
+  res->true_residual_limit = tol;
+  if (rhs_norm == 0.0) {
+    HF_LAUNCH(k_fill, nbv, kBlock, 0, p->stream, ch, solution, 0.0);
+    return finish(ok(), HF_TERM_ZERO_RHS);
+  }
+  {
+    const Chunk sc = make_chunk(kFgSlots);
+    HF_LAUNCH(k_fill, chunk_blocks(sc), kBlock, 0, p->stream, sc,
+              p->fg_scalars, 0.0);
+  }
+  HF_LAUNCH(k_copy, nbv, kBlock, 0, p->stream, ch, x, solution);
+  int term = 0;
+  HF_TRY(true_residual(x, V(0), &res->initial_true_residual, &term));
+  if (term) return fail_term(HF_NUMERICAL_FAILURE, term);
+  res->final_true_residual = res->initial_true_residual;
+  res->recursive_residual = res->initial_true_residual;
+  if (!std::isfinite(tol))
+    return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+  if (res->initial_true_residual <= tol) {
+    publish(x);
+    return finish(ok(), HF_TERM_CONVERGED);
+  }
+  while (res->iterations < ctl->maximum_iterations) {
+    const double beta = res->final_true_residual;
+    combine(V(0), V(0), nullptr, nullptr, 0, 1.0 / beta, 1);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    bool restarted = false;
+    for (size_t col = 0;
+         col < R && res->iterations < ctl->maximum_iterations; ++col) {
+      // z_col = M v_col (M applied to V v_col with volume rows)
+      const double* src = V(col);
+      if (vrows) {
+        HF_PROF(p, P_OTHER);
+        HF_LAUNCH(k_scale_volume, L0.tc.blocks, kBlock, 0, p->stream, L0.tc,
+                  scaled, static_cast<const double*>(V(col)), wd);
+        src = scaled;
+      }
+      HF_TRY(mg_apply(p, src, Z(col), nullptr, false));
+      res->preconditioner_applies++;
+      if (L0.line_mask != 0) {
+        const hf_status ls = read_scalars(p);
+        if (ls.code != HF_OK)
+          return finish(ls, ls.detail == kDetailLine
+                                ? HF_TERM_PRECONDITIONER_FAILURE
+                                : HF_TERM_INVALID_PLAN);
+      }
+      // w = A' z_col - shift v_col, its projections on v_0..v_col and w.w
+      const size_t dots = col + 1;
+      {
+        HF_PROF(p, P_OPERATOR);
+        HF_LAUNCH(k_fg_stencil<kFgArnoldi>, tz.blocks, kBlock, 0, p->stream,
+                  tz, A, p->bc, static_cast<const double*>(Z(col)),
+                  static_cast<const double*>(V(col)), ax,
+                  static_cast<const double*>(p->fg_v), N,
+                  static_cast<int>(std::min<size_t>(dots, kFgDots)),
+                  static_cast<int>(dots), 1.0, vrows, wd, p->fg_partials, pst);
+        if (dots > static_cast<size_t>(kFgDots))
+          basis_dots(ax, kFgDots, dots - kFgDots, kFgDots);
+      }
+      bool bad = false;
+      HF_TRY(reduce(static_cast<int>(dots) + 1, &bad));
+      if (bad) return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+      res->operator_applies++;
+      double* hc = h.data() + col * rows;
+      for (size_t row = 0; row <= dots; ++row) hc[row] = p->fg_host[row];
+      for (size_t row = 0; row <= dots; ++row)
+        if (!std::isfinite(hc[row]))
+          return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+      const double bar = hc[dots];
+      for (size_t row = 0; row < dots; ++row) scales.s[row] = -hc[row];
+      double t = 0.0, next_norm = 0.0;
+      const int nst = fg_norm_recurrence(hc, dots, bar, &t, &next_norm);
+      if (nst < 0) return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+      const bool unsafe = nst == 1;
+      if (unsafe && col != 0) {
+        // discard the column, publish the shorter cycle and restart
+        // (src/solver_krylov.cpp:2232-2340)
+        res->iterations++;
+        const int bt = fg_back_substitute(h, rows, g, col, y);
+        if (bt) return fail_term(HF_NUMERICAL_FAILURE, bt);
+        combine(x, x, p->fg_z, y.data(), col, 1.0, 0);
+        HF_TRY(true_residual(x, V(0), &res->final_true_residual, &term));
+        if (term) return fail_term(HF_NUMERICAL_FAILURE, term);
+        res->recursive_residual = res->final_true_residual;
+        if (res->final_true_residual <= tol) {
+          publish(x);
+          return finish(ok(), HF_TERM_CONVERGED);
+        }
+        if (res->iterations == ctl->maximum_iterations)
+          return finish(fail(HF_REJECTED_STEP, kDetailBreakdown),
+                        HF_TERM_MAXIMUM_ITERATIONS);
+        ++restarts;
+        std::fill(h.begin(), h.end(), 0.0);
+        std::fill(cs.begin(), cs.end(), 0.0);
+        std::fill(sn.begin(), sn.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        std::fill(y.begin(), y.end(), 0.0);
+        restarted = true;
+        break;
+      }
+      bool explicit_pass = false;
+      if (unsafe || next_norm == 0.0) {
+        // second orthogonalisation pass (src/solver_krylov.cpp:2342-2400)
+        combine(ax, ax, p->fg_v, scales.s, dots, 1.0, 0);
+        basis_dots(ax, 0, dots, 0);
+        HF_TRY(reduce(static_cast<int>(dots), &bad));
+        if (bad) return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+        for (size_t row = 0; row < dots; ++row) {
+          const double yr = p->fg_host[row];
+          hc[row] += yr;
+          if (!std::isfinite(hc[row]))
+            return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+          scales.s[row] = -yr;
+        }
+        combine(ax, ax, p->fg_v, scales.s, dots, 1.0, 0);
+        HF_TRY(norm_of(ax, &next_norm, &bad));
+        if (bad || !std::isfinite(next_norm))
+          return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+        explicit_pass = true;
+      }
+      hc[col] += 1.0;  // the Arnoldi shift
+      if (!std::isfinite(hc[col]))
+        return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
+      hc[col + 1] = next_norm;
+      for (size_t row = 0; row < col; ++row) {
+        const double upper = hc[row], lower = hc[row + 1];
+        const double rotated = cs[row] * upper + sn[row] * lower;
+        hc[row + 1] = -sn[row] * upper + cs[row] * lower;
+        hc[row] = rotated;
+      }
+      const double rn = std::hypot(hc[col], hc[col + 1]);
+      if (!(rn > 0.0) || !std::isfinite(rn))
+        return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_BREAKDOWN);
+      cs[col] = hc[col] / rn;
+      sn[col] = hc[col + 1] / rn;
+      hc[col] = rn;
+      hc[col + 1] = 0.0;
+      g[col + 1] = -sn[col] * g[col];
+      g[col] = cs[col] * g[col];
+      const double recursive = std::fabs(g[col + 1]);
+      res->recursive_residual = recursive;
+      res->iterations++;
+      const bool happy = next_norm == 0.0;
+      const bool cycle_end = happy || col + 1 == R ||
+                             res->iterations == ctl->maximum_iterations;
+      const bool verify = happy || recursive <= tol ||
+                          res->iterations % ctl->true_residual_interval == 0 ||
+                          cycle_end;
+      if (!cycle_end)
+        combine(V(col + 1), ax, p->fg_v, scales.s, explicit_pass ? 0 : dots,
+                1.0 / next_norm, 1);
+      if (!verify) continue;
+      const int bt = fg_back_substitute(h, rows, g, dots, y);
+      if (bt) return fail_term(HF_NUMERICAL_FAILURE, bt);
+      if (!cycle_end) {
+        // audit a trial solution in the unused last basis slot
+        double* trial = V(R);
+        combine(trial, x, p->fg_z, y.data(), dots, 1.0, 0);
+        double audited = 0.0;
+        HF_TRY(true_residual(trial, ax, &audited, &term));
+        if (term) return fail_term(HF_NUMERICAL_FAILURE, term);
+        res->final_true_residual = audited;
+        if (audited <= tol) {
+          publish(trial);
+          return finish(ok(), HF_TERM_CONVERGED);
+        }
+        continue;
+      }
+      combine(x, x, p->fg_z, y.data(), dots, 1.0, 0);
+      HF_TRY(true_residual(x, V(0), &res->final_true_residual, &term));
+      if (term) return fail_term(HF_NUMERICAL_FAILURE, term);
+      if (res->final_true_residual <= tol) {
+        publish(x);
+        return finish(ok(), HF_TERM_CONVERGED);
+      }
+      if (res->iterations == ctl->maximum_iterations)
+        return finish(fail(HF_REJECTED_STEP, kDetailBreakdown),
+                      HF_TERM_MAXIMUM_ITERATIONS);
+      if (happy) {
+        const double guard = 64.0 * eps * std::max(beta, tol);
+        if (!(res->final_true_residual + guard < beta))
+          return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_BREAKDOWN);
+        ++restarts;
+      }
+      restarted = true;
+      break;
+    }
+    if (!restarted) break;
+  }
+  return finish(fail(HF_REJECTED_STEP, kDetailBreakdown),
+                HF_TERM_MAXIMUM_ITERATIONS);
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1150,6 +1652,9 @@ hf_status hf_policy_default(hf_policy* policy) {
   policy->precision = HF_PRECISION_FP64;
   policy->coarse_face_scaling = 0;
   policy->line_order = 0;
+  policy->point_smoother = 0;
+  policy->chebyshev_lower_fraction = 0.3;
+  policy->diagonal_shortcut_ratio = 0.0;
   return ok();
 }
 
@@ -1171,7 +1676,12 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       policy->coarse_sweeps == 0 || policy->maximum_levels < 2 ||
       policy->maximum_levels > kMaxLevels || policy->minimum_coarse_extent < 2 ||
       !(policy->anisotropy_threshold > 1.0) || policy->precision > 2 ||
-      policy->coarse_face_scaling > 1 || policy->line_order > 1)
+      policy->coarse_face_scaling > 1 || policy->line_order > 1 ||
+      policy->point_smoother > 1 ||
+      !(policy->chebyshev_lower_fraction > 0.0) ||
+      !(policy->chebyshev_lower_fraction < 1.0) ||
+      !(policy->diagonal_shortcut_ratio >= 0.0) ||
+      !(policy->diagonal_shortcut_ratio < 1.0))
     return fail(HF_INVALID_PLAN, kDetailPlan);
   {
     // every kernel indexes cells and faces with 32-bit integers
@@ -1231,8 +1741,9 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   int64_t lds_doubles = 0;
   // line levels (every level of a line plan) are smoothed by the line
   // kernels, never in the tail
-  for (int l = static_cast<int>(p->levels.size()) - 1; l >= 0 && !line_plan;
-       --l) {
+  const bool cheb_plan = policy->point_smoother == 1;
+  for (int l = static_cast<int>(p->levels.size()) - 1;
+       l >= 0 && !line_plan && !cheb_plan; --l) {
     const int64_t need = lds_doubles + 3 * p->levels[l].cells;
     if (p->levels[l].cells <= threshold &&
         need * static_cast<int64_t>(sizeof(double)) <= kTailLdsBytes) {
@@ -1253,6 +1764,9 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     if (L.tail) tail_doubles += 3 * L.cells;
     const bool coarsest = l + 1 == p->levels.size();
     L.prec = (L.tail || coarsest) ? HF_PRECISION_FP64 : policy->precision;
+    // Chebyshev-Jacobi on every point level but the coarsest
+    // (src/solver_mg.cpp:1048-1062)
+    L.cheb = cheb_plan && !coarsest && L.line_mask == 0;
     L.tc = make_tiles(L.n[0], L.n[1], L.n[2], false);
     L.tcol = make_tiles((L.n[0] + 1) / 2, L.n[1], L.n[2], false);
     L.tr = make_tiles(L.n[0], L.n[1], L.n[2], true);
@@ -1279,7 +1793,7 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
                   static_cast<int64_t>(L.n[0]) * L.n[1] * (L.n[2] + 1)});
     // (each lane owns a pair of cells of a row: nx even)
     L.sweep = (l == 0 || L.cells >= sweep_min_cells()) && !L.tail &&
-              L.line_mask == 0 && L.fused &&
+              L.line_mask == 0 && L.fused && !L.cheb &&
               L.n[0] >= 32 && L.n[0] % 2 == 0 && L.n[1] >= 8 &&
               sweep_enabled() &&
               widest * static_cast<int64_t>(elem_size(L.prec)) <
@@ -1321,6 +1835,7 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       guard(dalloc_raw(p, &L.x2, es * L.cells));
       guard(dalloc_raw(p, &L.b2, es * L.cells));
     }
+    if (L.cheb) guard(dalloc_raw(p, &L.x2, es * L.cells));
     if (L.line_mask != 0) {
       const size_t cs = L.prec == HF_PRECISION_FP64 ? 8 : 4;
       guard(dalloc_raw(p, &L.lcw, cs * L.cells));
@@ -1344,6 +1859,7 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   guard(dalloc(p, &p->folded, 3 * kFoldStride));
   guard(dalloc(p, &p->scalars, S_COUNT));
   guard(dalloc(p, &p->d_flag, 4));
+  guard(dalloc(p, &p->d_ratio, 1));
   guard(dalloc(p, &p->kx, p->cells));
   guard(dalloc(p, &p->kr, p->cells));
   guard(dalloc(p, &p->kp, p->cells));
@@ -1410,6 +1926,115 @@ hf_status hf_assemble_coefficients(hf_plan* p, const double* mobility,
   return hip_check(hipGetLastError());
 }
 
+hf_status hf_assemble_pressure_system(
+    hf_plan* p, const double* bx, const double* by, const double* bz,
+    const double* phi_x, const double* phi_y, const double* phi_z,
+    const double* rho, const double* rho_n, const double* rho_nm1,
+    const double* drho_dp, const double* source, double a0, double a1,
+    double a2, double* diag, double* rhs) {
+  if (p == nullptr || bx == nullptr || by == nullptr || bz == nullptr ||
+      phi_x == nullptr || phi_y == nullptr || phi_z == nullptr ||
+      rho == nullptr || rho_n == nullptr || drho_dp == nullptr ||
+      diag == nullptr || rhs == nullptr)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  if (!(a0 > 0.0) || !std::isfinite(a0) || !std::isfinite(a1) ||
+      !std::isfinite(a2))
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Level& L = p->levels[0];
+  PressureSystem s{};
+  s.nx = p->n[0];
+  s.ny = p->n[1];
+  s.nz = p->n[2];
+  s.w = widths(p);
+  s.bx = bx;
+  s.by = by;
+  s.bz = bz;
+  s.phix = phi_x;
+  s.phiy = phi_y;
+  s.phiz = phi_z;
+  s.rho = rho;
+  s.rho_n = rho_n;
+  s.rho_nm1 = rho_nm1;
+  s.drho_dp = drho_dp;
+  s.source = source;
+  s.a0 = a0;
+  s.a1 = a1;
+  s.a2 = a2;
+  s.diag = diag;
+  s.rhs = rhs;
+  // the preflight checks everything before anything is written
+  HF_TRY(hip_check(hipMemsetAsync(p->d_flag, 0, sizeof(int), p->stream)));
+  const double* beta[3] = {bx, by, bz};
+  const double* phi[3] = {phi_x, phi_y, phi_z};
+  const int64_t counts[3] = {L.nfx, L.nfy, L.nfz};
+  for (int a = 0; a < 3; ++a)
+    HF_LAUNCH(k_check_faces, blocks_for(counts[a]), kBlock, 0, p->stream,
+              beta[a], phi[a], counts[a], p->d_flag);
+  HF_LAUNCH(k_pressure_system, blocks_for(L.cells), kBlock, 0, p->stream, s, 0,
+            p->d_flag);
+  int flag = 0;
+  HF_TRY(hip_check(hipMemcpyAsync(&flag, p->d_flag, sizeof(int),
+                                  hipMemcpyDeviceToHost, p->stream)));
+  HF_TRY(hip_check(hipStreamSynchronize(p->stream)));
+  if (flag != 0) return fail(HF_NUMERICAL_FAILURE, kDetailAssembly);
+  HF_LAUNCH(k_pressure_system, blocks_for(L.cells), kBlock, 0, p->stream, s, 1,
+            p->d_flag);
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_cold_coefficients(hf_plan* p, const double* proj_x,
+                               const double* proj_y, const double* proj_z,
+                               const double* lim_x, const double* lim_y,
+                               const double* lim_z, const uint8_t* fluid,
+                               const double* rho, const double* pressure,
+                               double pressure_reference, double dt,
+                               int32_t fixed_eos, const int32_t* kinds,
+                               double* diag, double* cx, double* cy,
+                               double* cz) {
+  if (p == nullptr || proj_x == nullptr || proj_y == nullptr ||
+      proj_z == nullptr || lim_x == nullptr || lim_y == nullptr ||
+      lim_z == nullptr || kinds == nullptr || diag == nullptr ||
+      cx == nullptr || cy == nullptr || cz == nullptr ||
+      (!fixed_eos && (rho == nullptr || pressure == nullptr)))
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(pressure_reference))
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  ColdSystem s{};
+  for (int f = 0; f < 6; ++f) {
+    if (kinds[f] < 0 || kinds[f] > 2) return fail(HF_INVALID_PLAN, kDetailPlan);
+    s.kind[f] = static_cast<int8_t>(kinds[f]);
+  }
+  for (int a = 0; a < 3; ++a)
+    if ((kinds[2 * a] == HF_COLD_PERIODIC) !=
+        (kinds[2 * a + 1] == HF_COLD_PERIODIC))
+      return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Level& L = p->levels[0];
+  for (int a = 0; a < 3; ++a) s.n[a] = p->n[a];
+  s.w = widths(p);
+  s.proj[0] = proj_x;
+  s.proj[1] = proj_y;
+  s.proj[2] = proj_z;
+  s.lim[0] = lim_x;
+  s.lim[1] = lim_y;
+  s.lim[2] = lim_z;
+  s.fluid = fluid;
+  s.rho = rho;
+  s.pressure = pressure;
+  s.pressure_reference = pressure_reference;
+  s.dt = dt;
+  s.fixed_eos = fixed_eos != 0 ? 1 : 0;
+  s.diag = diag;
+  s.coef[0] = cx;
+  s.coef[1] = cy;
+  s.coef[2] = cz;
+  const int64_t counts[3] = {L.nfx, L.nfy, L.nfz};
+  for (int a = 0; a < 3; ++a)
+    HF_LAUNCH(k_cold_faces, blocks_for(counts[a]), kBlock, 0, p->stream, s, a,
+              counts[a]);
+  HF_LAUNCH(k_cold_diag, blocks_for(L.cells), kBlock, 0, p->stream, s);
+  return hip_check(hipGetLastError());
+}
+
 hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
                                    const double* bx, const double* by,
                                    const double* bz, const uint8_t* ca,
@@ -1462,9 +2087,36 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
   f.vol = K0.vol;
   f.act = K0.act;
   HF_LAUNCH(k_mask_cells, blocks_for(L0.cells), kBlock, 0, p->stream, f);
-  // coarse levels
   HF_TRY(hip_check(hipMemsetAsync(p->d_flag, 0, sizeof(int), p->stream)));
-  for (size_t l = 1; l < p->levels.size(); ++l) {
+  // diagonal shortcut (src/solver_mg.cpp:4545-4575): the finest level is
+  // validated first, then its largest row ratio decides; a shortcut plan
+  // never builds or reads the coarse hierarchy
+  bool shortcut = false;
+  if (p->policy.diagonal_shortcut_ratio != 0.0 &&
+      !p->policy.null_space_constant) {
+    HF_LAUNCH(k_validate, blocks_for(L0.cells), kBlock, 0, p->stream, L0.n[0],
+              L0.n[1], L0.n[2], K0.D64, K0.fx64, K0.fy64, K0.fz64, K0.vol,
+              p->d_flag);
+    HF_TRY(hip_check(hipMemsetAsync(p->d_ratio, 0, sizeof(unsigned long long),
+                                    p->stream)));
+    HF_LAUNCH(k_row_ratio_max, blocks_for(L0.cells), kBlock, 0, p->stream,
+              L0.n[0], L0.n[1], L0.n[2], K0.D64, K0.fx64, K0.fy64, K0.fz64,
+              p->d_ratio);
+    int flag0 = 0;
+    unsigned long long bits = 0;
+    HF_TRY(hip_check(hipMemcpyAsync(&flag0, p->d_flag, sizeof(int),
+                                    hipMemcpyDeviceToHost, p->stream)));
+    HF_TRY(hip_check(hipMemcpyAsync(&bits, p->d_ratio, sizeof bits,
+                                    hipMemcpyDeviceToHost, p->stream)));
+    HF_TRY(hip_check(hipStreamSynchronize(p->stream)));
+    if (flag0 != 0) return fail(HF_NUMERICAL_FAILURE, kDetailCoefficients);
+    double ratio = 0.0;
+    std::memcpy(&ratio, &bits, sizeof ratio);
+    shortcut = ratio <= p->policy.diagonal_shortcut_ratio;
+  }
+  const size_t built = shortcut ? 1 : p->levels.size();
+  // coarse levels
+  for (size_t l = 1; l < built; ++l) {
     const Level& F = p->levels[l - 1];
     const Level& C = p->levels[l];
     const Coeffs& KF = *target[l - 1];
@@ -1488,13 +2140,13 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
     HF_LAUNCH(k_coarse_faces, blocks_for(C.nfz), kBlock, 0, p->stream, a, 2, KF.fz64, KC.fz64, sz);
     HF_LAUNCH(k_coarse_diag, blocks_for(C.cells), kBlock, 0, p->stream, C.n[0], C.n[1], C.n[2], coarse, p->d_flag);
   }
-  for (size_t l = 0; l < p->levels.size(); ++l) {
+  for (size_t l = 0; l < built; ++l) {
     const Level& L = p->levels[l];
     const Coeffs& K = *target[l];
     HF_LAUNCH(k_validate, blocks_for(L.cells), kBlock, 0, p->stream, L.n[0], L.n[1], L.n[2], K.D64, K.fx64, K.fy64, K.fz64, K.vol, p->d_flag);
   }
   // storage-precision copies
-  for (size_t l = 0; l < p->levels.size(); ++l) {
+  for (size_t l = 0; l < built; ++l) {
     const Level& L = p->levels[l];
     const Coeffs& K = *target[l];
     if (L.prec == HF_PRECISION_FP64) continue;
@@ -1507,7 +2159,7 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
     });
   }
   // projection volumes (sum of fluid volumes per level)
-  for (size_t l = 0; l < p->levels.size(); ++l) {
+  for (size_t l = 0; l < built; ++l) {
     const Level& L = p->levels[l];
     Coeffs& K = *target[l];
     const Chunk ch = make_chunk(static_cast<int>(L.cells));
@@ -1538,6 +2190,7 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
     HF_TRY(hip_check(hipMemcpyAsync(p->fza, fza, L0.nfz, hipMemcpyDeviceToDevice, p->stream)));
   }
   p->has_activity = all;
+  p->shortcut = shortcut;
   if (into_spare)
     for (Level& L : p->levels) std::swap(static_cast<Coeffs&>(L), L.spare);
   // the tail kernel reads its level descriptors from device memory; a plan
@@ -1574,8 +2227,8 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   const Tiles& tr = p->levels[0].tz;  // z-marching Krylov stencils
   // the PCG update stores the preconditioner's entry when level 0 is a grid
   // level (the tail and line-relaxed entries also zero x and project b)
-  const bool fuse_entry =
-      !p->levels[0].tail && p->levels[0].line_mask == 0;
+  const bool fuse_entry = !p->levels[0].tail &&
+                          p->levels[0].line_mask == 0 && !p->levels[0].cheb;
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
   double* x = p->kx;
@@ -1897,6 +2550,57 @@ hf_status hf_pcg_solve_static(hf_plan* p, const double* rhs, double* solution,
   p->profiling = profiling;
   const hipError_t last = hipGetLastError();
   if (st.code == HF_OK && (synced != hipSuccess || last != hipSuccess)) {
+    res->status = fail(HF_INVALID_PLAN, kDetailHip);
+    return res->status;
+  }
+  return st;
+}
+
+hf_status hf_fgmres_solve(hf_plan* p, const double* rhs, double* solution,
+                          const hf_fgmres_control* ctl,
+                          hf_solve_result* res) {
+  if (res != nullptr) std::memset(res, 0, sizeof *res);
+  if (p == nullptr || rhs == nullptr || solution == nullptr || ctl == nullptr ||
+      res == nullptr || !p->coefficients_ready)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  if (!std::isfinite(ctl->absolute_tolerance) ||
+      !std::isfinite(ctl->relative_tolerance) || ctl->absolute_tolerance < 0.0 ||
+      ctl->relative_tolerance < 0.0 ||
+      (ctl->absolute_tolerance == 0.0 && ctl->relative_tolerance == 0.0) ||
+      ctl->maximum_iterations == 0 || ctl->true_residual_interval == 0 ||
+      ctl->restart == 0 || ctl->restart > static_cast<uint32_t>(kFgMaxRestart)) {
+    res->status = fail(HF_INVALID_PLAN, kDetailPlan);
+    res->termination = HF_TERM_INVALID_PLAN;
+    return res->status;
+  }
+  {
+    const hf_status st = fg_reserve(p, static_cast<int>(ctl->restart));
+    if (st.code != HF_OK) {
+      res->status = st;
+      res->termination = HF_TERM_INVALID_PLAN;
+      return st;
+    }
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, p->stream);
+  hf_status st = fgmres_impl(p, rhs, solution, ctl, res);
+  if (st.code != HF_OK && res->termination == HF_TERM_CONVERGED) {
+    res->status = st;
+    res->termination = st.detail == kDetailLine ? HF_TERM_PRECONDITIONER_FAILURE
+                                                : HF_TERM_INVALID_PLAN;
+  }
+  (void)hipEventRecord(e1, p->stream);
+  (void)hipEventSynchronize(e1);
+  profile_collect(p);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  res->solve_milliseconds = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const hipError_t last = hipGetLastError();
+  if (st.code == HF_OK && last != hipSuccess) {
     res->status = fail(HF_INVALID_PLAN, kDetailHip);
     return res->status;
   }

@@ -36,9 +36,10 @@ PROFILE_CLASSES = ["rb_finest", "rb_coarse", "defect", "restrict", "prolong",
 
 ABI_FUNCTIONS = [
     "hf_policy_default", "hf_plan_create", "hf_plan_destroy",
-    "hf_assemble_coefficients", "hf_plan_set_coefficients",
+    "hf_assemble_coefficients", "hf_assemble_pressure_system",
+    "hf_cold_coefficients", "hf_plan_set_coefficients",
     "hf_operator_apply", "hf_mg_apply", "hf_pcg_solve",
-    "hf_pcg_solve_static", "hf_adjoint_gradients", "hf_project_mean", "hf_plan_level_count",
+    "hf_pcg_solve_static", "hf_fgmres_solve", "hf_adjoint_gradients", "hf_project_mean", "hf_plan_level_count",
     "hf_plan_level_info", "hf_plan_level_array", "hf_plan_device_bytes",
     "hf_plan_set_profiling", "hf_plan_profile", "hf_status_message",
 ]
@@ -69,7 +70,10 @@ class hf_policy(ctypes.Structure):
                 ("unit_linear", ctypes.c_uint8),
                 ("precision", ctypes.c_uint8),
                 ("coarse_face_scaling", ctypes.c_uint8),
-                ("line_order", ctypes.c_uint8)]
+                ("line_order", ctypes.c_uint8),
+                ("point_smoother", ctypes.c_uint8),
+                ("chebyshev_lower_fraction", ctypes.c_double),
+                ("diagonal_shortcut_ratio", ctypes.c_double)]
 
 
 class hf_solve_control(ctypes.Structure):
@@ -78,6 +82,15 @@ class hf_solve_control(ctypes.Structure):
                 ("maximum_iterations", ctypes.c_uint32),
                 ("true_residual_interval", ctypes.c_uint32),
                 ("flexible", ctypes.c_uint8)]
+
+
+class hf_fgmres_control(ctypes.Structure):
+    _fields_ = [("absolute_tolerance", ctypes.c_double),
+                ("relative_tolerance", ctypes.c_double),
+                ("maximum_iterations", ctypes.c_uint32),
+                ("true_residual_interval", ctypes.c_uint32),
+                ("restart", ctypes.c_uint32),
+                ("volume_rows", ctypes.c_uint8)]
 
 
 class hf_solve_result(ctypes.Structure):
@@ -92,7 +105,8 @@ class hf_solve_result(ctypes.Structure):
                 ("operator_applies", ctypes.c_uint64),
                 ("preconditioner_applies", ctypes.c_uint64),
                 ("reduction_calls", ctypes.c_uint64),
-                ("solve_milliseconds", ctypes.c_double)]
+                ("solve_milliseconds", ctypes.c_double),
+                ("norm_breakdown_restarts", ctypes.c_uint64)]
 
 
 class hf_level_info(ctypes.Structure):
@@ -138,12 +152,21 @@ def load_library(path: str = LIBRARY_PATH):
         "hf_plan_destroy": [P],
         "hf_assemble_coefficients": [P, D, D, D, D, D],
         "hf_plan_set_coefficients": [P, D, D, D, D, D, D, D, D],
+        "hf_assemble_pressure_system": [P, D, D, D, D, D, D, D, D, D, D, D,
+                                        ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, D, D],
+        "hf_cold_coefficients": [P, D, D, D, D, D, D, D, D, D,
+                                 ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int32,
+                                 ctypes.POINTER(ctypes.c_int32), D, D, D, D],
         "hf_operator_apply": [P, D, D],
         "hf_mg_apply": [P, D, D],
         "hf_pcg_solve": [P, D, D, ctypes.POINTER(hf_solve_control),
                          ctypes.POINTER(hf_solve_result)],
         "hf_pcg_solve_static": [P, D, D, ctypes.POINTER(hf_solve_control),
                                 ctypes.POINTER(hf_solve_result)],
+        "hf_fgmres_solve": [P, D, D, ctypes.POINTER(hf_fgmres_control),
+                            ctypes.POINTER(hf_solve_result)],
         "hf_adjoint_gradients": [P, D, D, D, ctypes.POINTER(hf_solve_control),
                                  D, D, ctypes.POINTER(hf_solve_result)],
         "hf_project_mean": [P, D],
@@ -198,6 +221,7 @@ class SolveResult:
     preconditioner_applies: int
     reduction_calls: int
     solve_milliseconds: float
+    norm_breakdown_restarts: int = 0
 
     @classmethod
     def from_c(cls, r: hf_solve_result) -> "SolveResult":
@@ -208,7 +232,7 @@ class SolveResult:
                    r.final_true_residual, r.recursive_residual, r.rhs_norm,
                    r.true_residual_limit, r.operator_applies,
                    r.preconditioner_applies, r.reduction_calls,
-                   r.solve_milliseconds)
+                   r.solve_milliseconds, r.norm_breakdown_restarts)
 
 
 def _ptr(t):
@@ -278,6 +302,39 @@ class Plan:
                                                  _ptr(bz)))
         return diag, bx, by, bz
 
+    def assemble_pressure_system(self, bx, by, bz, phi, rho, rho_n, drho_dp,
+                                 a, rho_nm1=None, source=None):
+        """D and b of the pressure-correction system
+        (assemble_pressure_system_impl); phi = (phi_x, phi_y, phi_z) face
+        fluxes, a = (a0, a1, a2) BDF coefficients."""
+        diag = self._f64(self.cells)
+        rhs = self._f64(self.cells)
+        _check("hf_assemble_pressure_system",
+               self.lib.hf_assemble_pressure_system(
+                   self.handle, _ptr(bx), _ptr(by), _ptr(bz), _ptr(phi[0]),
+                   _ptr(phi[1]), _ptr(phi[2]), _ptr(rho), _ptr(rho_n),
+                   _ptr(rho_nm1), _ptr(drho_dp), _ptr(source), float(a[0]),
+                   float(a[1]), float(a[2]), _ptr(diag), _ptr(rhs)))
+        return diag, rhs
+
+    def cold_coefficients(self, proj, lim, fluid, rho, pressure,
+                          pressure_reference, dt, kinds, fixed_eos=False):
+        """(D, cx, cy, cz) of the cold-path pressure coefficient loop;
+        proj, lim = per-axis face arrays, kinds = six side kinds (0 other,
+        1 periodic, 2 pressure outlet)."""
+        fx, fy, fz = self.face_counts()
+        diag = self._f64(self.cells)
+        c = [self._f64(fx), self._f64(fy), self._f64(fz)]
+        k = (ctypes.c_int32 * 6)(*[int(v) for v in kinds])
+        _check("hf_cold_coefficients",
+               self.lib.hf_cold_coefficients(
+                   self.handle, _ptr(proj[0]), _ptr(proj[1]), _ptr(proj[2]),
+                   _ptr(lim[0]), _ptr(lim[1]), _ptr(lim[2]), _ptr(fluid),
+                   _ptr(rho), _ptr(pressure), float(pressure_reference),
+                   float(dt), 1 if fixed_eos else 0, k, _ptr(diag),
+                   _ptr(c[0]), _ptr(c[1]), _ptr(c[2])))
+        return diag, c[0], c[1], c[2]
+
     def set_coefficients(self, diag, bx, by, bz, cell_active=None,
                          x_active=None, y_active=None, z_active=None):
         _check("hf_plan_set_coefficients",
@@ -325,6 +382,30 @@ class Plan:
         result = SolveResult.from_c(res)
         if check:
             _check(name, status)
+        return x, result
+
+    def solve_fgmres(self, rhs, solution=None, check=True, rtol=1e-10,
+                     atol=0.0, maximum_iterations=400,
+                     true_residual_interval=4, restart=12,
+                     volume_rows=False):
+        """Restarted single-reduction FGMRES with the plan's multigrid as the
+        flexible preconditioner (solve_fgmres); volume_rows selects the
+        per-volume rows of the cold pressure system with the volume-scaled
+        preconditioner."""
+        x = self._torch.zeros_like(rhs) if solution is None else solution
+        c = hf_fgmres_control()
+        c.absolute_tolerance = atol
+        c.relative_tolerance = rtol
+        c.maximum_iterations = maximum_iterations
+        c.true_residual_interval = true_residual_interval
+        c.restart = restart
+        c.volume_rows = 1 if volume_rows else 0
+        res = hf_solve_result()
+        status = self.lib.hf_fgmres_solve(self.handle, _ptr(rhs), _ptr(x),
+                                          ctypes.byref(c), ctypes.byref(res))
+        result = SolveResult.from_c(res)
+        if check:
+            _check("hf_fgmres_solve", status)
         return x, result
 
     def project_mean(self, field):

@@ -117,6 +117,17 @@ typedef struct hf_policy {
    *    (reverse sweeps odd, even), each colour one parallel pass; periodic
    *    ghosts are refreshed per colour pass. */
   uint8_t line_order;
+  /* MgPointSmootherKind (v04_linear.hpp:104-107): 0 red-black (omega 0.72),
+   * 1 Chebyshev-Jacobi on every point level but the coarsest
+   * (src/solver_mg.cpp:2916-3090). */
+  uint8_t point_smoother;
+  /* chebyshev_lower_spectrum_fraction, 0.3: the smoother targets the
+   * spectrum of D^-1 A in [2 fraction, 2]. */
+  double chebyshev_lower_fraction;
+  /* diagonal_shortcut_maximum_ratio, 0 = off: with no null space, the
+   * preconditioner is z = D^-1 r when every finest row's face sum over its
+   * diagonal is at most this ratio (< 1; src/solver_mg.cpp:1573-1611). */
+  double diagonal_shortcut_ratio;
 } hf_policy;
 
 /* LinearSolveControl (v04_linear.hpp:454-466) restricted to PCG. */
@@ -127,6 +138,20 @@ typedef struct hf_solve_control {
   uint32_t true_residual_interval;
   uint8_t flexible; /* 0: beta = rho'/rho (reference); 1: Polak-Ribiere */
 } hf_solve_control;
+
+/* LinearSolveControl (v04_linear.hpp:454-466) for FGMRES. */
+typedef struct hf_fgmres_control {
+  double absolute_tolerance;
+  double relative_tolerance;
+  uint32_t maximum_iterations;
+  uint32_t true_residual_interval;
+  uint32_t restart;     /* basis length, 1..64 (12 in the reference's
+                           production pressure solve) */
+  /* 1: operator rows per unit volume (A divided by the cell volume w_x w_y
+   * w_z) with the multigrid applied to V r: the cold pressure system and
+   * its VolumeScaledPreconditioner (src/solver_cold.hpp:378-415). */
+  uint8_t volume_rows;
+} hf_fgmres_control;
 
 /* LinearSolveResult subset (v04_linear.hpp:530-579). */
 typedef struct hf_solve_result {
@@ -142,6 +167,8 @@ typedef struct hf_solve_result {
   uint64_t preconditioner_applies;
   uint64_t reduction_calls;
   double solve_milliseconds; /* device time of the whole solve */
+  uint64_t norm_breakdown_restarts; /* FGMRES: unsafe-recurrence and
+                                       happy-breakdown restarts */
 } hf_solve_result;
 
 typedef struct hf_level_info {
@@ -173,6 +200,50 @@ hf_status hf_plan_destroy(hf_plan* plan);
 hf_status hf_assemble_coefficients(hf_plan* plan, const double* mobility,
                                    double* diag, double* bx, double* by,
                                    double* bz);
+
+/* Pressure-correction system of the low-Mach step: replaces
+ * assemble_pressure_system_impl (src/solver_pressure.cpp:110-332),
+ *   D = a0 V drho_dp + (six face coefficients of the cell),
+ *   b = -(V (a0 rho + a1 rho_n + a2 rho_nm1 - source) + outward flux of phi),
+ * V = w_x w_y w_z of the plan's grid.  Faces bx, by, bz (>= 0) and fluxes
+ * phi_x, phi_y, phi_z use the face layout; rho_nm1 NULL selects BDF1 (a2 is
+ * then unused), source NULL no mass source.  Every input and every D and b
+ * is checked first (positive densities, drho_dp, V and D; finite values);
+ * on a failure nothing is written (numerical_failure, detail 9111). */
+hf_status hf_assemble_pressure_system(
+    hf_plan* plan, const double* bx, const double* by, const double* bz,
+    const double* phi_x, const double* phi_y, const double* phi_z,
+    const double* rho, const double* rho_n, const double* rho_nm1,
+    const double* drho_dp, const double* source, double a0, double a1,
+    double a2, double* diag, double* rhs);
+
+/* Side kinds of the cold pressure coefficients. */
+enum hf_cold_side {
+  HF_COLD_OTHER = 0,          /* wall, inlet: exterior face coefficient 0 */
+  HF_COLD_PERIODIC = 1,
+  HF_COLD_PRESSURE_OUTLET = 2 /* exterior face adds 2 projection to D */
+};
+
+/* Cold-path pressure coefficients: replaces the coefficient loop of the
+ * product step (src/core_product_freeze.cpp:14279-14317) over the face
+ * records of src/solver_cold.hpp:990-1110, given as face arrays
+ * (projection and limiter_diffusion per face):
+ *   face = 0 on a non-periodic exterior face or next to an inactive cell,
+ *          else projection + limiter_diffusion / 2;
+ *   D    = V rho / ((p_ref + p) dt) (0 with fixed_eos) + the six faces
+ *          + 2 projection per pressure-outlet face (active cells), 1 on
+ *          inactive cells.
+ * fluid: cell activity bytes, NULL = all active.  kinds: x_min .. z_max,
+ * enum hf_cold_side (host array). */
+hf_status hf_cold_coefficients(hf_plan* plan, const double* proj_x,
+                               const double* proj_y, const double* proj_z,
+                               const double* lim_x, const double* lim_y,
+                               const double* lim_z, const uint8_t* fluid,
+                               const double* rho, const double* pressure,
+                               double pressure_reference, double dt,
+                               int32_t fixed_eos, const int32_t* kinds,
+                               double* diag, double* cx, double* cy,
+                               double* cz);
 
 /* Numeric setup: IBM masking of level 0 (src/solver_mg.cpp:1469-1570),
  * coarse aggregation (src/solver_mg.cpp:1706-1962) and the certified
@@ -221,6 +292,18 @@ hf_status hf_pcg_solve_static(hf_plan* plan, const double* rhs,
                               double* solution,
                               const hf_solve_control* control,
                               hf_solve_result* result);
+
+/* Restarted single-reduction FGMRES (shifted Arnoldi: one reduction per
+ * iteration gives the Hessenberg column and the new vector's norm), x holds
+ * the initial guess on entry.  Replaces solve_fgmres
+ * (src/solver_krylov.cpp:1557-2650) with the MG plan as the flexible
+ * preconditioner; with volume_rows, the operator and preconditioner pair of
+ * the cold pressure solve (src/core_product_freeze.cpp:14405-14412).  The
+ * basis (2 restart + 1 FP64 vectors) is allocated on the first call that
+ * needs it and kept by the plan. */
+hf_status hf_fgmres_solve(hf_plan* plan, const double* rhs, double* solution,
+                          const hf_fgmres_control* control,
+                          hf_solve_result* result);
 
 /* Adjoint gradient of a scalar loss L(p) with p the gauge-fixed pressure
  * solution of A(rho) p = b (zero fluid mean).  Given dL/dp (grad_p) and the

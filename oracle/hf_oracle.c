@@ -1759,3 +1759,84 @@ int or_assemble_pressure_system(const int n[3], const double* wx,
         }
   return ST_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Cold-path coefficient loop, src/core_product_freeze.cpp:14279-14317, */
+/* with the face records of src/solver_cold.hpp:990-1110 given as face   */
+/* arrays: projection and limiter_diffusion per face (a fluid cell's     */
+/* record of a face holds the face's values; an inactive cell's records  */
+/* are zero), solid_neighbour = the cell across the face is not fluid    */
+/* (periodic wrap across a periodic side).  The loop runs over cells in  */
+/* x-fastest order and, as the reference, writes each of the six face    */
+/* coefficients of every cell, so a shared face keeps the value of the   */
+/* later cell.  kinds: 0 other, 1 periodic, 2 pressure outlet.           */
+/* ------------------------------------------------------------------ */
+static int or_wrap_fluid(const int n[3], const uint8_t* fluid, const int kinds[6],
+                         int i, int j, int k) {
+  const int c[3] = {i, j, k};
+  int w[3] = {i, j, k};
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] >= 0 && c[a] < n[a]) continue;
+    if (kinds[2 * a] != 1) return 0; /* outside a non-periodic side */
+    w[a] = (c[a] + n[a]) % n[a];
+  }
+  return fluid == NULL || fluid[cidx(n, w[0], w[1], w[2])] != 0;
+}
+
+int or_cold_coefficients(const int n[3], const double* wx, const double* wy,
+                         const double* wz, const double* proj_x,
+                         const double* proj_y, const double* proj_z,
+                         const double* lim_x, const double* lim_y,
+                         const double* lim_z, const uint8_t* fluid,
+                         const double* rho, const double* pressure,
+                         double pressure_reference, double dt, int fixed_eos,
+                         const int kinds[6], double* diag, double* cx,
+                         double* cy, double* cz) {
+  for (int a = 0; a < 3; ++a)
+    if ((kinds[2 * a] == 1) != (kinds[2 * a + 1] == 1)) return ST_INVALID_PLAN;
+  if (!(dt > 0.0) || !isfinite(dt) || !isfinite(pressure_reference))
+    return ST_INVALID_PLAN;
+  const double* proj[3] = {proj_x, proj_y, proj_z};
+  const double* lim[3] = {lim_x, lim_y, lim_z};
+  double* out[3] = {cx, cy, cz};
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        const size_t c = cidx(n, i, j, k);
+        const int cell_fluid = fluid == NULL || fluid[c] != 0;
+        const double volume = wx[i] * wy[j] * wz[k];
+        double d = cell_fluid
+                       ? (fixed_eos ? 0.0
+                                    : volume * rho[c] /
+                                          ((pressure_reference + pressure[c]) * dt))
+                       : 1.0;
+        for (int f = 0; f < 6; ++f) {
+          const int a = f / 2;
+          int at[3] = {i, j, k};
+          if (f % 2) at[a]++;
+          const size_t fi = a == 0   ? xfidx(n, at[0], at[1], at[2])
+                            : a == 1 ? yfidx(n, at[0], at[1], at[2])
+                                     : zfidx(n, at[0], at[1], at[2]);
+          const int coordinate = at[a];
+          const int exterior =
+              (coordinate == 0 || coordinate == n[a]) && kinds[f] != 1;
+          int nb[3] = {i, j, k};
+          nb[a] += f % 2 ? 1 : -1;
+          /* an inactive cell's records are zero: projection 0, limiter 0,
+           * no solid neighbour */
+          const double p = cell_fluid ? proj[a][fi] : 0.0;
+          const double l = cell_fluid ? lim[a][fi] : 0.0;
+          const int solid =
+              cell_fluid && !or_wrap_fluid(n, fluid, kinds, nb[0], nb[1], nb[2]) &&
+              !exterior;
+          const double coefficient = exterior || solid ? 0.0 : p + 0.5 * l;
+          if (cell_fluid) {
+            d += coefficient;
+            if (exterior && kinds[f] == 2) d += 2.0 * p;
+          }
+          out[a][fi] = coefficient;
+        }
+        diag[c] = d;
+      }
+  return ST_OK;
+}

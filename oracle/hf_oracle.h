@@ -183,6 +183,23 @@ int or_assemble_pressure_system(const int n[3], const double* widths_x,
                                 const double* source, double a0, double a1,
                                 double a2, double* diag, double* rhs);
 
+/* Cold-path pressure coefficients (src/core_product_freeze.cpp:
+ * 14279-14317): face coefficient = 0 on non-periodic exterior faces and
+ * faces next to an inactive cell, else projection + 0.5 limiter_diffusion;
+ * D = V rho / ((p_ref + p) dt) (0 with fixed_eos) + the six faces
+ * + 2 projection per pressure-outlet face for active cells, 1 for inactive
+ * ones.  kinds per side: 0 other, 1 periodic, 2 pressure outlet.  fluid may
+ * be NULL (every cell active). */
+int or_cold_coefficients(const int n[3], const double* widths_x,
+                         const double* widths_y, const double* widths_z,
+                         const double* proj_x, const double* proj_y,
+                         const double* proj_z, const double* lim_x,
+                         const double* lim_y, const double* lim_z,
+                         const uint8_t* fluid, const double* rho,
+                         const double* pressure, double pressure_reference,
+                         double dt, int fixed_eos, const int kinds[6],
+                         double* diag, double* cx, double* cy, double* cz);
+
 /* Vector kernels, src/solver_krylov.cpp:176-204 and :650-711. */
 double or_dot(const double* a, const double* b, size_t n);
 double or_norm2(const double* v, size_t n);

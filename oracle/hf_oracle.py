@@ -121,6 +121,11 @@ def lib():
         L.or_assemble_pressure_system.argtypes = (
             [_I] + [_D] * 14 + [ctypes.c_double] * 3 + [_D, _D])
         L.or_assemble_pressure_system.restype = ctypes.c_int
+        L.or_cold_coefficients.argtypes = (
+            [_I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _U8, _D, _D,
+             ctypes.c_double, ctypes.c_double, ctypes.c_int, _I, _D, _D, _D,
+             _D])
+        L.or_cold_coefficients.restype = ctypes.c_int
         L.or_mg_diagonal_shortcut.argtypes = [ctypes.c_void_p]
         L.or_mg_diagonal_shortcut.restype = ctypes.c_int
         L.or_dot.argtypes = [_D, _D, ctypes.c_size_t]
@@ -322,6 +327,30 @@ def assemble_pressure_system(n, faces, bx, by, bz, phi, rho, rho_n,
         c(nm1), _d(keep[8]), c(src), float(a[0]), float(a[1]), float(a[2]),
         _d(diag), _d(rhs))
     return st, diag, rhs
+
+
+def cold_coefficients(n, faces, proj, lim, fluid, rho, pressure,
+                      pressure_reference, dt, kinds, fixed_eos=False):
+    """(status, D, cx, cy, cz) of the cold-path pressure coefficient loop;
+    proj, lim = per-axis face arrays, kinds = six side kinds (0 other,
+    1 periodic, 2 pressure outlet)."""
+    cells = n[0] * n[1] * n[2]
+    w = [np.ascontiguousarray(np.diff(f), dtype=np.float64) for f in faces]
+    fc = face_counts(n)
+    diag = np.full(cells, np.nan)
+    out = [np.full(c, np.nan) for c in fc]
+    keep = [np.ascontiguousarray(v, dtype=np.float64)
+            for v in (*proj, *lim, rho, pressure)]
+    fl = None if fluid is None else np.ascontiguousarray(fluid,
+                                                         dtype=np.uint8)
+    pn, _k1 = _i(n)
+    pk, _k2 = _i(kinds)
+    st = lib().or_cold_coefficients(
+        pn, _d(w[0]), _d(w[1]), _d(w[2]), *[_d(k) for k in keep[:6]],
+        _u8(fl), _d(keep[6]), _d(keep[7]), float(pressure_reference),
+        float(dt), 1 if fixed_eos else 0, pk, _d(diag), _d(out[0]),
+        _d(out[1]), _d(out[2]))
+    return st, diag, out[0], out[1], out[2]
 
 
 def production_policy(**kw) -> or_policy:

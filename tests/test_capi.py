@@ -32,7 +32,9 @@ def test_struct_layouts_match_header():
     import hundun_flow as H
     # sizes follow the C declarations (natural alignment, x86-64)
     assert ctypes.sizeof(H.hf_status) == 8
-    assert ctypes.sizeof(H.hf_policy) == 24
+    assert ctypes.sizeof(H.hf_policy) == 40
+    assert ctypes.sizeof(H.hf_fgmres_control) == 32
+    assert ctypes.sizeof(H.hf_solve_result) == 96
     assert ctypes.sizeof(H.hf_solve_control) == 32
     assert ctypes.sizeof(H.hf_grid_desc) == 64
     assert ctypes.sizeof(H.hf_level_info) == 16
@@ -46,6 +48,11 @@ def test_policy_defaults_match_reference():
     assert (p.pre_sweeps, p.post_sweeps, p.maximum_levels) == (2, 2, 32)
     assert (p.coarse_sweeps, p.minimum_coarse_extent) == (24, 3)
     assert p.f_cycle == 0
+    # point_smoother red_black, chebyshev_lower_spectrum_fraction 0.3,
+    # diagonal_shortcut_maximum_ratio 0 (off)
+    assert p.point_smoother == 0
+    assert p.chebyshev_lower_fraction == 0.3
+    assert p.diagonal_shortcut_ratio == 0.0
 
 
 def test_invalid_arguments_are_rejected_without_gpu():
