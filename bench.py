@@ -128,39 +128,83 @@ def gpu_solves(plan, b, steps, warmup, flexible, torch):
     return x, result, times
 
 
-def cpu_baseline(case, iterations):
-    """Oracle (single-thread C restatement) on the full grid: one V(2,2)
-    application + one operator application + three dot passes, i.e. one PCG
-    iteration, scaled by the FP64 iteration count."""
-    import numpy as np
-
+def cpu_port_sample(case):
+    """The oracle (single-thread C restatement) on the full grid: one V(2,2)
+    application + one operator application + three dot passes, i.e. one
+    PCG iteration (seconds)."""
     import hf_oracle as O
-    t_setup = time.perf_counter()
     (d, bx, by, bz), mg = O.from_case(case, O.policy())
-    setup_s = time.perf_counter() - t_setup
     r = case["rhs"]
     t0 = time.perf_counter()
     z = mg.apply(r)
-    t_mg = time.perf_counter() - t0
-    t0 = time.perf_counter()
     O.operator_apply(case["n"], case["bc"], d, bx, by, bz,
                      case["cell_active"], z)
-    t_op = time.perf_counter() - t0
-    t0 = time.perf_counter()
     for _ in range(3):
         O.dot(r, z)
-    t_vec = time.perf_counter() - t0
-    per_iteration = t_mg + t_op + t_vec
+    per_iteration = time.perf_counter() - t0
     del mg
+    return per_iteration
+
+
+def cpu_reference_sample(case, short=1, long=3):
+    """The reference's own CPU path (oracle/_ref/hf_ref_driver: the
+    reference NativeCartesianMgPlan in the reference solve_pcg, compiled from
+    the reference sources, one MPI rank) on the full grid, timed for `short`
+    and `long` PCG iterations: the solve costs start + iterations x
+    per_iteration.  Returns (start, per_iteration, compile) in seconds, or
+    None when the driver was not built."""
+    import hf_oracle as O
+    import ref_case
+    if not os.path.exists(ref_case.DRIVER):
+        return None
+    pol = O.policy()  # V(2,2) red-black, 24 coarse sweeps, constant null space
+    run = dict(case)
+    run["coeffs"] = O.assemble(case["n"], case["faces"], case["bc"],
+                               case["mobility"])
+    b = case["rhs"]
+    seconds = {}
+    compile_s = 0.0
+    for its in (short, long):
+        out = ref_case.run_driver(run, pol, b, b, rtol=1e-10, maxit=its,
+                                  interval=4, repeats=1, timing_only=True)
+        if int(out["krylov_iterations"]) != its:
+            return None
+        seconds[its] = float(out["seconds"][0])
+        compile_s = float(out["compile_seconds"])
+    per_iteration = (seconds[long] - seconds[short]) / (long - short)
+    start = max(0.0, seconds[short] - short * per_iteration)
+    return start, per_iteration, compile_s
+
+
+def cpu_baseline(case, iterations):
+    """CPU baseline on the host cores: the reference's CPU path when its
+    build is present (kind "reference"), else the C oracle (kind "port"),
+    on a bounded sample of the same grid, scaled to one solve of
+    `iterations` FP64 PCG iterations (the reference hierarchy's count)."""
+    ref = cpu_reference_sample(case)
+    port = cpu_port_sample(case)
+    if ref is not None:
+        start, per_it, compile_s = ref
+        return {
+            "value": round((start + iterations * per_it) * 1e3, 1),
+            "unit": "ms",
+            "cores": 1,
+            "kind": "reference",
+            "sample": (f"reference solve_pcg + NativeCartesianMgPlan (one MPI "
+                       f"rank) on the full grid, timed at 1 and 3 iterations: "
+                       f"{per_it:.2f} s per iteration + {start:.2f} s start "
+                       f"(plan compile {compile_s:.1f} s untimed), scaled to "
+                       f"{iterations} FP64 iterations"),
+            "port_seconds_per_iteration": round(port, 3),
+            "port_over_reference": round(port / per_it, 3),
+        }
     return {
-        "value": round(per_iteration * iterations * 1e3, 1),
+        "value": round(port * iterations * 1e3, 1),
         "unit": "ms",
         "cores": 1,
         "kind": "port",
         "sample": (f"1 PCG iteration of the C oracle on the full grid "
-                   f"(V(2,2) {t_mg:.2f}s + operator {t_op:.2f}s + 3 dots "
-                   f"{t_vec:.2f}s; setup {setup_s:.1f}s untimed) x "
-                   f"{iterations} FP64 iterations"),
+                   f"({port:.2f} s) x {iterations} FP64 iterations"),
     }
 
 
@@ -275,7 +319,19 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (x10) as "
                          "the known-size reference for HBM counters")
+    ap.add_argument("--cpu-only", type=int, default=0, metavar="ITERATIONS",
+                    help="only time the CPU baseline (reference CPU path, or "
+                         "the C oracle without its build) for a solve of this "
+                         "many FP64 iterations; needs no GPU")
     args = ap.parse_args()
+    if args.cpu_only:
+        nx, ny, nz = (int(v) for v in args.shape.split(","))
+        log(f"[bench] generating {args.workload} {nx}x{ny}x{nz}")
+        case = make_case(args.workload, nx, ny, nz)
+        print(json.dumps({"workload": f"{args.workload} {nx}x{ny}x{nz}",
+                          "cpu_baseline": cpu_baseline(case, args.cpu_only)}),
+              flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the "
               "single-GPU share of the pressure solve (multi-GPU sharding is "
@@ -444,7 +500,7 @@ def main():
             its = results[("fp64", 1, False)]["res"].iterations
         else:
             its = main["res"].iterations
-        log("[bench] cpu baseline (oracle, full grid, 1 iteration)")
+        log("[bench] cpu baseline (reference CPU path, bounded sample)")
         cpu = cpu_baseline(case, its)
 
     line = {

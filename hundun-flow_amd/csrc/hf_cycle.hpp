@@ -117,6 +117,36 @@ __global__ void k_copy(Chunk ch, double* dst, const double* src) {
   for (int c = lo + threadIdx.x; c < hi; c += kBlock) dst[c] = src[c];
 }
 
+// dest = a * left + b * right (axpby, src/solver_krylov.cpp:344-361); a
+// non-finite result sets *flag.
+__global__ void k_axpby(Chunk ch, double a, const double* left, double b,
+                        const double* right, double* dest, double* flag) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  bool bad = false;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
+    const double v = a * left[c] + b * right[c];
+    bad = bad || !isfinite(v);
+    dest[c] = v;
+  }
+  if (bad) *flag = 1.0;
+}
+
+// dst = src (copy_field, src/solver_krylov.cpp:300-316); a non-finite
+// value sets *flag.
+__global__ void k_copy_checked(Chunk ch, double* dst, const double* src,
+                               double* flag) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  bool bad = false;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
+    const double v = src[c];
+    bad = bad || !isfinite(v);
+    dst[c] = v;
+  }
+  if (bad) *flag = 1.0;
+}
+
 // dst[0..n) = value: clears memory with vector stores, where a solve may run
 // inside a caller's captured stream work (no memset nodes).
 __global__ void k_fill(Chunk ch, double* dst, double value) {

@@ -2607,6 +2607,135 @@ hf_status hf_fgmres_solve(hf_plan* p, const double* rhs, double* solution,
   return st;
 }
 
+struct hf_vector_engine {
+  hipStream_t stream = nullptr;
+  int64_t capacity = 0;
+  int32_t slots = 0;
+  double* vectors = nullptr;
+  double* partials = nullptr;
+  double* folded = nullptr;
+  double* scalars = nullptr;  // [0] the last sum, [1] the non-finite flag
+  double* host = nullptr;     // pinned mirror of scalars
+};
+
+hf_status hf_vector_engine_destroy(hf_vector_engine* e) {
+  if (e == nullptr) return ok();
+  (void)hipStreamSynchronize(e->stream);
+  for (void* ptr : {static_cast<void*>(e->vectors),
+                    static_cast<void*>(e->partials),
+                    static_cast<void*>(e->folded),
+                    static_cast<void*>(e->scalars)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  if (e->host != nullptr) (void)hipHostFree(e->host);
+  delete e;
+  return ok();
+}
+
+hf_status hf_vector_engine_create(int64_t capacity, int32_t slots,
+                                  void* stream, hf_vector_engine** out) {
+  if (out == nullptr) return fail(HF_INVALID_PLAN, kDetailPlan);
+  *out = nullptr;
+  // vector kernels index with 32-bit integers
+  if (capacity <= 0 || slots < 0 || slots > 256 ||
+      capacity > std::numeric_limits<int32_t>::max() - kBlock)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  hf_vector_engine* e = new (std::nothrow) hf_vector_engine();
+  if (e == nullptr) return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  e->stream = static_cast<hipStream_t>(stream);
+  e->capacity = capacity;
+  e->slots = slots;
+  const int64_t blocks = blocks_for(capacity);
+  bool good =
+      (slots == 0 ||
+       hipMalloc(reinterpret_cast<void**>(&e->vectors),
+                 static_cast<size_t>(slots) * capacity * sizeof(double)) ==
+           hipSuccess) &&
+      hipMalloc(reinterpret_cast<void**>(&e->partials),
+                static_cast<size_t>(blocks) * sizeof(double)) == hipSuccess &&
+      hipMalloc(reinterpret_cast<void**>(&e->folded),
+                kFoldStride * sizeof(double)) == hipSuccess &&
+      hipMalloc(reinterpret_cast<void**>(&e->scalars), 2 * sizeof(double)) ==
+          hipSuccess &&
+      hipHostMalloc(reinterpret_cast<void**>(&e->host), 2 * sizeof(double),
+                    hipHostMallocDefault) == hipSuccess;
+  if (good)
+    good = hipMemsetAsync(e->scalars, 0, 2 * sizeof(double), e->stream) ==
+               hipSuccess &&
+           hipStreamSynchronize(e->stream) == hipSuccess;
+  if (!good) {
+    hf_vector_engine_destroy(e);
+    return fail(HF_ALLOCATION_FAILURE, kDetailAlloc);
+  }
+  *out = e;
+  return ok();
+}
+
+hf_status hf_vector_engine_slot(hf_vector_engine* e, int32_t slot,
+                                double** vector) {
+  if (e == nullptr || vector == nullptr || slot < 0 || slot >= e->slots)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  *vector = e->vectors + static_cast<int64_t>(slot) * e->capacity;
+  return ok();
+}
+
+hf_status hf_vector_dot(hf_vector_engine* e, int64_t n, const double* a,
+                        const double* b, double* result) {
+  if (e == nullptr || a == nullptr || b == nullptr || result == nullptr ||
+      n <= 0 || n > e->capacity)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Chunk ch = make_chunk(static_cast<int>(n));
+  const int64_t blocks = chunk_blocks(ch);
+  HF_LAUNCH(k_dot, blocks, kBlock, 0, e->stream, ch, a, b, e->partials);
+  const double* src = e->partials;
+  int64_t count = blocks;
+  if (blocks > kFoldSpan) {
+    const int64_t spans = (blocks + kFoldSpan - 1) / kFoldSpan;
+    hipLaunchKernelGGL(k_fold_partials, dim3(static_cast<uint32_t>(spans), 1),
+                       dim3(kBlock), 0, e->stream, e->partials, blocks,
+                       blocks, e->folded, static_cast<int64_t>(kFoldStride));
+    src = e->folded;
+    count = spans;
+  }
+  HF_LAUNCH(k_finalize, 1, 1024, 0, e->stream, src, count,
+            static_cast<int64_t>(kFoldStride), 1, e->scalars,
+            static_cast<int>(F_STORE), 0, 0.0, 1);
+  if (hipMemcpyAsync(e->host, e->scalars, 2 * sizeof(double),
+                     hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return fail(HF_INVALID_PLAN, kDetailHip);
+  if (e->host[1] != 0.0) {
+    // a non-finite vector value since the last sum
+    const Chunk one = make_chunk(2);
+    HF_LAUNCH(k_fill, 1, kBlock, 0, e->stream, one, e->scalars, 0.0);
+    return fail(HF_NUMERICAL_FAILURE, kDetailNonFinite);
+  }
+  *result = e->host[0];
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_vector_axpby(hf_vector_engine* e, int64_t n, double a,
+                          const double* left, double b, const double* right,
+                          double* dest) {
+  if (e == nullptr || left == nullptr || right == nullptr || dest == nullptr ||
+      n <= 0 || n > e->capacity)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Chunk ch = make_chunk(static_cast<int>(n));
+  HF_LAUNCH(k_axpby, chunk_blocks(ch), kBlock, 0, e->stream, ch, a, left, b,
+            right, dest, e->scalars + 1);
+  return hip_check(hipGetLastError());
+}
+
+hf_status hf_vector_copy(hf_vector_engine* e, int64_t n, const double* source,
+                         double* destination) {
+  if (e == nullptr || source == nullptr || destination == nullptr || n <= 0 ||
+      n > e->capacity)
+    return fail(HF_INVALID_PLAN, kDetailPlan);
+  const Chunk ch = make_chunk(static_cast<int>(n));
+  HF_LAUNCH(k_copy_checked, chunk_blocks(ch), kBlock, 0, e->stream, ch,
+            destination, source, e->scalars + 1);
+  return hip_check(hipGetLastError());
+}
+
 hf_status hf_project_mean(hf_plan* p, double* field) {
   if (p == nullptr || field == nullptr || !p->coefficients_ready)
     return fail(HF_INVALID_PLAN, kDetailPlan);

@@ -39,9 +39,12 @@ ABI_FUNCTIONS = [
     "hf_assemble_coefficients", "hf_assemble_pressure_system",
     "hf_cold_coefficients", "hf_plan_set_coefficients",
     "hf_operator_apply", "hf_mg_apply", "hf_pcg_solve",
-    "hf_pcg_solve_static", "hf_fgmres_solve", "hf_adjoint_gradients", "hf_project_mean", "hf_plan_level_count",
-    "hf_plan_level_info", "hf_plan_level_array", "hf_plan_device_bytes",
-    "hf_plan_set_profiling", "hf_plan_profile", "hf_status_message",
+    "hf_pcg_solve_static", "hf_fgmres_solve", "hf_adjoint_gradients",
+    "hf_project_mean", "hf_plan_level_count", "hf_plan_level_info",
+    "hf_plan_level_array", "hf_plan_device_bytes", "hf_plan_set_profiling",
+    "hf_plan_profile", "hf_status_message", "hf_vector_engine_create",
+    "hf_vector_engine_destroy", "hf_vector_engine_slot", "hf_vector_dot",
+    "hf_vector_axpby", "hf_vector_copy",
 ]
 
 
@@ -170,6 +173,16 @@ def load_library(path: str = LIBRARY_PATH):
         "hf_adjoint_gradients": [P, D, D, D, ctypes.POINTER(hf_solve_control),
                                  D, D, ctypes.POINTER(hf_solve_result)],
         "hf_project_mean": [P, D],
+        "hf_vector_engine_create": [ctypes.c_int64, ctypes.c_int32, P,
+                                    ctypes.POINTER(ctypes.c_void_p)],
+        "hf_vector_engine_destroy": [P],
+        "hf_vector_engine_slot": [P, ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_void_p)],
+        "hf_vector_dot": [P, ctypes.c_int64, D, D,
+                          ctypes.POINTER(ctypes.c_double)],
+        "hf_vector_axpby": [P, ctypes.c_int64, ctypes.c_double, D,
+                            ctypes.c_double, D, D],
+        "hf_vector_copy": [P, ctypes.c_int64, D, D],
         "hf_plan_level_count": [P, ctypes.POINTER(ctypes.c_int32)],
         "hf_plan_level_info": [P, ctypes.c_int32,
                                ctypes.POINTER(hf_level_info)],

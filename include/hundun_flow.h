@@ -319,6 +319,31 @@ hf_status hf_adjoint_gradients(hf_plan* plan, const double* mobility,
                                double* grad_b, double* grad_logrho,
                                hf_solve_result* result);
 
+/* Device vector engine for Krylov solves over operators and
+ * preconditioners that are not a plan's own pair (any LinearOperator /
+ * LinearPreconditioner of the C++ layer): the reference's Krylov vector
+ * kernels and ReductionEngine sums (src/solver_krylov.cpp:176-399, :631-711;
+ * src/solver_reduction.cpp:651-700) on one stream, with `slots` device
+ * vectors of `capacity` FP64 elements owned by the engine.  Sums are
+ * deterministic two-stage block reductions.  A non-finite value produced by
+ * hf_vector_axpby or hf_vector_copy is reported by the next hf_vector_dot
+ * (numerical_failure, detail 9107). */
+typedef struct hf_vector_engine hf_vector_engine;
+hf_status hf_vector_engine_create(int64_t capacity, int32_t slots,
+                                  void* stream, hf_vector_engine** out);
+hf_status hf_vector_engine_destroy(hf_vector_engine* engine);
+hf_status hf_vector_engine_slot(hf_vector_engine* engine, int32_t slot,
+                                double** vector);
+/* result = a . b (host value; waits for the stream). */
+hf_status hf_vector_dot(hf_vector_engine* engine, int64_t n, const double* a,
+                        const double* b, double* result);
+/* dest = a left + b right (dest may alias left or right). */
+hf_status hf_vector_axpby(hf_vector_engine* engine, int64_t n, double a,
+                          const double* left, double b, const double* right,
+                          double* dest);
+hf_status hf_vector_copy(hf_vector_engine* engine, int64_t n,
+                         const double* source, double* destination);
+
 /* Gauge: subtract the Euclidean mean over fluid cells (solids stay 0). */
 hf_status hf_project_mean(hf_plan* plan, double* field);
 

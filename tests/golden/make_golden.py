@@ -14,10 +14,7 @@ Krylov solution with its iteration and application counts.
 Usage: python tests/golden/make_golden.py   (needs the reference tree)
 """
 import os
-import struct
-import subprocess
 import sys
-import tempfile
 
 import numpy as np
 
@@ -28,8 +25,9 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import hf_cases  # noqa: E402
 import hf_oracle as O  # noqa: E402
+import ref_case  # noqa: E402
+from ref_case import run_driver  # noqa: E402
 
-DRIVER = os.path.join(ROOT, "oracle", "_ref", "hf_ref_driver")
 
 
 def with_bc(case, bc):
@@ -134,96 +132,8 @@ def compressible(n, seed, cfl):
     return case
 
 
-def write_case(path, case, pol, r, b, rtol, maxit, interval, solver=0,
-               restart=0, volume_rows=0, repeats=0):
-    n = case["n"]
-    with open(path, "wb") as f:
-        f.write(b"HFREF002")
-        f.write(struct.pack("<3i", *n))
-        f.write(struct.pack("<6i", *case["bc"]))
-        f.write(struct.pack("<d", pol.anisotropy_threshold))
-        f.write(struct.pack("<8i", pol.pre_sweeps, pol.post_sweeps,
-                            pol.maximum_levels, pol.coarse_sweeps,
-                            pol.minimum_coarse_extent, pol.f_cycle,
-                            pol.null_space_constant, pol.unit_linear))
-        f.write(struct.pack("<i", 1))
-        f.write(struct.pack("<dii", rtol, maxit, interval))
-        f.write(struct.pack("<i", pol.point_smoother))
-        f.write(struct.pack("<dd", pol.chebyshev_lower_fraction,
-                            pol.diagonal_shortcut_ratio))
-        f.write(struct.pack("<4i", solver, restart, volume_rows, repeats))
-        for a in case["faces"]:
-            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
-        for a in case["coeffs"]:
-            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
-        for k in ("cell_active", "x_active", "y_active", "z_active"):
-            f.write(np.ascontiguousarray(case[k], dtype=np.uint8).tobytes())
-        f.write(np.ascontiguousarray(r, dtype="<f8").tobytes())
-        f.write(np.ascontiguousarray(b, dtype="<f8").tobytes())
-
-
-def read_output(path):
-    data = open(path, "rb").read()
-    assert data[:8] == b"HFOUT002"
-    off = 8
-
-    def ints(k):
-        nonlocal off
-        v = struct.unpack_from(f"<{k}i", data, off)
-        off += 4 * k
-        return v
-
-    def doubles(k):
-        nonlocal off
-        v = np.frombuffer(data, dtype="<f8", count=k, offset=off).copy()
-        off += 8 * k
-        return v
-
-    def int64s(k):
-        nonlocal off
-        v = struct.unpack_from(f"<{k}q", data, off)
-        off += 8 * k
-        return v
-
-    (levels,) = ints(1)
-    (shortcut,) = ints(1)
-    plan = np.array([ints(5) for _ in range(levels)], dtype=np.int32)
-    cells = [int(p[0] * p[1] * p[2]) for p in plan]
-    out = {"plan": plan, "shortcut": np.int32(shortcut)}
-    for l in range(levels):
-        out[f"diag_{l}"] = doubles(cells[l])
-    out["z"] = doubles(cells[0])
-    for l in range(levels):
-        for s in ("x", "b", "r"):
-            out[f"slot_{s}_{l}"] = doubles(cells[l])
-    status, termination, iterations = ints(3)
-    out["krylov_status"] = np.int32(status)
-    out["krylov_termination"] = np.int32(termination)
-    out["krylov_iterations"] = np.int32(iterations)
-    (out["krylov_initial"], out["krylov_final"],
-     out["krylov_rhs_norm"]) = doubles(3)
-    applies = int64s(3)
-    out["krylov_applies"] = np.array(applies, dtype=np.int64)
-    out["krylov_x"] = doubles(cells[0])
-    (repeats,) = ints(1)
-    out["seconds"] = doubles(repeats)
-    assert off == len(data)
-    return out
-
-
-def run_driver(case, pol, r, b, rtol=1e-12, maxit=400, interval=4, solver=0,
-               restart=0, volume_rows=0, repeats=0):
-    with tempfile.TemporaryDirectory() as tmp:
-        cin = os.path.join(tmp, "case.bin")
-        cout = os.path.join(tmp, "out.bin")
-        write_case(cin, case, pol, r, b, rtol, maxit, interval, solver,
-                   restart, volume_rows, repeats)
-        subprocess.run([DRIVER, cin, cout], check=True)
-        return read_output(cout)
-
-
 def main():
-    if not os.path.exists(DRIVER):
+    if not os.path.exists(ref_case.DRIVER):
         raise SystemExit("build the reference closure first: "
                          "make -C oracle -f ref.mk")
     for name, case, overrides in cases():
