@@ -163,3 +163,28 @@ def test_cpp_layer_drop_in_surface():
     assert torch.equal(xf, x_fg)
     assert oi[9] >= 1 + oi[1] and oi[10] == 2 and oi[11] > 0
     plan.close()
+
+
+REFERENCE_INCLUDE = "/root/reference/versions/v0.4/include"
+
+
+def test_reference_adapter_compiles_against_reference_headers():
+    """tests/cpp/reference_adapter.cpp -- the reference-side binding of
+    INTEGRATION.md (NativeCartesianMgSpec, MgCoefficientViews and
+    LinearSolveInvocation of the reference translated to this library's
+    C++ layer) -- compiles against the reference's own headers.  Compiled
+    only; skipped where the reference tree or its MPI headers are absent."""
+    import shutil
+    import subprocess
+    mpi = "/opt/conda/include"
+    if not os.path.isdir(REFERENCE_INCLUDE) or not os.path.exists(
+            os.path.join(mpi, "mpi.h")) or shutil.which("g++") is None:
+        pytest.skip("reference headers or MPI headers not present")
+    root = os.path.dirname(HERE)
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra",
+           "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + REFERENCE_INCLUDE, "-I" + mpi,
+           "-I" + os.path.join(root, "include"),
+           os.path.join(HERE, "cpp", "reference_adapter.cpp")]
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr[-2000:]
