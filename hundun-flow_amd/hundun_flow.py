@@ -427,16 +427,17 @@ class Plan:
         return field
 
     def adjoint(self, mobility, pressure, grad_p, want_b=True,
-                want_logrho=True, **control):
+                want_logrho=True, check=True, **control):
         gb = self._f64(self.cells) if want_b else None
         gr = self._f64(self.cells) if want_logrho else None
         ctl = self.control(**control)
         res = hf_solve_result()
-        _check("hf_adjoint_gradients",
-               self.lib.hf_adjoint_gradients(self.handle, _ptr(mobility),
-                                             _ptr(pressure), _ptr(grad_p),
-                                             ctypes.byref(ctl), _ptr(gb),
-                                             _ptr(gr), ctypes.byref(res)))
+        status = self.lib.hf_adjoint_gradients(self.handle, _ptr(mobility),
+                                               _ptr(pressure), _ptr(grad_p),
+                                               ctypes.byref(ctl), _ptr(gb),
+                                               _ptr(gr), ctypes.byref(res))
+        if check:
+            _check("hf_adjoint_gradients", status)
         return gb, gr, SolveResult.from_c(res)
 
     # -- inspection ----------------------------------------------------
