@@ -510,6 +510,17 @@ int64_t sweep_min_cells() {
   return int64_t{1} << 21;
 }
 
+// Planes per block of the fused sweep on every level that takes it (0: the
+// planner's choice); HF_SWEEP_ZC sets it.
+int sweep_zc_override() {
+  const char* env = std::getenv("HF_SWEEP_ZC");
+  if (env != nullptr) {
+    const int v = std::atoi(env);
+    if (v > 0) return v;
+  }
+  return 0;
+}
+
 // The retained defect of a fused-sweep level and its restriction run as one
 // kernel where the layout allows (defect_restrict_ok); HF_DEFECT_RESTRICT=0
 // keeps the two launches.
@@ -1805,6 +1816,8 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       // about half a million threads in flight, split into z-chunks
       const int want = std::max(1, (1 << 19) / (kSwThreads * per_plane));
       L.sweep_zc = std::min(32, std::max(4, (L.n[2] + want - 1) / want));
+      if (sweep_zc_override() > 0)
+        L.sweep_zc = std::min(L.n[2], sweep_zc_override());
       L.sweep_blocks = per_plane * ((L.n[2] + L.sweep_zc - 1) / L.sweep_zc);
     }
   }

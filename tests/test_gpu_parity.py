@@ -44,6 +44,10 @@ CASES = {
     "mixed_wide": lambda: mixed_case(n=(72, 40, 20), seed=12),
     # periodic x and y exercise the wrapped halo ring of the fused sweep
     "periodic_xy": lambda: periodic_xy_case(),
+    # rows of 70 cells (nx % 4 == 2) fill part of one 128-wide tile of the
+    # fused sweep and 36 rows leave a partial tile in y (spacing ratio 5.8:
+    # planned with point smoothing under a threshold of 16)
+    "half_quad": lambda: mixed_case(n=(70, 36, 12), seed=21),
 }
 
 
@@ -60,9 +64,9 @@ def _policy(case_name, **kw):
     # stretched grids select line relaxation in the reference planner; a
     # threshold above the spacing ratio keeps point red-black + full
     # coarsening on both sides
-    thr = 16.0 if case_name == "stretched24" else 4.0
+    thr = 16.0 if case_name in ("stretched24", "half_quad") else 4.0
     null = 0 if case_name in ("mixed", "odd_periodic", "mixed_wide",
-                              "periodic_xy") else 1
+                              "periodic_xy", "half_quad") else 1
     base = dict(anisotropy_threshold=thr, null_space_constant=null)
     base.update(kw)
     return base
@@ -355,6 +359,46 @@ def test_pcg_start_gates(torch_cuda, precision):
     plan.close()
 
 
+_SPECTRUM = {}
+
+
+def _spectrum(name, case, coef, mg, null):
+    """Extreme eigenvalues of the FP64 operator on the fluid cells (with the
+    constant null space, on its complement): the largest by Lanczos on A,
+    the smallest by Lanczos on A^-1 applied with the oracle's PCG."""
+    if name in _SPECTRUM:
+        return _SPECTRUM[name]
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    d, bx, by, bz = coef
+    n, bc = case["n"], case["bc"]
+    act = case["cell_active"] != 0
+    cells = int(np.prod(n))
+
+    def proj(v):
+        v = np.where(act, v, 0.0)
+        if null:
+            v[act] -= v[act].mean()
+        return v
+
+    def inv(v):
+        y, _ = O.pcg(n, bc, d, bx, by, bz, case["cell_active"], mg, proj(v),
+                     rtol=1e-12, maximum_iterations=400)
+        return proj(y)
+
+    def fwd(v):
+        return proj(O.operator_apply(n, bc, d, bx, by, bz,
+                                     case["cell_active"], proj(v)))
+
+    v0 = proj(np.random.default_rng(1).standard_normal(cells))
+    top = eigsh(LinearOperator((cells, cells), matvec=fwd), k=1, which="LA",
+                v0=v0, tol=1e-8, return_eigenvectors=False)[0]
+    low = 1.0 / eigsh(LinearOperator((cells, cells), matvec=inv), k=1,
+                      which="LA", v0=v0, tol=1e-8,
+                      return_eigenvectors=False)[0]
+    _SPECTRUM[name] = (low, top)
+    return low, top
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("name", ["sphere32", "step"])
 @pytest.mark.parametrize("scaling", [0, 1])
@@ -363,7 +407,13 @@ def test_mixed_precision_reaches_tolerance(torch_cuda, monkeypatch, precision,
     """FP32 / BF16-storage V-cycle inside the FP64 CG: the FP64 true residual
     reaches 1e-10 and the solution agrees with the FP64 oracle to the
     conditioning bound.  A zero tail threshold keeps every level but the
-    coarsest in the storage precision."""
+    coarsest in the storage precision.
+
+    The bound: A e = -(r - r_o) for e = x - x_o, so on the complement of the
+    null space |e| <= (|r| + |r_o|) / lambda_min, and |e| / |x_o| is at most
+    that over |x_o| (printed with kappa times the relative residual, the
+    looser textbook form).  lambda_min and lambda_max are Lanczos estimates
+    on the oracle's operator."""
     torch = torch_cuda
     _say(f"[mixed precision] case={name} precision={precision} "
          f"scaling={scaling}")
@@ -379,19 +429,27 @@ def test_mixed_precision_reaches_tolerance(torch_cuda, monkeypatch, precision,
     xg = xg.cpu().numpy()
     act = case["cell_active"]
     Ax = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xg)
+    Axo = O.operator_apply(case["n"], case["bc"], d, bx, by, bz, act, xo)
     true_rel = np.linalg.norm(b - Ax) / np.linalg.norm(b)
     diff = (np.linalg.norm(_gauge(xg, act) - _gauge(xo, act))
             / np.linalg.norm(_gauge(xo, act)))
+    null = pol.get("null_space_constant", 1)
+    low, top = _spectrum(name, case, (d, bx, by, bz), mg, null)
+    bound = ((np.linalg.norm(b - Ax) + np.linalg.norm(b - Axo))
+             / (low * np.linalg.norm(_gauge(xo, act))))
     _say(f"  it={res.iterations} (fp64 oracle {ores.iterations}) "
-         f"true_rel={true_rel:.2e} err_vs_fp64={diff:.2e}")
+         f"true_rel={true_rel:.2e} err_vs_fp64={diff:.2e} bound "
+         f"|r|/(lambda_min |x|)={bound:.2e} kappa={top / low:.0f} "
+         f"kappa*true_rel={top / low * true_rel:.2e}")
     assert res.termination == "converged"
     assert true_rel <= 1.01e-10
-    assert diff <= 1e-6
+    assert diff <= 1.05 * bound
     assert np.all(xg[act == 0] == 0.0)
     plan.close()
 
 
-@pytest.mark.parametrize("name", ["mixed_wide", "periodic_xy", "sphere32"])
+@pytest.mark.parametrize("name", ["mixed_wide", "periodic_xy", "sphere32",
+                                  "half_quad"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_fused_sweep_matches_two_pass(torch_cuda, monkeypatch, name,
                                       precision):
