@@ -231,7 +231,8 @@ __global__ void k_vol_dot(Tiles t, const T* field, const double* vol,
 // preconditioner's entry (k_mg_entry): b = fluid ? r : 0 at the level-0
 // storage type.  Partial sums: r.r in accumulator 0 and, with the constant
 // null space, vol*b in accumulator 1 (stride apart).  Tiles of the reduction
-// layout (kRedPer per block).
+// layout (kRedPer per block).  x == nullptr: x is left for the direction
+// update (k_update_p_exit), which reads p anyway.
 template <class T>
 __global__ void k_update_xr_entry(Tiles t, double* x, const double* pv,
                                   double* r, const double* q,
@@ -241,6 +242,7 @@ __global__ void k_update_xr_entry(Tiles t, double* x, const double* pv,
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
   const double a = *alpha;
+  const bool with_x = x != nullptr;
   // two groups of four tiles: each group's loads are issued together (eight
   // tiles of five FP64 operands would not fit the register file)
   constexpr int kGroup = 4;
@@ -255,8 +257,8 @@ __global__ void k_update_xr_entry(Tiles t, double* x, const double* pv,
     for (int u = 0; u < kGroup; ++u) {
       int i, j, k;
       cc[u] = red_cell(t, lb, g + u, i, j, k, ok[u]);
-      xv[u] = x[cc[u]];
-      pp[u] = pv[cc[u]];
+      xv[u] = with_x ? x[cc[u]] : 0.0;
+      pp[u] = with_x ? pv[cc[u]] : 0.0;
       rv[u] = r[cc[u]];
       qv[u] = q[cc[u]];
       fluid[u] = act[cc[u]] != 0;
@@ -265,7 +267,7 @@ __global__ void k_update_xr_entry(Tiles t, double* x, const double* pv,
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
       if (!ok[u]) continue;
-      x[cc[u]] = xv[u] + a * pp[u];
+      if (with_x) x[cc[u]] = xv[u] + a * pp[u];
       const double rc = rv[u] + -a * qv[u];
       r[cc[u]] = rc;
       d += rc * rc;
@@ -752,17 +754,22 @@ __global__ void k_mg_exit(Tiles t, const T* x, const uint8_t* act,
 // p = 1.0*z + beta*p (axpby, src/solver_krylov.cpp:321-339) with z formed
 // from the cycle's level-0 candidate as
 // k_mg_exit forms it (fluid ? (x - mean, stored and reloaded) * alpha : 0),
-// so the exit need not store z.  Tiles of the reduction layout.
+// so the exit need not store z.  With sol: first the iteration's solution
+// update sol += step p (add_scaled, src/solver_krylov.cpp:383-399) with the
+// p it replaces, which k_update_xr_entry left out.  Tiles of the reduction
+// layout.
 template <class T>
 __global__ void k_update_p_exit(Tiles t, const T* x, const uint8_t* act,
                                 const double* mean, const double* alpha,
-                                double* p, const double* beta) {
+                                double* p, const double* beta, double* sol,
+                                const double* step) {
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
   const double a = alpha == nullptr ? 1.0 : *alpha;
   const double m = mean == nullptr ? 0.0 : *mean;
   const double be = *beta;
+  const double sa = sol != nullptr ? *step : 0.0;
   T xs[kRedPer];
-  double pv[kRedPer];
+  double pv[kRedPer], sv[kRedPer];
   bool ok[kRedPer], fluid[kRedPer];
   int cc[kRedPer];
 #pragma unroll
@@ -772,10 +779,12 @@ __global__ void k_update_p_exit(Tiles t, const T* x, const uint8_t* act,
     fluid[u] = act[cc[u]] != 0;
     xs[u] = x[cc[u]];
     pv[u] = p[cc[u]];
+    sv[u] = sol != nullptr ? sol[cc[u]] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < kRedPer; ++u) {
     if (!ok[u]) continue;
+    if (sol != nullptr) sol[cc[u]] = sv[u] + sa * pv[u];
     double v = 0.0;
     if (fluid[u]) {
       double xv = static_cast<double>(ld(xs[u]));

@@ -2328,6 +2328,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   // with the counts of the reference, which stopped before the operator
   // apply and the update issued since.
   bool rho_pending = false;
+  // x += alpha p of the current iteration left to the direction update (or
+  // to the check that needs x first)
+  bool x_pending = false;
   while (res->iterations < ctl->maximum_iterations) {
     const uint64_t operator_applies0 = res->operator_applies;
     const uint32_t iterations0 = res->iterations;
@@ -2364,13 +2367,16 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                 static_cast<const double*>(pp), p->scalars + S_ALPHA);
     } else if (fuse_entry) {
       // the update also stores the preconditioner's input b (used unless
-      // this iteration verifies the true residual, which replaces r)
+      // this iteration verifies the true residual, which replaces r); x is
+      // updated by the direction update, which reads p anyway
       HF_PROF(p, P_UPDATE_XR);
       Level& L0 = p->levels[0];
+      x_pending = true;
       dispatch(L0.prec, [&](auto t) {
         using T = decltype(t);
         HF_LAUNCH(k_update_xr_entry<T>, L0.tred.blocks, kBlock, 0, p->stream,
-                  L0.tred, x, static_cast<const double*>(pp), r,
+                  L0.tred, static_cast<double*>(nullptr),
+                  static_cast<const double*>(nullptr), r,
                   static_cast<const double*>(q), p->scalars + S_ALPHA,
                   static_cast<const uint8_t*>(L0.act), widths(p),
                   static_cast<T*>(L0.b), p->partials,
@@ -2411,6 +2417,12 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       verify = recursive <= tol;
     }
     if (verify) {
+      if (x_pending) {
+        HF_PROF(p, P_UPDATE_XR);
+        HF_LAUNCH(k_update_x, nbv, kBlock, 0, p->stream, ch, x,
+                  static_cast<const double*>(pp), p->scalars + S_ALPHA);
+        x_pending = false;
+      }
       HF_TRY(true_residual(false));
       finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
       HF_TRY(read_scalars(p));
@@ -2451,8 +2463,11 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
         HF_LAUNCH(k_update_p_exit<T>, L0.tred.blocks, kBlock, 0, p->stream,
                   L0.tred, static_cast<const T*>(ex.x),
                   static_cast<const uint8_t*>(L0.act), ex.mean, ex.alpha, pp,
-                  static_cast<const double*>(p->scalars + S_BETA));
+                  static_cast<const double*>(p->scalars + S_BETA),
+                  x_pending ? x : static_cast<double*>(nullptr),
+                  static_cast<const double*>(p->scalars + S_ALPHA));
       });
+      x_pending = false;
     }
   }
   return finish(fail(HF_REJECTED_STEP, kDetailBreakdown), HF_TERM_MAXIMUM_ITERATIONS);
