@@ -1228,7 +1228,7 @@ namespace hf {
 // ---------------------------------------------------------------------------
 // Fused red-black sweep (both colour passes of src/solver_mg.cpp:3149-3222 in
 // one launch).  A block owns a 128 x 8 tile of a z-chunk and marches through
-// its planes with four LDS plane buffers holding x with a two-cell ring.  At
+// its planes with five LDS plane buffers holding x with a two-cell ring.  At
 // step k the first colour is updated on plane k over the tile plus a one-cell
 // ring (recomputing what the neighbouring tiles own, from the same inputs),
 // then the second colour on plane k-1 over the tile, which then has every
@@ -1262,6 +1262,13 @@ constexpr int kSwThreads = 256;  // threads per block of the fused sweep
 constexpr int kSwPX = kSwTX + 4;
 constexpr int kSwPY = kSwTY + 4;
 constexpr int kSwPlane = kSwPX * kSwPY;
+// LDS planes k - 3 .. k + 1: the x load of step k + 1 overwrites plane k - 3,
+// which no thread still working on step k reads (its reads are planes
+// k - 2 .. k + 1), so one barrier per step suffices
+constexpr int kSwPlanes = 5;
+__device__ __forceinline__ int sw_slot(int kk) {  // kk >= -2
+  return (kk + 2 * kSwPlanes) % kSwPlanes;
+}
 constexpr int kSwRingRow = kSwTX / 2 + 1;  // first-colour cells per ring row
 
 template <class T>
@@ -1436,7 +1443,7 @@ struct SwPairCoef {
 template <class T, bool kZero, bool kMean = false>
 __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
   using C = typename Compute<T>::type;
-  __shared__ C buf[4][kSwPlane];
+  __shared__ C buf[kSwPlanes][kSwPlane];
   __shared__ double red[16];
   double msum = 0.0;
   const Lv<T>& L = a.L;
@@ -1578,7 +1585,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     }
   };
   auto store_plane = [&](int kk, const C (&v)[kSwLoadSlots]) {
-    C* P = buf[kk & 3];
+    C* P = buf[sw_slot(kk)];
 #pragma unroll
     for (int s = 0; s < kSwLoadSlots; ++s) {
       const int p = tid + s * kSwThreads;
@@ -1738,9 +1745,9 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
     SwCoef<C> next[2]{};
     uint32_t act_next[2] = {0u, 0u};
     if (first_plane) {
-      const C* Pm = buf[(k - 1) & 3];
-      C* P0 = buf[k & 3];
-      const C* Pp = buf[(k + 1) & 3];
+      const C* Pm = buf[sw_slot(k - 1)];
+      C* P0 = buf[sw_slot(k)];
+      const C* Pp = buf[sw_slot(k + 1)];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         if (!rvalid[s]) continue;
@@ -1757,11 +1764,13 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
       }
       if (ring_ok) update(fr, Pm, P0, Pp, ring_q);
     }
-    __syncthreads();
+    // no barrier before the second colour: it reads first-colour cells of
+    // plane kb written in the previous step, and plane k only at its own
+    // pair, written by this lane above
     if (second_plane) {
-      const C* Pm = buf[(kb - 1) & 3];
-      C* P0 = buf[kb & 3];
-      const C* Pp = buf[(kb + 1) & 3];
+      const C* Pm = buf[sw_slot(kb - 1)];
+      C* P0 = buf[sw_slot(kb)];
+      const C* Pp = buf[sw_slot(kb + 1)];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         if (!rvalid[s]) continue;
@@ -1769,7 +1778,7 @@ __global__ void __launch_bounds__(kSwThreads) k_rb_sweep(SweepArgs<T> a) {
         const int q = (2 * lane + 2 + (hi ? 1 : 0)) + kSwPX * (rr[s] + 2);
         if (lane_in) update(carry[s], Pm, P0, Pp, q);
       }
-      __syncthreads();
+      // both cells of a stored pair were written by this lane
       const PlaneOff pb = plane_off(kb);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
