@@ -369,6 +369,8 @@ enum ScalarSlot : int {
   S_SUM = 52,
   S_SUM2 = 53,
   S_LINE_FAIL = 54,  // nonzero after a line relaxation pivot vanished
+  S_NSCALE = 55,     // scaled norm: largest magnitude
+  S_NSUM = 56,       // scaled norm: sum of scaled squares
   S_COUNT = 64
 };
 
@@ -459,6 +461,74 @@ k_finalize(const double* partials, int64_t blocks, int64_t stride, int nacc,
 #include "hf_cycle.hpp"
 
 namespace hf {
+
+// Scaled 2-norm (global_norm, src/solver_krylov.cpp:650-711): the largest
+// magnitude, then the compensated sum of squares of the elements divided by
+// it.  The solvers take it where a plain sum of squares overflowed, so a
+// norm overflows only where the reference's does.  A non-finite element
+// makes its block's maximum +inf.
+__device__ __forceinline__ double block_max(double v, double* scratch) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    v = fmax(v, __shfl_down(v, offset, kWave));
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double m = 0.0;
+  if (threadIdx.x == 0) {
+    const int waves = (blockDim.x + kWave - 1) / kWave;
+    for (int w = 0; w < waves; ++w) m = fmax(m, scratch[w]);
+  }
+  __syncthreads();
+  return m;
+}
+
+__global__ void k_absmax(Chunk ch, const double* v, double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int lo = lb * ch.chunk;
+  const int hi = min(ch.n, lo + ch.chunk);
+  double m = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+    const double a = v[c];
+    m = isfinite(a) ? fmax(m, fabs(a)) : INFINITY;
+  }
+  m = block_max(m, scratch);
+  if (threadIdx.x == 0) partials[lb] = m;
+}
+
+__global__ void __launch_bounds__(1024)
+k_max_fold(const double* partials, int64_t count, double* scalars, int slot) {
+  __shared__ double scratch[16];
+  double m = 0.0;
+  for (int64_t b = threadIdx.x; b < count; b += blockDim.x)
+    m = fmax(m, partials[b]);
+  m = block_max(m, scratch);
+  if (threadIdx.x == 0) scalars[slot] = m;
+}
+
+// Per block: the Neumaier-compensated sum of (v / scale)^2 of each thread's
+// elements, then the block's sum of those.
+__global__ void k_scaled_squares(Chunk ch, const double* v,
+                                 const double* scale, double* partials) {
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int lo = lb * ch.chunk;
+  const int hi = min(ch.n, lo + ch.chunk);
+  const double s = *scale;
+  double sum = 0.0, correction = 0.0;
+  for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+    const double q = v[c] / s;
+    const double square = q * q;
+    const double next = sum + square;
+    correction += fabs(sum) >= fabs(square) ? (sum - next) + square
+                                            : (square - next) + sum;
+    sum = next;
+  }
+  sum = block_sum(sum + correction, scratch);
+  if (threadIdx.x == 0) partials[lb] = sum;
+}
 
 // Adjoint: grad_logrho(c) = -sum over the six faces f of c of
 //   (lam_L - lam_R)(p_L - p_R) * d beta_f / d m_c * d m_c / d log rho_c

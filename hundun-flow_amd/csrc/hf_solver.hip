@@ -676,6 +676,32 @@ hf_status read_scalars(hf_plan* p) {
   return ok();
 }
 
+// The scaled 2-norm of a cells-long device vector (global_norm,
+// src/solver_krylov.cpp:650-711), taken where a plain sum of squares is not
+// finite: *finite is false when an element is not finite (the reference's
+// non-finite failure), else *norm is max|v| sqrt(sum (v / max|v|)^2).
+hf_status stable_norm(hf_plan* p, const double* v, double* norm,
+                      bool* finite) {
+  const Chunk ch = p->chunk0;
+  const int nb = chunk_blocks(ch);
+  HF_LAUNCH(k_absmax, nb, kBlock, 0, p->stream, ch, v, p->partials);
+  HF_LAUNCH(k_max_fold, 1, 1024, 0, p->stream,
+            static_cast<const double*>(p->partials), static_cast<int64_t>(nb),
+            p->scalars, static_cast<int>(S_NSCALE));
+  HF_TRY(read_scalars(p));
+  const double scale = p->host_scalars[S_NSCALE];
+  *norm = 0.0;
+  *finite = std::isfinite(scale);
+  if (!*finite || scale == 0.0) return ok();
+  HF_LAUNCH(k_scaled_squares, nb, kBlock, 0, p->stream, ch, v,
+            static_cast<const double*>(p->scalars + S_NSCALE), p->partials);
+  finalize(p, nb, 1, F_STORE, S_NSUM);
+  HF_TRY(read_scalars(p));
+  *norm = scale * std::sqrt(p->host_scalars[S_NSUM]);
+  *finite = std::isfinite(*norm);
+  return ok();
+}
+
 // ---- multigrid ------------------------------------------------------------
 Widths widths(const hf_plan* p) {
   return Widths{p->d_widths[0], p->d_widths[1], p->d_widths[2]};
@@ -1398,6 +1424,11 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
     }
     res->operator_applies++;
     *norm = std::sqrt(p->fg_host[0]);
+    if (!std::isfinite(*norm)) {
+      bool finite = false;
+      HF_TRY(stable_norm(p, out, norm, &finite));
+      if (!finite) *norm = std::numeric_limits<double>::quiet_NaN();
+    }
     *term = std::isfinite(*norm) ? 0 : HF_TERM_OPERATOR_FAILURE;
     return ok();
   };
@@ -1442,7 +1473,12 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
   HF_LAUNCH(k_dot, nbv, kBlock, 0, p->stream, ch, rhs, rhs, p->partials);
   finalize(p, nbv, 1, F_STORE, S_SUM);
   HF_TRY(read_scalars(p));
-  const double rhs_norm = std::sqrt(p->host_scalars[S_SUM]);
+  double rhs_norm = std::sqrt(p->host_scalars[S_SUM]);
+  if (!std::isfinite(rhs_norm)) {
+    bool finite = false;
+    HF_TRY(stable_norm(p, rhs, &rhs_norm, &finite));
+    if (!finite) rhs_norm = std::numeric_limits<double>::quiet_NaN();
+  }
   res->rhs_norm = rhs_norm;
   if (!std::isfinite(rhs_norm))
     return fail_term(HF_NUMERICAL_FAILURE, HF_TERM_NON_FINITE);
@@ -2287,7 +2323,13 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   finalize(p, tr.blocks, 1, F_STORE, S_SUM, 0.0, 1, 2);
   finalize(p, tr.blocks, 1, F_STORE, S_RNORM2);
   HF_TRY(read_scalars(p));
-  const double bnorm = std::sqrt(p->host_scalars[S_SUM]);
+  double bnorm = std::sqrt(p->host_scalars[S_SUM]);
+  if (!std::isfinite(bnorm)) {
+    // the squares overflowed, or b holds a non-finite value
+    bool finite = false;
+    HF_TRY(stable_norm(p, rhs, &bnorm, &finite));
+    if (!finite) bnorm = std::numeric_limits<double>::quiet_NaN();
+  }
   res->rhs_norm = bnorm;
   if (!std::isfinite(bnorm))
     return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
@@ -2299,6 +2341,12 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   }
   res->operator_applies++;
   res->initial_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
+  if (!std::isfinite(res->initial_true_residual)) {
+    bool finite = false;
+    HF_TRY(stable_norm(p, r, &res->initial_true_residual, &finite));
+    if (!finite)
+      res->initial_true_residual = std::numeric_limits<double>::quiet_NaN();
+  }
   res->final_true_residual = res->initial_true_residual;
   res->recursive_residual = res->initial_true_residual;
   if (!std::isfinite(res->initial_true_residual))
@@ -2410,7 +2458,12 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       HF_TRY(read_scalars(p));
       if (rho_failed()) return fail_rho();
       if (const int term = step_failure()) return fail_step(term);
-      const double recursive = std::sqrt(p->host_scalars[S_RNORM2]);
+      double recursive = std::sqrt(p->host_scalars[S_RNORM2]);
+      if (!std::isfinite(recursive)) {
+        bool finite = false;
+        HF_TRY(stable_norm(p, r, &recursive, &finite));
+        if (!finite) recursive = std::numeric_limits<double>::quiet_NaN();
+      }
       if (!std::isfinite(recursive))
         return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_NON_FINITE);
       res->recursive_residual = recursive;
@@ -2434,6 +2487,12 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       }
       res->operator_applies++;
       res->final_true_residual = std::sqrt(p->host_scalars[S_RNORM2]);
+      if (!std::isfinite(res->final_true_residual)) {
+        bool finite = false;
+        HF_TRY(stable_norm(p, r, &res->final_true_residual, &finite));
+        if (!finite)
+          res->final_true_residual = std::numeric_limits<double>::quiet_NaN();
+      }
       res->recursive_residual = res->final_true_residual;
       if (!std::isfinite(res->final_true_residual))
         return finish(fail(HF_NUMERICAL_FAILURE, kDetailNonFinite), HF_TERM_OPERATOR_FAILURE);
