@@ -128,6 +128,77 @@ def gpu_solves(plan, b, steps, warmup, flexible, torch):
     return x, result, times
 
 
+def production_pairing(shape, steps, warmup, torch, H, dev):
+    """The reference's production pressure path at full size: the low-Mach
+    pressure-correction system assembled on the device from the compressible
+    flame case (acoustic CFL 4, BDF2, cylinder mask), then FGMRES(12) with the
+    Chebyshev-Jacobi F(1,2) cycle and the diagonal-shortcut policy
+    (src/solver_piso_detail.hpp:10-19), FP64 and FP32 cycle storage, to 1e-10.
+    Times are per solve (assembly and hierarchy build excluded, timed apart)."""
+    import numpy as np
+
+    import hf_cases
+    nx, ny, nz = shape
+    log(f"[bench] production pairing: compressible_flame {nx}x{ny}x{nz}")
+    case = hf_cases.compressible_flame(n=(nx, ny, nz), seed=6,
+                                       acoustic_cfl=4.0)
+    tt = lambda a: None if a is None else torch.from_numpy(
+        np.ascontiguousarray(a)).to(dev)
+    masks = [tt(case[k]) for k in ("cell_active", "x_active", "y_active",
+                                   "z_active")]
+    inputs = dict(phi=[tt(v) for v in case["phi"]], rho=tt(case["rho"]),
+                  rho_n=tt(case["rho_n"]), drho_dp=tt(case["drho_dp"]),
+                  rho_nm1=tt(case["rho_nm1"]), source=tt(case["source"]))
+    mobility = tt(case["mobility"])
+    out = {"workload": f"compressible_flame {nx}x{ny}x{nz}, acoustic CFL 4",
+           "krylov": "FGMRES(12), single reduction",
+           "cycle": "F(1,2) Chebyshev-Jacobi, diagonal shortcut policy"}
+    for prec in ("fp64", "fp32"):
+        pol = H.default_policy(pre_sweeps=1, post_sweeps=2, point_smoother=1,
+                               f_cycle=1, chebyshev_lower_fraction=0.3,
+                               diagonal_shortcut_ratio=0.1,
+                               null_space_constant=0, precision=prec)
+        plan = H.Plan(case["n"], case["faces"], case["bc"], pol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, bx, by, bz = plan.assemble(mobility)
+        d, b = plan.assemble_pressure_system(
+            bx, by, bz, inputs["phi"], inputs["rho"], inputs["rho_n"],
+            inputs["drho_dp"], case["bdf"], rho_nm1=inputs["rho_nm1"],
+            source=inputs["source"])
+        b = b * masks[0].to(b.dtype)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.set_coefficients(d, bx, by, bz, *masks)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res = None
+        times = []
+        for i in range(warmup + steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            x, res = plan.solve_fgmres(b, rtol=1e-10, restart=12,
+                                       maximum_iterations=400)
+            torch.cuda.synchronize()
+            if i >= warmup:
+                times.append((time.perf_counter() - t) * 1e3)
+        ax = plan.operator_apply(x)
+        true_rel = float(torch.linalg.norm(b - ax) / torch.linalg.norm(b))
+        log(f"[bench] production {prec}: iterations={res.iterations} "
+            f"ms={['%.2f' % v for v in times]} true_rel={true_rel:.3e}")
+        assert res.termination == "converged", res
+        assert true_rel <= 1.01e-10, true_rel
+        out[prec] = {"ms_per_solve": round(sum(times) / len(times), 3),
+                     "iterations": res.iterations,
+                     "true_relative_residual": true_rel,
+                     "assembly_ms": round((t1 - t0) * 1e3, 3),
+                     "hierarchy_build_ms": round((t2 - t1) * 1e3, 3)}
+        plan.close()
+    out["speedup_fp32_vs_fp64"] = round(out["fp64"]["ms_per_solve"] /
+                                        out["fp32"]["ms_per_solve"], 3)
+    return out
+
+
 def cpu_port_sample(case):
     """The oracle (single-thread C restatement) on the full grid: one V(2,2)
     application + one operator application + three dot passes, i.e. one
@@ -319,6 +390,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (x10) as "
                          "the known-size reference for HBM counters")
+    ap.add_argument("--no-production", dest="production",
+                    action="store_false",
+                    help="skip the production pairing (FGMRES + Chebyshev "
+                         "F-cycle on the assembled compressible system)")
     ap.add_argument("--cpu-only", type=int, default=0, metavar="ITERATIONS",
                     help="only time the CPU baseline (reference CPU path, or "
                          "the C oracle without its build) for a solve of this "
@@ -491,6 +566,13 @@ def main():
                               "speedup_vs_fp64": round(mean_ms(r64) /
                                                        mean_ms(r32), 3)})
 
+    production = None
+    if args.production and args.workload == "flame_step":
+        for key in list(plans):
+            plans.pop(key).close()
+        production = production_pairing((nx, ny, nz), args.steps, args.warmup,
+                                         torch, H, dev)
+
     cpu = None
     if not args.no_cpu_baseline:
         # the reference CPU algorithm: reference aggregation, FP64 PCG
@@ -528,6 +610,7 @@ def main():
         "true_relative_residual": main["true_rel"],
         **extra,
         "reference_hierarchy": reference,
+        "production_pairing": production,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
