@@ -571,9 +571,10 @@ bool op_on_tail(const hf_plan* p, const CycleOp& op) {
   return lo >= p->tail_first;
 }
 
-// k_defect_restrict applies on level l: a fused-sweep level (its retained
-// defect comes from one stencil pass), coarsened along every axis with even
-// extents, full z-marching tiles of even width and height, even z-chunks.
+// k_defect_restrict applies on level l: a fused-sweep or Chebyshev level
+// (its retained defect comes from one stencil pass after the smoothing),
+// coarsened along every axis with even extents, full z-marching tiles of
+// even width and height, even z-chunks.
 bool defect_restrict_ok(const hf_plan* p, int l) {
   if (!defect_restrict_enabled() || l + 1 >= static_cast<int>(p->levels.size()))
     return false;
@@ -581,7 +582,7 @@ bool defect_restrict_ok(const hf_plan* p, int l) {
   const Level& Cn = p->levels[l + 1];
   const Tiles& t = L.tz;
   const int tw = 1 << t.txl, th = kBlock >> t.txl;
-  return L.sweep && L.coarsen_mask == 7 && L.n[0] % 2 == 0 &&
+  return (L.sweep || L.cheb) && L.coarsen_mask == 7 && L.n[0] % 2 == 0 &&
          L.n[1] % 2 == 0 && L.n[2] % 2 == 0 && Cn.n[0] * 2 == L.n[0] &&
          Cn.n[1] * 2 == L.n[1] && Cn.n[2] * 2 == L.n[2] && tw % 2 == 0 &&
          th % 2 == 0 && L.n[0] % tw == 0 && L.n[1] % th == 0 &&

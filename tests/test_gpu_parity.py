@@ -561,16 +561,22 @@ DEFECT_RESTRICT = {
 @pytest.mark.parametrize("name", list(DEFECT_RESTRICT))
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("null", [0, 1])
+@pytest.mark.parametrize("smoother", ["red_black", "chebyshev"])
 def test_defect_restrict_fused_matches_two_kernels(torch_cuda, monkeypatch,
-                                                    name, precision, null):
+                                                    name, precision, null,
+                                                    smoother):
     """The fused defect + restriction stores the coarse b of the two
     kernels bit for bit, so the cycle is unchanged without the null space;
-    with it, the coarse mean's partial sums come in another block order."""
+    with it, the coarse mean's partial sums come in another block order.
+    Fused-sweep levels and Chebyshev levels (the production F-cycle) alike."""
     torch = torch_cuda
-    _say(f"[defect restrict] case={name} precision={precision} null={null}")
+    _say(f"[defect restrict] case={name} precision={precision} null={null} "
+         f"smoother={smoother}")
     case = DEFECT_RESTRICT[name]()
     pol = dict(null_space_constant=null, anisotropy_threshold=1e9,
                precision=precision)
+    if smoother == "chebyshev":
+        pol.update(point_smoother=1, f_cycle=1, pre_sweeps=1, post_sweeps=2)
     r = _t(torch, np.random.default_rng(6).standard_normal(
         int(np.prod(case["n"]))))
     out = {}
