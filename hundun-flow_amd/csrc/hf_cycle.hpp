@@ -852,11 +852,21 @@ struct TailLevel {
   Lv<double> L;      // coefficients in global memory; x/b/r live in LDS
   int coarsen_mask;
   int fused;
+  int cheb;          // Chebyshev-Jacobi smoothing (else red-black)
   double volume;
   int lds;           // offset (doubles) of this level's x, b, r in LDS
 };
 
 constexpr int kTailLdsBytes = 64 * 1024;
+
+// Chebyshev-Jacobi stage factors of the tail's Chebyshev levels, stage s:
+// direction d = pf[s] d + rf[s] (b - A x) / D (d = rf[0] (b - A x) / D at
+// s = 0).  The host fills them with the recurrence of the grid levels.
+constexpr int kTailChebStages = 16;
+struct TailCheb {
+  double rf[kTailChebStages];
+  double pf[kTailChebStages];
+};
 
 __device__ __forceinline__ void tail_ijk(const Lv<double>& L, int c, int& i,
                                          int& j, int& k) {
@@ -960,6 +970,63 @@ __device__ void tail_smooth(const Lv<double>& L, int fused, const Bc& bc,
   }
 }
 
+// Chebyshev-Jacobi smoothing of a tail level (src/solver_mg.cpp:2916-3090):
+// the stages of k_cheb_stage with x updated in place.  A stage stores its
+// direction in r, and x += r follows a barrier, so every neighbour the
+// stencil reads is the stage's input x (Jacobi), the operands and order of
+// operations being those of the grid kernel.  retain: then r = b - A x.
+__device__ void tail_cheb(const Lv<double>& L, const Bc& bc, int stages,
+                          int retain, const TailCheb& ch) {
+  if (L.cells <= static_cast<int>(blockDim.x)) {
+    // at most one cell per thread: its coefficients stay in registers
+    const int c = threadIdx.x;
+    const bool mine = c < L.cells;
+    int i = 0, j = 0, k = 0;
+    StencilCoef<double> f{};
+    if (mine) {
+      tail_ijk(L, c, i, j, k);
+      f = stencil_coef<double>(L, j, k, c);
+    }
+    for (int s = 0; s < stages; ++s) {
+      double xn = 0.0;
+      if (mine) {
+        const double xc = L.x[c];
+        const double v = stencil_with<double>(L, bc, L.x, i, j, k, c, xc, f);
+        const double defect = L.b[c] - v;
+        const double scaled = ch.rf[s] * defect / f.d;
+        const double dir = s == 0 ? scaled : ch.pf[s] * L.r[c] + scaled;
+        L.r[c] = dir;
+        xn = xc + dir;
+      }
+      __syncthreads();
+      if (mine) L.x[c] = xn;
+      __syncthreads();
+    }
+    if (retain) {
+      if (mine)
+        L.r[c] = L.b[c] - stencil_with<double>(L, bc, L.x, i, j, k, c, L.x[c], f);
+      __syncthreads();
+    }
+    return;
+  }
+  for (int s = 0; s < stages; ++s) {
+    for (int c = threadIdx.x; c < L.cells; c += blockDim.x) {
+      int i, j, k;
+      tail_ijk(L, c, i, j, k);
+      const double xc = L.x[c];
+      const double v = stencil<double>(L, bc, L.x, i, j, k, c, xc);
+      const double defect = L.b[c] - v;
+      const double scaled = ch.rf[s] * defect / L.D[c];
+      L.r[c] = s == 0 ? scaled : ch.pf[s] * L.r[c] + scaled;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < L.cells; c += blockDim.x)
+      L.x[c] = L.x[c] + L.r[c];
+    __syncthreads();
+  }
+  if (retain) tail_apply(L, bc, 1);
+}
+
 __device__ void tail_project(const Lv<double>& L, double volume, double* field,
                              double* scratch, double* mean) {
   double v = 0.0;
@@ -978,7 +1045,7 @@ __device__ void tail_project(const Lv<double>& L, double volume, double* field,
 // grid-level restriction; its x is stored back for the grid prolongation.
 __global__ void __launch_bounds__(1024)
 k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
-       Bc bc, int periodic_mask, int null_space) {
+       Bc bc, int periodic_mask, int null_space, TailCheb cheb) {
   extern __shared__ double lds[];
   __shared__ double scratch[16];
   __shared__ double mean;
@@ -996,7 +1063,10 @@ k_tail(const TailLevel* levels, int first_level, const TailOp* ops, int nops,
     const TailLevel& T = levels[op.level - first_level];
     const Lv<double> L = tail_view(T, lds);
     if (op.kind == T_SMOOTH) {
-      tail_smooth(L, T.fused, bc, op.sweeps, op.reverse, op.retain);
+      if (T.cheb)
+        tail_cheb(L, bc, op.sweeps, op.retain, cheb);
+      else
+        tail_smooth(L, T.fused, bc, op.sweeps, op.reverse, op.retain);
     } else if (op.kind == T_RESTRICT) {
       const TailLevel& Cn = levels[op.level + 1 - first_level];
       const Lv<double> Cl = tail_view(Cn, lds);

@@ -269,6 +269,7 @@ struct hf_plan {
   std::vector<ExecItem> exec;
   std::vector<TailOp> tail_ops_host;
   TailOp* d_tail_ops = nullptr;
+  TailCheb tail_cheb{};  // stage factors of the tail's Chebyshev levels
   TailLevel* d_tail_levels = nullptr;
   size_t tail_lds_bytes = 0;
   bool coefficients_ready = false;
@@ -854,6 +855,31 @@ void line_smooth(hf_plan* p, int l, int sweeps, int reverse) {
   }
 }
 
+// Stage factors of Chebyshev-Jacobi smoothing over the spectrum
+// [2 fraction, 2] of D^-1 A (src/solver_mg.cpp:2916-3090): stage s moves x
+// by d = pf d + rf (b - A x) / D, from pf = 0, rf = 1 / theta at s = 0 and
+// the three-term recurrence of rho after it.  One sequence serves the grid
+// levels and the tail's table, so both apply the same values.
+struct ChebStage {
+  double theta, delta, sigma, previous_rho;
+  int stage = 0;
+  explicit ChebStage(double fraction) {
+    const double upper = 2.0;
+    const double lower = fraction * upper;
+    theta = (upper + lower) * 0.5;
+    delta = (upper - lower) * 0.5;
+    sigma = theta / delta;
+    previous_rho = 1.0 / sigma;
+  }
+  void next(double* rf, double* pf) {
+    const double rho = stage == 0 ? 0.0 : 1.0 / (2.0 * sigma - previous_rho);
+    *pf = stage == 0 ? 0.0 : rho * previous_rho;
+    *rf = stage == 0 ? 1.0 / theta : 2.0 * rho / delta;
+    if (stage != 0) previous_rho = rho;
+    ++stage;
+  }
+};
+
 // Chebyshev-Jacobi smoothing of level l (src/solver_mg.cpp:2916-3090):
 // `stages` Jacobi stages with the three-term recurrence over the spectrum
 // [2 fraction, 2] of D^-1 A, from the x the level holds (zero on a first
@@ -862,18 +888,11 @@ template <class T>
 void cheb_smooth(hf_plan* p, int l, int stages, int retain) {
   using C = typename Compute<T>::type;
   Level& L = p->levels[l];
-  const double upper = 2.0;
-  const double lower = p->policy.chebyshev_lower_fraction * upper;
-  const double theta = (upper + lower) * 0.5;
-  const double delta = (upper - lower) * 0.5;
-  const double sigma = theta / delta;
-  double previous_rho = 1.0 / sigma;
+  ChebStage stage_factors(p->policy.chebyshev_lower_fraction);
   const int rb_class = l == 0 ? P_RB_FINEST : P_RB_COARSE;
   for (int stage = 0; stage < stages; ++stage) {
-    const double rho = stage == 0 ? 0.0 : 1.0 / (2.0 * sigma - previous_rho);
-    const double pf = stage == 0 ? 0.0 : rho * previous_rho;
-    const double rf = stage == 0 ? 1.0 / theta : 2.0 * rho / delta;
-    if (stage != 0) previous_rho = rho;
+    double rf = 0.0, pf = 0.0;
+    stage_factors.next(&rf, &pf);
     {
       HF_PROF(p, rb_class);
       HF_LAUNCH(k_cheb_stage<T>, L.tz.blocks, kBlock, 0, p->stream, L.tz,
@@ -1057,7 +1076,8 @@ void run_cycle(hf_plan* p, double* mean_partials = nullptr) {
     HF_LAUNCH(k_tail, 1, 1024, p->tail_lds_bytes, p->stream, p->d_tail_levels,
               p->tail_first,
               p->d_tail_ops + e.tail_begin, e.tail_count, p->bc,
-              p->periodic_mask, static_cast<int>(p->policy.null_space_constant));
+              p->periodic_mask, static_cast<int>(p->policy.null_space_constant),
+              p->tail_cheb);
   }
 }
 
@@ -1175,6 +1195,7 @@ hf_status upload_tail(hf_plan* p) {
     t.L = view<double>(p->levels[l]);
     t.coarsen_mask = p->levels[l].coarsen_mask;
     t.fused = p->levels[l].fused;
+    t.cheb = p->levels[l].cheb ? 1 : 0;
     t.volume = p->levels[l].volume;
     t.lds = static_cast<int>(lds);
     lds += 3 * p->levels[l].cells;
@@ -1788,10 +1809,13 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   p->tail_first = static_cast<int>(p->levels.size());
   int64_t lds_doubles = 0;
   // line levels (every level of a line plan) are smoothed by the line
-  // kernels, never in the tail
+  // kernels, never in the tail; Chebyshev levels join it while their stage
+  // counts fit its table of stage factors
   const bool cheb_plan = policy->point_smoother == 1;
+  const bool cheb_fits =
+      std::max(policy->pre_sweeps, policy->post_sweeps) <= kTailChebStages;
   for (int l = static_cast<int>(p->levels.size()) - 1;
-       l >= 0 && !line_plan && !cheb_plan; --l) {
+       l >= 0 && !line_plan && (!cheb_plan || cheb_fits); --l) {
     const int64_t need = lds_doubles + 3 * p->levels[l].cells;
     if (p->levels[l].cells <= threshold &&
         need * static_cast<int64_t>(sizeof(double)) <= kTailLdsBytes) {
@@ -1800,6 +1824,11 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
     } else {
       break;
     }
+  }
+  {
+    ChebStage stage_factors(policy->chebyshev_lower_fraction);
+    for (int s = 0; s < kTailChebStages; ++s)
+      stage_factors.next(&p->tail_cheb.rf[s], &p->tail_cheb.pf[s]);
   }
   // the coarsest level is always FP64 and, on point plans, handled by the
   // tail kernel

@@ -199,6 +199,66 @@ def test_mg_apply_projected(torch_cuda, monkeypatch, name, unit_linear, tail,
     plan.close()
 
 
+_CHEB = dict(point_smoother=1, chebyshev_lower_fraction=0.3)
+
+
+@pytest.mark.parametrize("name", ["mixed", "odd_periodic", "mixed_wide",
+                                  "periodic_xy"])
+@pytest.mark.parametrize("stages", [(1, 2), (3, 2)])
+@pytest.mark.parametrize("f_cycle", [1, 0])
+@pytest.mark.parametrize("tail", ["default", "0"])
+def test_mg_apply_chebyshev_bitwise(torch_cuda, monkeypatch, name, stages,
+                                    f_cycle, tail):
+    """Chebyshev-Jacobi cycles without a null space are fixed-order FP64
+    computations: the GPU cycle equals the oracle bit for bit, with the
+    coarse Chebyshev levels inside the single-workgroup tail (default
+    threshold) and as grid-level stage kernels (threshold 0)."""
+    torch = torch_cuda
+    _say(f"[mg chebyshev bitwise] case={name} stages={stages} "
+         f"f_cycle={f_cycle} tail={tail}")
+    if tail != "default":
+        monkeypatch.setenv("HF_TAIL_CELLS", tail)
+    case = CASES[name]()
+    pol = _policy(name, f_cycle=f_cycle, pre_sweeps=stages[0],
+                  post_sweeps=stages[1], **_CHEB)
+    plan = _gpu_plan(torch, case, **pol)
+    levels = plan.level_count()
+    on_tail = [plan.level_info(l)["on_tail"] for l in range(levels)]
+    # the default threshold takes Chebyshev levels (all but the coarsest)
+    # into the tail; threshold 0 leaves it the coarsest alone
+    assert sum(on_tail) >= (2 if tail == "default" else 1), on_tail
+    assert tail == "default" or sum(on_tail) == 1, on_tail
+    _, mg = O.from_case(case, O.policy(**pol))
+    r = np.random.default_rng(7).standard_normal(plan.cells)
+    z = plan.mg_apply(_t(torch, r)).cpu().numpy()
+    zo = mg.apply(r)
+    assert np.array_equal(z, zo), np.max(np.abs(z - zo))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["sphere32", "step"])
+@pytest.mark.parametrize("tail", ["default", "0"])
+def test_mg_apply_chebyshev_projected(torch_cuda, monkeypatch, name, tail):
+    """Chebyshev F-cycle with the constant null space (projection means from
+    a parallel reduction): agreement to 1e-12 relative through the tail and
+    through grid-level stage kernels."""
+    torch = torch_cuda
+    _say(f"[mg chebyshev projected] case={name} tail={tail}")
+    if tail != "default":
+        monkeypatch.setenv("HF_TAIL_CELLS", tail)
+    case = CASES[name]()
+    pol = _policy(name, f_cycle=1, pre_sweeps=1, post_sweeps=2, **_CHEB)
+    plan = _gpu_plan(torch, case, **pol)
+    _, mg = O.from_case(case, O.policy(**pol))
+    r = case["rhs"]
+    z = plan.mg_apply(_t(torch, r)).cpu().numpy()
+    zo = mg.apply(r)
+    rel = np.linalg.norm(z - zo) / np.linalg.norm(zo)
+    _say(f"  rel={rel:.3e}")
+    assert rel <= 1e-12
+    plan.close()
+
+
 def _gauge(x, act):
     fluid = act != 0
     y = x.copy()
