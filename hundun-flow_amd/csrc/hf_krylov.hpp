@@ -82,6 +82,42 @@ __global__ void k_cheb_stage(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
   }
 }
 
+// First Chebyshev-Jacobi stage of a level's first visit in a cycle, where x
+// holds zero: A x = +0 on every row (non-negative faces, positive D), so the
+// stage's d = rf (b - A x) / D is rf b / D and x_next = 0 + d, the values
+// k_cheb_stage stores there, from b and D alone.  Four consecutive cells per
+// thread.
+template <class T>
+__global__ void k_cheb_first(int cells, Lv<T> L, T* xn,
+                             typename Compute<T>::type rf) {
+  using C = typename Compute<T>::type;
+  struct alignas(4 * sizeof(T)) Quad {
+    T v[4];
+  };
+  const int64_t c0 =
+      4 * (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+  if (c0 >= cells) return;
+  if (c0 + 4 <= cells) {
+    const Quad b = *reinterpret_cast<const Quad*>(L.b + c0);
+    const Quad d = *reinterpret_cast<const Quad*>(L.D + c0);
+    Quad r, x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const C dir = rf * C(ld(b.v[e])) / C(ld(d.v[e]));
+      r.v[e] = st<T>(dir);
+      x.v[e] = st<T>(C(0) + dir);
+    }
+    *reinterpret_cast<Quad*>(L.r + c0) = r;
+    *reinterpret_cast<Quad*>(xn + c0) = x;
+    return;
+  }
+  for (int64_t c = c0; c < cells; ++c) {
+    const C dir = rf * C(ld(L.b[c])) / C(ld(L.D[c]));
+    L.r[c] = st<T>(dir);
+    xn[c] = st<T>(C(0) + dir);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Diagonal shortcut (src/solver_mg.cpp:1573-1611 selects it, :4926-4936
 // applies it): x = b / D on level 0.
@@ -137,16 +173,21 @@ enum FgMode : int { kFgArnoldi = 0, kFgResidual = 1 };
 // Level-0 FP64 action y = A' x, A' = A or A divided by the geometric cell
 // volume (the per-volume rows of the cold system), z-marching as
 // k_stencil_z, with the FGMRES epilogues:
-//  kFgArnoldi:  out = A' x - shift * aux (aux = v_col); accumulator a < nd
+//  kFgArnoldi:  out = A' x - shift * aux (aux = v_col); accumulator a < kNd
 //               holds basis_a . out, accumulator wacc holds out . out;
-//  kFgResidual: out = 1 b + (-1) A' x (b = aux); accumulator 0 = out . out.
-// Partials of accumulator a sit at partials + a * pstride.
-template <int kMode>
+//  kFgResidual: out = 1 b + (-1) A' x (b = aux); accumulator 0 = out . out
+//               (kNd = 0).
+// Partials of accumulator a sit at partials + a * pstride.  One instance
+// per dot count, so the kNd basis loads of a step are unconditional and
+// issue together: at 512 x 256 x 256 a basis vector adds 56 us to a launch,
+// against 85 us when a runtime count guarded each load.
+template <int kMode, int kNd>
 __global__ void k_fg_stencil(Tiles t, Lv<double> A, Bc bc, const double* x,
                              const double* aux, double* out,
-                             const double* basis, int64_t vstride, int nd,
+                             const double* basis, int64_t vstride,
                              int wacc, double shift, int volume_rows,
                              Widths w, double* partials, int64_t pstride) {
+  static_assert(kNd >= 0 && kNd <= kFgDots, "dots of one pass");
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
   const int gxy = t.gx * t.gy;
@@ -156,9 +197,9 @@ __global__ void k_fg_stencil(Tiles t, Lv<double> A, Bc bc, const double* x,
   const int nx = t.nx, ny = t.ny, nz = t.nz;
   const int k0 = zb * t.per;
   const int k1 = min(nz, k0 + t.per);
-  double acc[kFgDots];
+  double acc[kNd > 0 ? kNd : 1];
 #pragma unroll
-  for (int a = 0; a < kFgDots; ++a) acc[a] = 0.0;
+  for (int a = 0; a < kNd; ++a) acc[a] = 0.0;
   double ww = 0.0;
   if (i < nx && j < ny) {
     const int plane = nx * ny;
@@ -203,8 +244,7 @@ __global__ void k_fg_stencil(Tiles t, Lv<double> A, Bc bc, const double* x,
         const double o = v + -shift * aux[c];
         out[c] = o;
 #pragma unroll
-        for (int a = 0; a < kFgDots; ++a)
-          if (a < nd) acc[a] += basis[a * vstride + c] * o;
+        for (int a = 0; a < kNd; ++a) acc[a] += basis[a * vstride + c] * o;
         ww += o * o;
       } else {
         const double o = 1.0 * aux[c] + -1.0 * v;
@@ -219,11 +259,9 @@ __global__ void k_fg_stencil(Tiles t, Lv<double> A, Bc bc, const double* x,
   }
   if constexpr (kMode == kFgArnoldi) {
 #pragma unroll
-    for (int a = 0; a < kFgDots; ++a) {
-      if (a < nd) {
-        const double s = block_sum(acc[a], scratch);
-        if (threadIdx.x == 0) partials[a * pstride + lb] = s;
-      }
+    for (int a = 0; a < kNd; ++a) {
+      const double s = block_sum(acc[a], scratch);
+      if (threadIdx.x == 0) partials[a * pstride + lb] = s;
     }
   }
   const double s = block_sum(ww, scratch);
@@ -231,11 +269,14 @@ __global__ void k_fg_stencil(Tiles t, Lv<double> A, Bc bc, const double* x,
 }
 
 // More dots of the Arnoldi column when it has more rows than one pass
-// holds: accumulator acc0 + a = basis_(row0 + a) . wv, a < nd, in the
-// z-marching tile layout of k_fg_stencil (same partial count).
+// holds: accumulator acc0 + a = basis_(row0 + a) . wv, a < kNd, in the
+// z-marching tile layout of k_fg_stencil (same partial count; one instance
+// per dot count, as there).
+template <int kNd>
 __global__ void k_fg_dots(Tiles t, const double* wv, const double* basis,
-                          int64_t vstride, int row0, int nd, int acc0,
+                          int64_t vstride, int row0, int acc0,
                           double* partials, int64_t pstride) {
+  static_assert(kNd >= 1 && kNd <= kFgDots, "dots of one pass");
   __shared__ double scratch[16];
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
   const int gxy = t.gx * t.gy;
@@ -245,26 +286,24 @@ __global__ void k_fg_dots(Tiles t, const double* wv, const double* basis,
   const int nx = t.nx, ny = t.ny, nz = t.nz;
   const int k0 = zb * t.per;
   const int k1 = min(nz, k0 + t.per);
-  double acc[kFgDots];
+  double acc[kNd];
 #pragma unroll
-  for (int a = 0; a < kFgDots; ++a) acc[a] = 0.0;
+  for (int a = 0; a < kNd; ++a) acc[a] = 0.0;
   if (i < nx && j < ny) {
     const int plane = nx * ny;
     int c = cell_index(nx, ny, i, j, k0);
     for (int k = k0; k < k1; ++k) {
       const double o = wv[c];
 #pragma unroll
-      for (int a = 0; a < kFgDots; ++a)
-        if (a < nd) acc[a] += basis[(row0 + a) * vstride + c] * o;
+      for (int a = 0; a < kNd; ++a)
+        acc[a] += basis[(row0 + a) * vstride + c] * o;
       c += plane;
     }
   }
 #pragma unroll
-  for (int a = 0; a < kFgDots; ++a) {
-    if (a < nd) {
-      const double s = block_sum(acc[a], scratch);
-      if (threadIdx.x == 0) partials[(acc0 + a) * pstride + lb] = s;
-    }
+  for (int a = 0; a < kNd; ++a) {
+    const double s = block_sum(acc[a], scratch);
+    if (threadIdx.x == 0) partials[(acc0 + a) * pstride + lb] = s;
   }
 }
 

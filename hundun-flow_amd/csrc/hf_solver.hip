@@ -885,7 +885,7 @@ struct ChebStage {
 // [2 fraction, 2] of D^-1 A, from the x the level holds (zero on a first
 // visit: the entry or the restriction stored it), then the retained defect.
 template <class T>
-void cheb_smooth(hf_plan* p, int l, int stages, int retain) {
+void cheb_smooth(hf_plan* p, int l, int stages, int retain, int from_zero) {
   using C = typename Compute<T>::type;
   Level& L = p->levels[l];
   ChebStage stage_factors(p->policy.chebyshev_lower_fraction);
@@ -893,7 +893,14 @@ void cheb_smooth(hf_plan* p, int l, int stages, int retain) {
   for (int stage = 0; stage < stages; ++stage) {
     double rf = 0.0, pf = 0.0;
     stage_factors.next(&rf, &pf);
-    {
+    if (stage == 0 && from_zero) {
+      // x = 0 (stored by the entry or the restriction): b and D suffice
+      HF_PROF(p, rb_class);
+      const int64_t groups = (L.cells + 3) / 4;
+      HF_LAUNCH(k_cheb_first<T>, (groups + kBlock - 1) / kBlock, kBlock, 0,
+                p->stream, static_cast<int>(L.cells), view<T>(L),
+                static_cast<T*>(L.x2), static_cast<C>(rf));
+    } else {
       HF_PROF(p, rb_class);
       HF_LAUNCH(k_cheb_stage<T>, L.tz.blocks, kBlock, 0, p->stream, L.tz,
                 view<T>(L), p->bc, static_cast<const T*>(L.x),
@@ -913,7 +920,7 @@ void grid_smooth(hf_plan* p, int l, int sweeps, int reverse, int retain,
                  int from_zero, double* mean_partials = nullptr) {
   Level& L = p->levels[l];
   if (L.cheb) {
-    cheb_smooth<T>(p, l, sweeps, retain);
+    cheb_smooth<T>(p, l, sweeps, retain, from_zero);
     return;
   }
   if (L.line_mask != 0) {
@@ -1103,6 +1110,9 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
   // whole cycle inside the tail kernel, or a line level 0: x = 0 and the
   // projected b are stored before the cycle
   const bool tail0 = L0.tail || L0.line_mask != 0 || L0.cheb;
+  // (a grid Chebyshev level 0 starts with a from-zero stage, which reads
+  // b and D only: its x needs no zero)
+  const bool zero_x = L0.tail || L0.line_mask != 0;
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
@@ -1111,7 +1121,7 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
       HF_PROF(p, P_MG_ENTRY_EXIT);
       HF_LAUNCH(k_mg_entry<T>, L0.tred.blocks, kBlock, 0, p->stream, L0.tred, r,
                 static_cast<const uint8_t*>(L0.act), widths(p), b,
-                tail0 ? x : static_cast<T*>(nullptr),
+                zero_x ? x : static_cast<T*>(nullptr),
                 proj ? p->partials : static_cast<double*>(nullptr));
     }
     if (proj) {
@@ -1351,6 +1361,32 @@ int fg_back_substitute(const std::vector<double>& h, size_t rows,
   return 0;
 }
 
+// k_fg_stencil<kFgArnoldi, nd> and k_fg_dots<nd> for a dot count nd in
+// [1, kFgDots] known at run time.
+template <int N = 1, class... Args>
+void launch_fg_arnoldi(int nd, int64_t blocks, hipStream_t stream,
+                       Args... args) {
+  if constexpr (N <= kFgDots) {
+    if (nd == N) {
+      HF_LAUNCH((k_fg_stencil<kFgArnoldi, N>), blocks, kBlock, 0, stream,
+                args...);
+      return;
+    }
+    launch_fg_arnoldi<N + 1>(nd, blocks, stream, args...);
+  }
+}
+
+template <int N = 1, class... Args>
+void launch_fg_dots(int nd, int64_t blocks, hipStream_t stream, Args... args) {
+  if constexpr (N <= kFgDots) {
+    if (nd == N) {
+      HF_LAUNCH(k_fg_dots<N>, blocks, kBlock, 0, stream, args...);
+      return;
+    }
+    launch_fg_dots<N + 1>(nd, blocks, stream, args...);
+  }
+}
+
 hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
                       const hf_fgmres_control* ctl, hf_solve_result* res) {
   const size_t R = ctl->restart;
@@ -1434,9 +1470,10 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
                            int* term) -> hf_status {
     {
       HF_PROF(p, P_TRUE_RESIDUAL);
-      HF_LAUNCH(k_fg_stencil<kFgResidual>, tz.blocks, kBlock, 0, p->stream,
-                tz, A, p->bc, vec, rhs, out, static_cast<const double*>(nullptr),
-                int64_t{0}, 0, 0, 0.0, vrows, wd, p->fg_partials, pst);
+      HF_LAUNCH((k_fg_stencil<kFgResidual, 0>), tz.blocks, kBlock, 0,
+                p->stream, tz, A, p->bc, vec, rhs, out,
+                static_cast<const double*>(nullptr), int64_t{0}, 0, 0.0, vrows,
+                wd, p->fg_partials, pst);
     }
     bool bad = false;
     HF_TRY(reduce(1, &bad));
@@ -1475,14 +1512,15 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
                         int acc0) {
     for (size_t r = row0; r < row0 + count; r += kFgDots) {
       const int nd = static_cast<int>(std::min<size_t>(kFgDots, row0 + count - r));
-      HF_LAUNCH(k_fg_dots, tz.blocks, kBlock, 0, p->stream, tz, wv, p->fg_v, N,
-                static_cast<int>(r), nd, acc0 + static_cast<int>(r - row0),
-                p->fg_partials, pst);
+      launch_fg_dots(nd, tz.blocks, p->stream, tz, wv,
+                     static_cast<const double*>(p->fg_v), N,
+                     static_cast<int>(r), acc0 + static_cast<int>(r - row0),
+                     p->fg_partials, pst);
     }
   };
   auto norm_of = [&](const double* wv, double* norm, bool* bad) -> hf_status {
-    HF_LAUNCH(k_fg_dots, tz.blocks, kBlock, 0, p->stream, tz, wv, wv,
-              int64_t{0}, 0, 1, 0, p->fg_partials, pst);
+    HF_LAUNCH(k_fg_dots<1>, tz.blocks, kBlock, 0, p->stream, tz, wv, wv,
+              int64_t{0}, 0, 0, p->fg_partials, pst);
     HF_TRY(reduce(1, bad));
     *norm = std::sqrt(p->fg_host[0]);
     return ok();
@@ -1557,12 +1595,13 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
       const size_t dots = col + 1;
       {
         HF_PROF(p, P_OPERATOR);
-        HF_LAUNCH(k_fg_stencil<kFgArnoldi>, tz.blocks, kBlock, 0, p->stream,
-                  tz, A, p->bc, static_cast<const double*>(Z(col)),
-                  static_cast<const double*>(V(col)), ax,
-                  static_cast<const double*>(p->fg_v), N,
-                  static_cast<int>(std::min<size_t>(dots, kFgDots)),
-                  static_cast<int>(dots), 1.0, vrows, wd, p->fg_partials, pst);
+        launch_fg_arnoldi(static_cast<int>(std::min<size_t>(dots, kFgDots)),
+                          tz.blocks, p->stream, tz, A, p->bc,
+                          static_cast<const double*>(Z(col)),
+                          static_cast<const double*>(V(col)), ax,
+                          static_cast<const double*>(p->fg_v), N,
+                          static_cast<int>(dots), 1.0, vrows, wd,
+                          p->fg_partials, pst);
         if (dots > static_cast<size_t>(kFgDots))
           basis_dots(ax, kFgDots, dots - kFgDots, kFgDots);
       }
@@ -2305,9 +2344,10 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   const Lv<double> A = krylov_view(p);
   const Tiles& tr = p->levels[0].tz;  // z-marching Krylov stencils
   // the PCG update stores the preconditioner's entry when level 0 is a grid
-  // level (the tail and line-relaxed entries also zero x and project b)
+  // level (the tail and line-relaxed entries also zero x; a Chebyshev level 0
+  // reads no x in its first stage, and mg_apply projects its b)
   const bool fuse_entry = !p->levels[0].tail &&
-                          p->levels[0].line_mask == 0 && !p->levels[0].cheb;
+                          p->levels[0].line_mask == 0;
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
   double* x = p->kx;
