@@ -134,16 +134,22 @@ __global__ void k_diag_solve(Chunk ch, Lv<T> L) {
 // Largest finest-row ratio (sum of the six masked faces) / D, physical
 // boundary faces included, as the non-negative bit pattern max into *out
 // (ratios are >= 0 on a validated hierarchy, so integer order is float
-// order; the maximum does not depend on the order of the updates).
+// order; the maximum does not depend on the order of the updates).  A
+// grid-stride loop over a bounded grid, then one atomic per block: one
+// atomic per wave put half a million updates on one address (6 ms at
+// 512 x 256 x 256).
+constexpr int kRatioBlocks = 2048;
+
 __global__ void k_row_ratio_max(int nx, int ny, int nz, const double* D,
                                 const double* fx, const double* fy,
                                 const double* fz,
                                 unsigned long long* out) {
-  const int64_t c =
-      xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  __shared__ double scratch[16];
   const int64_t cells = static_cast<int64_t>(nx) * ny * nz;
-  double ratio = 0.0;
-  if (c < cells) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  double m = 0.0;
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       c < cells; c += stride) {
     const int i = static_cast<int>(c % nx);
     const int j = static_cast<int>((c / nx) % ny);
     const int k = static_cast<int>(c / (static_cast<int64_t>(nx) * ny));
@@ -152,14 +158,13 @@ __global__ void k_row_ratio_max(int nx, int ny, int nz, const double* D,
     const int64_t zf = c;
     const double sum = fx[xf] + fx[xf + 1] + fy[yf] + fy[yf + nx] + fz[zf] +
                        fz[zf + static_cast<int64_t>(nx) * ny];
-    ratio = sum / D[c];
+    double ratio = sum / D[c];
     if (!(ratio >= 0.0)) ratio = 0.0;  // max(0, NaN) keeps 0 (std::max)
+    m = fmax(m, ratio);
   }
-  // wave maximum, then one atomic per wave
-  for (int off = kWave / 2; off > 0; off >>= 1)
-    ratio = fmax(ratio, __shfl_down(ratio, off, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0)
-    atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(ratio)));
+  m = block_max(m, scratch);
+  if (threadIdx.x == 0)
+    atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
 }
 
 // ---------------------------------------------------------------------------

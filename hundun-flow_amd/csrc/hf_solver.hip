@@ -2217,9 +2217,10 @@ hf_status hf_plan_set_coefficients(hf_plan* p, const double* diag,
               p->d_flag);
     HF_TRY(hip_check(hipMemsetAsync(p->d_ratio, 0, sizeof(unsigned long long),
                                     p->stream)));
-    HF_LAUNCH(k_row_ratio_max, blocks_for(L0.cells), kBlock, 0, p->stream,
-              L0.n[0], L0.n[1], L0.n[2], K0.D64, K0.fx64, K0.fy64, K0.fz64,
-              p->d_ratio);
+    HF_LAUNCH(k_row_ratio_max,
+              std::min<int64_t>(blocks_for(L0.cells), kRatioBlocks), kBlock, 0,
+              p->stream, L0.n[0], L0.n[1], L0.n[2], K0.D64, K0.fx64, K0.fy64,
+              K0.fz64, p->d_ratio);
     int flag0 = 0;
     unsigned long long bits = 0;
     HF_TRY(hip_check(hipMemcpyAsync(&flag0, p->d_flag, sizeof(int),
