@@ -120,6 +120,7 @@ struct Level : Coeffs {
   Tiles tr{};
   Tiles tred{};  // kRedPer tiles per block: reduction-only kernels
   Tiles tz{};    // 2-D tiles x z-chunks of t.per planes: z-marching stencils
+  Tiles tzp{};   // tz over pairs of cells of a row (k_prolong_z2)
 };
 
 constexpr int kFoldSpan = 2048;   // partials per block of the first fold
@@ -1051,9 +1052,14 @@ void exec_grid_op(hf_plan* p, const CycleOp& op,
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       HF_PROF(p, P_PROLONG);
-      HF_LAUNCH((k_prolong_z<Tc, Tf>), L.tz.blocks, kBlock, 0, p->stream,
-                L.tz, view<Tc>(Cn), view<Tf>(L), L.coarsen_mask,
-                p->periodic_mask);
+      if (L.n[0] % 2 == 0)
+        HF_LAUNCH((k_prolong_z2<Tc, Tf>), L.tzp.blocks, kBlock, 0, p->stream,
+                  L.tzp, view<Tc>(Cn), view<Tf>(L), L.coarsen_mask,
+                  p->periodic_mask);
+      else
+        HF_LAUNCH((k_prolong_z<Tc, Tf>), L.tz.blocks, kBlock, 0, p->stream,
+                  L.tz, view<Tc>(Cn), view<Tf>(L), L.coarsen_mask,
+                  p->periodic_mask);
     });
   });
 }
@@ -1899,6 +1905,11 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
       int per = static_cast<int>(std::max<int64_t>(2, std::min<int64_t>(64, want)));
       L.tz.per = per + (per & 1);
       L.tz.blocks = gxy * ((L.n[2] + L.tz.per - 1) / L.tz.per);
+      // pairs of cells of a row (even x extents), the same z-chunks
+      L.tzp = make_tiles(std::max(1, L.n[0] / 2), L.n[1], L.n[2], false);
+      L.tzp.per = L.tz.per;
+      L.tzp.blocks = L.tzp.gx * L.tzp.gy *
+                     ((L.n[2] + L.tzp.per - 1) / L.tzp.per);
     }
     // fused sweeps on grid levels with the fused colour route and rows long
     // enough for the tiles to stay mostly busy
