@@ -82,6 +82,123 @@ __global__ void k_cheb_stage(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
   }
 }
 
+// k_cheb_stage over pairs of cells (2m, 2m + 1) of a row, for an even x
+// extent: x, D, b, r, the y and z faces and both stores move two cells per
+// access (8-byte words on FP32 levels); the x faces of a pair are three
+// loads.  Each cell's operands and order of operations are those of
+// k_cheb_stage, the pair's inner x neighbours being each other's x.  t tiles
+// (nx / 2) x ny columns in z-chunks.
+template <class T>
+__global__ void k_cheb_stage2(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
+                              typename Compute<T>::type rf,
+                              typename Compute<T>::type pf, int first) {
+  using C = typename Compute<T>::type;
+  struct alignas(2 * sizeof(T)) Pair {
+    T v[2];
+  };
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int ip, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, ip, j, k_unused);
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  if (ip >= t.nx || j >= ny) return;
+  const int i0 = 2 * ip;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  const int plane = nx * ny;
+  auto pair = [](const T* a, int c, C* out) {
+    const Pair q = *reinterpret_cast<const Pair*>(a + c);
+    out[0] = C(ld(q.v[0]));
+    out[1] = C(ld(q.v[1]));
+  };
+  int c = cell_index(nx, ny, i0, j, k0);
+  C xc[2], zl[2], fzl[2];
+  pair(x, c, xc);
+  if (k0 > 0) {
+    pair(x, c - plane, zl);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      zl[e] = neighbour_or_boundary<C>(bc.m[4], xc[e], x,
+                                       c + e + (nz - 1) * plane);
+  }
+  pair(L.fz, c, fzl);
+  for (int k = k0; k < k1; ++k) {
+    C zh[2], fzh[2], yl[2], yh[2], fyl[2], fyh[2], d[2], b[2], r[2];
+    if (k + 1 < nz) {
+      pair(x, c + plane, zh);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        zh[e] = neighbour_or_boundary<C>(bc.m[5], xc[e], x,
+                                         c + e - (nz - 1) * plane);
+    }
+    pair(L.fz, c + plane, fzh);
+    if (j > 0) {
+      pair(x, c - nx, yl);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        yl[e] = neighbour_or_boundary<C>(bc.m[2], xc[e], x,
+                                         c + e + (ny - 1) * nx);
+    }
+    if (j + 1 < ny) {
+      pair(x, c + nx, yh);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        yh[e] = neighbour_or_boundary<C>(bc.m[3], xc[e], x,
+                                         c + e - (ny - 1) * nx);
+    }
+    const int row = j + ny * k;
+    const int fyo = c + nx * k;
+    pair(L.fy, fyo, fyl);
+    pair(L.fy, fyo + nx, fyh);
+    pair(L.D, c, d);
+    pair(L.b, c, b);
+    if (!first) pair(L.r, c, r);
+    const C fx0 = C(ld(L.fx[c + row]));
+    const C fx1 = C(ld(L.fx[c + row + 1]));
+    const C fx2 = C(ld(L.fx[c + row + 2]));
+    const C xl0 = i0 > 0 ? C(ld(x[c - 1]))
+                         : neighbour_or_boundary<C>(bc.m[0], xc[0], x,
+                                                    c + (nx - 1));
+    const C xh1 = i0 + 2 < nx ? C(ld(x[c + 2]))
+                              : neighbour_or_boundary<C>(bc.m[1], xc[1], x,
+                                                         c + 1 - (nx - 1));
+    const C xl[2] = {xl0, xc[0]};
+    const C xh[2] = {xc[1], xh1};
+    const C fxl[2] = {fx0, fx1};
+    const C fxh[2] = {fx1, fx2};
+    Pair ro, xo;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      C v = d[e] * xc[e];
+      v -= fxl[e] * xl[e];
+      v -= fxh[e] * xh[e];
+      v -= fyl[e] * yl[e];
+      v -= fyh[e] * yh[e];
+      v -= fzl[e] * zl[e];
+      v -= fzh[e] * zh[e];
+      const C defect = b[e] - v;
+      const C scaled = rf * defect / d[e];
+      const C dir = first ? scaled : pf * r[e] + scaled;
+      ro.v[e] = st<T>(dir);
+      xo.v[e] = st<T>(xc[e] + dir);
+    }
+    *reinterpret_cast<Pair*>(L.r + c) = ro;
+    *reinterpret_cast<Pair*>(xn + c) = xo;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      zl[e] = xc[e];
+      xc[e] = zh[e];
+      fzl[e] = fzh[e];
+    }
+    c += plane;
+  }
+}
+
 // First Chebyshev-Jacobi stage of a level's first visit in a cycle, where x
 // holds zero: A x = +0 on every row (non-negative faces, positive D), so the
 // stage's d = rf (b - A x) / D is rf b / D and x_next = 0 + d, the values

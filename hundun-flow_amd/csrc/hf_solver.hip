@@ -901,6 +901,12 @@ void cheb_smooth(hf_plan* p, int l, int stages, int retain, int from_zero) {
       HF_LAUNCH(k_cheb_first<T>, (groups + kBlock - 1) / kBlock, kBlock, 0,
                 p->stream, static_cast<int>(L.cells), view<T>(L),
                 static_cast<T*>(L.x2), static_cast<C>(rf));
+    } else if (L.n[0] % 2 == 0) {
+      HF_PROF(p, rb_class);
+      HF_LAUNCH(k_cheb_stage2<T>, L.tzp.blocks, kBlock, 0, p->stream, L.tzp,
+                view<T>(L), p->bc, static_cast<const T*>(L.x),
+                static_cast<T*>(L.x2), static_cast<C>(rf), static_cast<C>(pf),
+                stage == 0 ? 1 : 0);
     } else {
       HF_PROF(p, rb_class);
       HF_LAUNCH(k_cheb_stage<T>, L.tz.blocks, kBlock, 0, p->stream, L.tz,
