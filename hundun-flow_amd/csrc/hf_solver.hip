@@ -266,6 +266,8 @@ struct hf_plan {
   double* d_widths[3] = {nullptr, nullptr, nullptr};
   std::vector<Level> levels;
   int tail_first = 0;  // first level handled by the tail kernel
+  bool row_pairs = true;   // pair kernels on even rows (HF_ROW_PAIRS)
+  bool cheb_first = true;  // b-and-D-only first Chebyshev stage (HF_CHEB_FIRST)
   std::vector<CycleOp> program;
   std::vector<ExecItem> exec;
   std::vector<TailOp> tail_ops_host;
@@ -528,6 +530,21 @@ int sweep_zc_override() {
 // keeps the two launches.
 bool defect_restrict_enabled() {
   const char* env = std::getenv("HF_DEFECT_RESTRICT");
+  return env == nullptr || std::atoi(env) != 0;
+}
+
+// HF_ROW_PAIRS=0: the per-cell prolongation and Chebyshev stage on every
+// level (the pair kernels are bit for bit the same; the switch lets the
+// parity tests compare the two).
+bool row_pairs_enabled() {
+  const char* env = std::getenv("HF_ROW_PAIRS");
+  return env == nullptr || std::atoi(env) != 0;
+}
+
+// HF_CHEB_FIRST=0: a from-zero Chebyshev visit starts with the general
+// stage over an explicit zero x (stored by the entry and the restriction).
+bool cheb_first_enabled() {
+  const char* env = std::getenv("HF_CHEB_FIRST");
   return env == nullptr || std::atoi(env) != 0;
 }
 
@@ -894,14 +911,14 @@ void cheb_smooth(hf_plan* p, int l, int stages, int retain, int from_zero) {
   for (int stage = 0; stage < stages; ++stage) {
     double rf = 0.0, pf = 0.0;
     stage_factors.next(&rf, &pf);
-    if (stage == 0 && from_zero) {
+    if (stage == 0 && from_zero && p->cheb_first) {
       // x = 0 (stored by the entry or the restriction): b and D suffice
       HF_PROF(p, rb_class);
       const int64_t groups = (L.cells + 3) / 4;
       HF_LAUNCH(k_cheb_first<T>, (groups + kBlock - 1) / kBlock, kBlock, 0,
                 p->stream, static_cast<int>(L.cells), view<T>(L),
                 static_cast<T*>(L.x2), static_cast<C>(rf));
-    } else if (L.n[0] % 2 == 0) {
+    } else if (p->row_pairs && L.n[0] % 2 == 0) {
       HF_PROF(p, rb_class);
       HF_LAUNCH(k_cheb_stage2<T>, L.tzp.blocks, kBlock, 0, p->stream, L.tzp,
                 view<T>(L), p->bc, static_cast<const T*>(L.x),
@@ -1058,7 +1075,7 @@ void exec_grid_op(hf_plan* p, const CycleOp& op,
     dispatch(L.prec, [&](auto tf) {
       using Tf = decltype(tf);
       HF_PROF(p, P_PROLONG);
-      if (L.n[0] % 2 == 0)
+      if (p->row_pairs && L.n[0] % 2 == 0)
         HF_LAUNCH((k_prolong_z2<Tc, Tf>), L.tzp.blocks, kBlock, 0, p->stream,
                   L.tzp, view<Tc>(Cn), view<Tf>(L), L.coarsen_mask,
                   p->periodic_mask);
@@ -1124,7 +1141,8 @@ hf_status mg_apply(hf_plan* p, const double* r, double* z, const double* q,
   const bool tail0 = L0.tail || L0.line_mask != 0 || L0.cheb;
   // (a grid Chebyshev level 0 starts with a from-zero stage, which reads
   // b and D only: its x needs no zero)
-  const bool zero_x = L0.tail || L0.line_mask != 0;
+  const bool zero_x =
+      L0.tail || L0.line_mask != 0 || (L0.cheb && !p->cheb_first);
   dispatch(L0.prec, [&](auto t) {
     using T = decltype(t);
     T* b = static_cast<T*>(L0.b);
@@ -1857,6 +1875,8 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   // the tail is the longest run of coarsest levels whose x, b, r fit the
   // tail kernel's LDS and each stay under the cell threshold
   const int64_t threshold = tail_threshold();
+  p->row_pairs = row_pairs_enabled();
+  p->cheb_first = cheb_first_enabled();
   p->tail_first = static_cast<int>(p->levels.size());
   int64_t lds_doubles = 0;
   // line levels (every level of a line plan) are smoothed by the line
@@ -2365,7 +2385,8 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
   // level (the tail and line-relaxed entries also zero x; a Chebyshev level 0
   // reads no x in its first stage, and mg_apply projects its b)
   const bool fuse_entry = !p->levels[0].tail &&
-                          p->levels[0].line_mask == 0;
+                          p->levels[0].line_mask == 0 &&
+                          (!p->levels[0].cheb || p->cheb_first);
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
   double* x = p->kx;

@@ -652,3 +652,45 @@ def test_defect_restrict_fused_matches_two_kernels(torch_cuda, monkeypatch,
         rel = np.linalg.norm(out["1"] - out["0"]) / np.linalg.norm(out["0"])
         bound = 1e-12 if precision == "fp64" else 1e-4
         assert rel <= bound, rel
+
+
+# Even and odd row lengths on every level (72 -> 36 -> 18 -> 9; 20 -> 10 ->
+# 5), the periodic wrap along x and Dirichlet/Neumann walls.
+ROW_PAIRS = {
+    "mixed_wide": CASES["mixed_wide"],
+    "odd_periodic": CASES["odd_periodic"],
+    "periodic_xy": CASES["periodic_xy"],
+}
+
+
+@pytest.mark.parametrize("name", list(ROW_PAIRS))
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("null", [0, 1])
+@pytest.mark.parametrize("smoother", ["red_black", "chebyshev"])
+def test_row_pair_kernels_match_per_cell(torch_cuda, monkeypatch, name,
+                                         precision, null, smoother):
+    """The prolongation and Chebyshev stages over pairs of cells of a row,
+    and the first Chebyshev stage of a from-zero visit formed from b and D
+    alone, store what the per-cell kernels over an explicit zero x store,
+    bit for bit, in every storage precision (the projection sums come from
+    the same kernels on both routes)."""
+    torch = torch_cuda
+    _say(f"[row pairs] case={name} precision={precision} null={null} "
+         f"smoother={smoother}")
+    case = ROW_PAIRS[name]()
+    pol = _policy(name, null_space_constant=null, precision=precision)
+    if smoother == "chebyshev":
+        pol.update(point_smoother=1, f_cycle=1, pre_sweeps=1, post_sweeps=2)
+    r = _t(torch, np.random.default_rng(8).standard_normal(
+        int(np.prod(case["n"]))))
+    out = {}
+    for on in ("1", "0"):
+        monkeypatch.setenv("HF_ROW_PAIRS", on)
+        monkeypatch.setenv("HF_CHEB_FIRST", on)
+        # grid levels down to the coarsest
+        monkeypatch.setenv("HF_TAIL_CELLS", "0")
+        plan = _gpu_plan(torch, case, **pol)
+        out[on] = plan.mg_apply(r).cpu().numpy()
+        plan.close()
+    assert np.array_equal(out["1"], out["0"]), \
+        np.max(np.abs(out["1"] - out["0"]))
