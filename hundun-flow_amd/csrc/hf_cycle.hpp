@@ -999,6 +999,56 @@ __device__ __forceinline__ void tail_apply(const Lv<double>& L, const Bc& bc,
   __syncthreads();
 }
 
+// Where a tail cell's six neighbours (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
+// sit in its level's x: the neighbour cell, the cell itself on a Neumann
+// face, the wrapped cell on a periodic face, -1 on a Dirichlet face (read
+// as +0).  The values are those neighbour_or_boundary gives; set up once
+// per smoothing call, they spare every pass the boundary-mode branches.
+struct TailNb {
+  int n[6];
+};
+
+__device__ __forceinline__ TailNb tail_neighbours(const Lv<double>& L,
+                                                  const Bc& bc, int i, int j,
+                                                  int k, int c) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz, plane = nx * ny;
+  auto face = [&](bool inside, int next, int mode, int wrapped) {
+    if (inside) return next;
+    return mode == 0 ? -1 : (mode == 1 ? c : wrapped);
+  };
+  TailNb nb;
+  nb.n[0] = face(i > 0, c - 1, bc.m[0], c + (nx - 1));
+  nb.n[1] = face(i + 1 < nx, c + 1, bc.m[1], c - (nx - 1));
+  nb.n[2] = face(j > 0, c - nx, bc.m[2], c + (ny - 1) * nx);
+  nb.n[3] = face(j + 1 < ny, c + nx, bc.m[3], c - (ny - 1) * nx);
+  nb.n[4] = face(k > 0, c - plane, bc.m[4], c + (nz - 1) * plane);
+  nb.n[5] = face(k + 1 < nz, c + plane, bc.m[5], c - (nz - 1) * plane);
+  return nb;
+}
+
+// stencil_with on a tail cell through its neighbour positions: the same
+// operands in the same order.
+__device__ __forceinline__ double tail_stencil(const Lv<double>& L,
+                                               const TailNb& nb, int c,
+                                               double xc,
+                                               const StencilCoef<double>& f) {
+  double v6[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int at = nb.n[q];
+    const double v = L.x[at < 0 ? c : at];
+    v6[q] = at < 0 ? 0.0 : v;
+  }
+  double v = f.d * xc;
+  v -= f.xl * v6[0];
+  v -= f.xh * v6[1];
+  v -= f.yl * v6[2];
+  v -= f.yh * v6[3];
+  v -= f.zl * v6[4];
+  v -= f.zh * v6[5];
+  return v;
+}
+
 // tail_smooth on a level of at most one cell per thread: the cell's
 // coefficients are loaded once and kept in registers across the passes (the
 // same values and arithmetic as the general form below).
@@ -1009,13 +1059,15 @@ __device__ void tail_smooth_small(const Lv<double>& L, int fused, const Bc& bc,
   const bool mine = c < L.cells;
   int i = 0, j = 0, k = 0;
   StencilCoef<double> f{};
+  TailNb nb{};
   if (mine) {
     tail_ijk(L, c, i, j, k);
     f = stencil_coef<double>(L, j, k, c);
+    nb = tail_neighbours(L, bc, i, j, k, c);
   }
   auto apply = [&](int defect) {
     if (mine) {
-      const double v = stencil_with<double>(L, bc, L.x, i, j, k, c, L.x[c], f);
+      const double v = tail_stencil(L, nb, c, L.x[c], f);
       L.r[c] = defect ? L.b[c] - v : v;
     }
     __syncthreads();
@@ -1028,7 +1080,7 @@ __device__ void tail_smooth_small(const Lv<double>& L, int fused, const Bc& bc,
       if (mine && ((i + j + k) & 1) == color) {
         const double xc = L.x[c];
         const double v =
-            fused ? stencil_with<double>(L, bc, L.x, i, j, k, c, xc, f) : L.r[c];
+            fused ? tail_stencil(L, nb, c, xc, f) : L.r[c];
         L.x[c] = xc + omega * (L.b[c] - v) / f.d;
       }
       __syncthreads();
@@ -1084,15 +1136,17 @@ __device__ void tail_cheb(const Lv<double>& L, const Bc& bc, int stages,
     const bool mine = c < L.cells;
     int i = 0, j = 0, k = 0;
     StencilCoef<double> f{};
+    TailNb nb{};
     if (mine) {
       tail_ijk(L, c, i, j, k);
       f = stencil_coef<double>(L, j, k, c);
+      nb = tail_neighbours(L, bc, i, j, k, c);
     }
     for (int s = 0; s < stages; ++s) {
       double xn = 0.0;
       if (mine) {
         const double xc = L.x[c];
-        const double v = stencil_with<double>(L, bc, L.x, i, j, k, c, xc, f);
+        const double v = tail_stencil(L, nb, c, xc, f);
         const double defect = L.b[c] - v;
         const double scaled = ch.rf[s] * defect / f.d;
         const double dir = s == 0 ? scaled : ch.pf[s] * L.r[c] + scaled;
@@ -1105,7 +1159,7 @@ __device__ void tail_cheb(const Lv<double>& L, const Bc& bc, int stages,
     }
     if (retain) {
       if (mine)
-        L.r[c] = L.b[c] - stencil_with<double>(L, bc, L.x, i, j, k, c, L.x[c], f);
+        L.r[c] = L.b[c] - tail_stencil(L, nb, c, L.x[c], f);
       __syncthreads();
     }
     return;
