@@ -457,6 +457,30 @@ __global__ void k_fg_combine(Chunk ch, double* dst, const double* src,
   if (bad) *flag = 1.0;
 }
 
+// k_fg_combine with the normalisation (scale), also storing the
+// multigrid's level-0 entry of the result, b = fluid ? v : 0 at level
+// storage (k_mg_entry's values, without projection sums): the
+// preconditioner applied to dst next then starts from b without reading dst
+// again.
+template <class T>
+__global__ void k_fg_combine_entry(Chunk ch, double* dst, const double* src,
+                                   const double* basis, int64_t vstride,
+                                   FgScales s, int count, double post,
+                                   double* flag, const uint8_t* act, T* b) {
+  int lb, lo, hi;
+  chunk_range(ch, lb, lo, hi);
+  bool bad = false;
+  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
+    double v = src[c];
+    for (int row = 0; row < count; ++row) v = v + s.s[row] * basis[row * vstride + c];
+    v = post * v;
+    bad = bad || !isfinite(v);
+    dst[c] = v;
+    b[c] = act[c] != 0 ? st<T>(v) : st<T>(0.0);
+  }
+  if (bad) *flag = 1.0;
+}
+
 // dst = src * (w_x w_y w_z): the residual handed to the multigrid by the
 // volume-scaled preconditioner (src/solver_cold.hpp:378-415).
 __global__ void k_scale_volume(Tiles t, double* dst, const double* src,

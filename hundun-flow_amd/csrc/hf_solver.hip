@@ -1522,14 +1522,34 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
     return ok();
   };
   // dst = src + sum s[row] basis_row (scale: then times post, checked)
+  // A normalised basis vector is the next preconditioner input; its
+  // combine also stores the multigrid's level-0 entry when that entry is a
+  // plain copy: a grid level 0 whose first smoothing reads no x (point
+  // levels, Chebyshev with the b-and-D-only first stage), no constant
+  // projection, no volume-scaled rows.
+  const bool entry_in_combine =
+      !vrows && p->policy.null_space_constant == 0 && !L0.tail &&
+      L0.line_mask == 0 && (!L0.cheb || p->cheb_first);
+  bool entered_next = false;
   auto combine = [&](double* dst, const double* src, const double* basis,
                      const double* s, size_t count, double post, int scale) {
     FgScales k{};
     for (size_t r = 0; r < count; ++r) k.s[r] = s[r];
     HF_PROF(p, P_OTHER);
-    HF_LAUNCH(k_fg_combine, nbv, kBlock, 0, p->stream, ch, dst, src, basis, N,
-              k, static_cast<int>(count), post, scale,
-              p->fg_scalars + kFgFlag);
+    if (scale && entry_in_combine) {
+      dispatch(L0.prec, [&](auto t) {
+        using T = decltype(t);
+        HF_LAUNCH(k_fg_combine_entry<T>, nbv, kBlock, 0, p->stream, ch, dst,
+                  src, basis, N, k, static_cast<int>(count), post,
+                  p->fg_scalars + kFgFlag,
+                  static_cast<const uint8_t*>(L0.act), static_cast<T*>(L0.b));
+      });
+      entered_next = true;
+    } else {
+      HF_LAUNCH(k_fg_combine, nbv, kBlock, 0, p->stream, ch, dst, src, basis,
+                N, k, static_cast<int>(count), post, scale,
+                p->fg_scalars + kFgFlag);
+    }
     if (scale) {
       flag_pending = true;
       ops_at = res->operator_applies;
@@ -1612,7 +1632,8 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
                   scaled, static_cast<const double*>(V(col)), wd);
         src = scaled;
       }
-      HF_TRY(mg_apply(p, src, Z(col), nullptr, false));
+      HF_TRY(mg_apply(p, src, Z(col), nullptr, false, entered_next));
+      entered_next = false;
       res->preconditioner_applies++;
       if (L0.line_mask != 0) {
         const hf_status ls = read_scalars(p);
