@@ -188,3 +188,44 @@ def test_production_pairing_matches_oracle(torch_cuda, cfl, shortcut):
     assert r32.termination == "converged" and rel <= 1.01e-10
     plan.close()
     plan32.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_production_solve_same_on_per_cell_route(torch_cuda, monkeypatch,
+                                                 precision):
+    """The production solve (FGMRES(12), Chebyshev F(1,2)) through the pair
+    kernels, the b-and-D-only first stage and the combine that stores the
+    multigrid entry gives the iterates of the per-cell route over a stored
+    zero x with the separate entry: the same iteration count and the same
+    solution bit for bit."""
+    torch = torch_cuda
+    import hundun_flow as H
+    case = hf_cases.compressible_flame(n=(40, 24, 20), seed=76,
+                                       acoustic_cfl=4.0)
+    masks = [_t(torch, case[k]) for k in ("cell_active", "x_active",
+                                          "y_active", "z_active")]
+    out = {}
+    for on in ("1", "0"):
+        monkeypatch.setenv("HF_ROW_PAIRS", on)
+        monkeypatch.setenv("HF_CHEB_FIRST", on)
+        plan = H.Plan(case["n"], case["faces"], case["bc"], H.default_policy(
+            pre_sweeps=1, post_sweeps=2, point_smoother=1, f_cycle=1,
+            diagonal_shortcut_ratio=0.0, null_space_constant=0,
+            precision=precision))  # (ratio 0: the cycle always runs)
+        _, bx, by, bz = plan.assemble(_t(torch, case["mobility"]))
+        d, b = plan.assemble_pressure_system(
+            bx, by, bz, [_t(torch, p) for p in case["phi"]],
+            _t(torch, case["rho"]), _t(torch, case["rho_n"]),
+            _t(torch, case["drho_dp"]), case["bdf"],
+            rho_nm1=_t(torch, case["rho_nm1"]),
+            source=_t(torch, case["source"]))
+        b = b * masks[0].to(b.dtype)
+        plan.set_coefficients(d, bx, by, bz, *masks)
+        x, res = plan.solve_fgmres(b, rtol=1e-10, restart=12)
+        out[on] = (x.cpu().numpy(), res.iterations, res.termination)
+        plan.close()
+    _say(f"[production routes] {precision} it={out['1'][1]}/{out['0'][1]}")
+    assert out["1"][2] == "converged"
+    assert out["1"][1] == out["0"][1]
+    assert np.array_equal(out["1"][0], out["0"][0]), \
+        np.max(np.abs(out["1"][0] - out["0"][0]))
