@@ -534,9 +534,18 @@ __device__ __forceinline__ void prolong_cell(const Lv<Tc>& Cl, const Lv<Tf>& F,
 // coarse cell (even i, even j), which adds its eight children in the order
 // of k_restrict (dz, dy, dx) and stores the coarse b at the end of the odd
 // plane.  The fine r is not stored.  zero_x and partials as in k_restrict.
-template <class Tf, class Tc>
+//
+// kFirstStage: the level was just smoothed by one Chebyshev-Jacobi stage
+// from x = 0 (the pre-smoothing of a from-zero visit), which is not run:
+// every x this kernel reads is formed from that cell's b and D as the
+// stage stores it, st(0 + rf0 b / D), and the kernel also stores x of its
+// own cells into xs (the stage's direction is dead: the next stage on the
+// level starts a new recurrence).
+template <class Tf, class Tc, bool kFirstStage = false>
 __global__ void k_defect_restrict(Tiles t, Lv<Tf> L, Bc bc, Lv<Tc> Cl,
-                                  int zero_x, double* partials) {
+                                  int zero_x, double* partials,
+                                  typename Compute<Tf>::type rf0 = 0,
+                                  Tf* xs = nullptr) {
   using C = typename Compute<Tf>::type;
   using Cc = typename Compute<Tc>::type;
   __shared__ double scratch[16];
@@ -554,30 +563,36 @@ __global__ void k_defect_restrict(Tiles t, Lv<Tf> L, Bc bc, Lv<Tc> Cl,
   const int tid = threadIdx.x;
   const bool owner = ((i | j) & 1) == 0;
   const Tf* x = L.x;
+  // x at cell q: stored, or formed from b and D as the first stage stores it
+  auto xat = [&](int q) -> C {
+    if constexpr (kFirstStage)
+      return C(ld(st<Tf>(C(0) + rf0 * C(ld(L.b[q])) / C(ld(L.D[q])))));
+    else
+      return C(ld(x[q]));
+  };
+  // neighbour_or_boundary through xat
+  auto nob = [&](int mode, C self, int wrapped) -> C {
+    if (mode == 0) return C(0);
+    if (mode == 1) return self;
+    return xat(wrapped);
+  };
   int c = cell_index(nx, ny, i, j, k0);
-  C xc = C(ld(x[c]));
-  C zl = k0 > 0 ? C(ld(x[c - plane]))
-                : neighbour_or_boundary<C>(bc.m[4], xc, x, c + (nz - 1) * plane);
+  C xc = xat(c);
+  C zl = k0 > 0 ? xat(c - plane) : nob(bc.m[4], xc, c + (nz - 1) * plane);
   C fzl = C(ld(L.fz[c]));
   Cc sum = Cc(0);
   double pv = 0.0;
   for (int k = k0; k < k1; ++k) {
-    const C zh = k + 1 < nz ? C(ld(x[c + plane]))
-                            : neighbour_or_boundary<C>(bc.m[5], xc, x,
-                                                       c - (nz - 1) * plane);
+    const C zh = k + 1 < nz ? xat(c + plane)
+                            : nob(bc.m[5], xc, c - (nz - 1) * plane);
     const C fzh = C(ld(L.fz[c + plane]));
     const int row = j + ny * k;
-    const C xl = i > 0 ? C(ld(x[c - 1]))
-                       : neighbour_or_boundary<C>(bc.m[0], xc, x, c + (nx - 1));
-    const C xh = i + 1 < nx ? C(ld(x[c + 1]))
-                            : neighbour_or_boundary<C>(bc.m[1], xc, x,
-                                                       c - (nx - 1));
-    const C yl = j > 0 ? C(ld(x[c - nx]))
-                       : neighbour_or_boundary<C>(bc.m[2], xc, x,
-                                                  c + (ny - 1) * nx);
-    const C yh = j + 1 < ny ? C(ld(x[c + nx]))
-                            : neighbour_or_boundary<C>(bc.m[3], xc, x,
-                                                       c - (ny - 1) * nx);
+    const C xl = i > 0 ? xat(c - 1) : nob(bc.m[0], xc, c + (nx - 1));
+    const C xh = i + 1 < nx ? xat(c + 1) : nob(bc.m[1], xc, c - (nx - 1));
+    const C yl = j > 0 ? xat(c - nx) : nob(bc.m[2], xc, c + (ny - 1) * nx);
+    const C yh = j + 1 < ny ? xat(c + nx)
+                            : nob(bc.m[3], xc, c - (ny - 1) * nx);
+    if constexpr (kFirstStage) xs[c] = st<Tf>(xc);
     const int fyo = c + nx * k;
     C v = C(ld(L.D[c])) * xc;
     v -= C(ld(L.fx[c + row])) * xl;
