@@ -18,7 +18,9 @@ __device__ __forceinline__ double geom_volume(const Widths& w, int i, int j,
 // ---------------------------------------------------------------------------
 // Chebyshev-Jacobi smoothing (src/solver_mg.cpp:2916-3090).  One launch is
 // one stage: every neighbour comes from the stage's input x (Jacobi), the
-// direction lives in the level's r slot, and
+// direction lives in the level's r slot (read from L.r, stored to dout; a
+// smoothing's last stage passes dout == nullptr, as no stage reads its
+// direction), and
 //   d = rf (b - A x) / D            (first stage)
 //   d = pf d + rf (b - A x) / D     (later stages)
 //   x_next = x + d.
@@ -27,7 +29,8 @@ __device__ __forceinline__ double geom_volume(const Widths& w, int i, int j,
 template <class T>
 __global__ void k_cheb_stage(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
                              typename Compute<T>::type rf,
-                             typename Compute<T>::type pf, int first) {
+                             typename Compute<T>::type pf, int first,
+                             T* dout) {
   using C = typename Compute<T>::type;
   const int lb = xcd_block32(blockIdx.x, gridDim.x);
   const int gxy = t.gx * t.gy;
@@ -73,7 +76,7 @@ __global__ void k_cheb_stage(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
     const C defect = C(ld(L.b[c])) - v;
     const C scaled = rf * defect / d;
     const C dir = first ? scaled : pf * C(ld(L.r[c])) + scaled;
-    L.r[c] = st<T>(dir);
+    if (dout != nullptr) dout[c] = st<T>(dir);
     xn[c] = st<T>(xc + dir);
     zl = xc;
     xc = zh;
@@ -91,7 +94,8 @@ __global__ void k_cheb_stage(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
 template <class T>
 __global__ void k_cheb_stage2(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
                               typename Compute<T>::type rf,
-                              typename Compute<T>::type pf, int first) {
+                              typename Compute<T>::type pf, int first,
+                              T* dout) {
   using C = typename Compute<T>::type;
   struct alignas(2 * sizeof(T)) Pair {
     T v[2];
@@ -187,7 +191,7 @@ __global__ void k_cheb_stage2(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
       ro.v[e] = st<T>(dir);
       xo.v[e] = st<T>(xc[e] + dir);
     }
-    *reinterpret_cast<Pair*>(L.r + c) = ro;
+    if (dout != nullptr) *reinterpret_cast<Pair*>(dout + c) = ro;
     *reinterpret_cast<Pair*>(xn + c) = xo;
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
@@ -203,10 +207,11 @@ __global__ void k_cheb_stage2(Tiles t, Lv<T> L, Bc bc, const T* x, T* xn,
 // holds zero: A x = +0 on every row (non-negative faces, positive D), so the
 // stage's d = rf (b - A x) / D is rf b / D and x_next = 0 + d, the values
 // k_cheb_stage stores there, from b and D alone.  Four consecutive cells per
-// thread.
+// thread.  dir == nullptr: the direction is not stored (a one-stage
+// smoothing, whose direction no later stage reads).
 template <class T>
 __global__ void k_cheb_first(int cells, Lv<T> L, T* xn,
-                             typename Compute<T>::type rf) {
+                             typename Compute<T>::type rf, T* dir) {
   using C = typename Compute<T>::type;
   struct alignas(4 * sizeof(T)) Quad {
     T v[4];
@@ -220,18 +225,18 @@ __global__ void k_cheb_first(int cells, Lv<T> L, T* xn,
     Quad r, x;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const C dir = rf * C(ld(b.v[e])) / C(ld(d.v[e]));
-      r.v[e] = st<T>(dir);
-      x.v[e] = st<T>(C(0) + dir);
+      const C q = rf * C(ld(b.v[e])) / C(ld(d.v[e]));
+      r.v[e] = st<T>(q);
+      x.v[e] = st<T>(C(0) + q);
     }
-    *reinterpret_cast<Quad*>(L.r + c0) = r;
+    if (dir != nullptr) *reinterpret_cast<Quad*>(dir + c0) = r;
     *reinterpret_cast<Quad*>(xn + c0) = x;
     return;
   }
   for (int64_t c = c0; c < cells; ++c) {
-    const C dir = rf * C(ld(L.b[c])) / C(ld(L.D[c]));
-    L.r[c] = st<T>(dir);
-    xn[c] = st<T>(C(0) + dir);
+    const C d = rf * C(ld(L.b[c])) / C(ld(L.D[c]));
+    if (dir != nullptr) dir[c] = st<T>(d);
+    xn[c] = st<T>(C(0) + d);
   }
 }
 

@@ -911,25 +911,27 @@ void cheb_smooth(hf_plan* p, int l, int stages, int retain, int from_zero) {
   for (int stage = 0; stage < stages; ++stage) {
     double rf = 0.0, pf = 0.0;
     stage_factors.next(&rf, &pf);
+    // the last stage's direction is read by no later stage
+    T* dout = stage + 1 < stages ? static_cast<T*>(L.r) : nullptr;
     if (stage == 0 && from_zero && p->cheb_first) {
       // x = 0 (stored by the entry or the restriction): b and D suffice
       HF_PROF(p, rb_class);
       const int64_t groups = (L.cells + 3) / 4;
       HF_LAUNCH(k_cheb_first<T>, (groups + kBlock - 1) / kBlock, kBlock, 0,
                 p->stream, static_cast<int>(L.cells), view<T>(L),
-                static_cast<T*>(L.x2), static_cast<C>(rf));
+                static_cast<T*>(L.x2), static_cast<C>(rf), dout);
     } else if (p->row_pairs && L.n[0] % 2 == 0) {
       HF_PROF(p, rb_class);
       HF_LAUNCH(k_cheb_stage2<T>, L.tzp.blocks, kBlock, 0, p->stream, L.tzp,
                 view<T>(L), p->bc, static_cast<const T*>(L.x),
                 static_cast<T*>(L.x2), static_cast<C>(rf), static_cast<C>(pf),
-                stage == 0 ? 1 : 0);
+                stage == 0 ? 1 : 0, dout);
     } else {
       HF_PROF(p, rb_class);
       HF_LAUNCH(k_cheb_stage<T>, L.tz.blocks, kBlock, 0, p->stream, L.tz,
                 view<T>(L), p->bc, static_cast<const T*>(L.x),
                 static_cast<T*>(L.x2), static_cast<C>(rf), static_cast<C>(pf),
-                stage == 0 ? 1 : 0);
+                stage == 0 ? 1 : 0, dout);
     }
     std::swap(L.x, L.x2);
   }
