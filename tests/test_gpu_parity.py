@@ -660,6 +660,8 @@ ROW_PAIRS = {
     "mixed_wide": CASES["mixed_wide"],
     "odd_periodic": CASES["odd_periodic"],
     "periodic_xy": CASES["periodic_xy"],
+    # periodic in every direction, levels far narrower than their tiles
+    "periodic_all": lambda: _walls((36, 10, 14), [2, 2, 2, 2, 2, 2], 74),
 }
 
 
