@@ -434,6 +434,199 @@ __global__ void k_fg_dots(Tiles t, const double* wv, const double* basis,
   }
 }
 
+// k_fg_stencil over pairs of cells (2m, 2m + 1) of a row, for an even x
+// extent: x, aux, the basis rows, D, the y and z faces and out move two
+// cells (16 bytes) per access; the x faces of a pair are three loads.  Each
+// cell's operator value is formed as in k_fg_stencil; a thread adds its two
+// cells' products to its accumulators in cell order.  t tiles (nx / 2) x ny
+// columns in z-chunks; partials as in k_fg_stencil, one per block of t.
+template <int kMode, int kNd>
+__global__ void k_fg_stencil2(Tiles t, Lv<double> A, Bc bc, const double* x,
+                              const double* aux, double* out,
+                              const double* basis, int64_t vstride,
+                              int wacc, double shift, int volume_rows,
+                              Widths w, double* partials, int64_t pstride) {
+  static_assert(kNd >= 0 && kNd <= kFgDots, "dots of one pass");
+  struct alignas(16) Pair {
+    double v[2];
+  };
+  auto pair = [](const double* a, int64_t c, double* o2) {
+    const Pair q = *reinterpret_cast<const Pair*>(a + c);
+    o2[0] = q.v[0];
+    o2[1] = q.v[1];
+  };
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int ip, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, ip, j, k_unused);
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  double acc[kNd > 0 ? kNd : 1];
+#pragma unroll
+  for (int a = 0; a < kNd; ++a) acc[a] = 0.0;
+  double ww = 0.0;
+  if (ip < t.nx && j < ny) {
+    const int i0 = 2 * ip;
+    const int plane = nx * ny;
+    int c = cell_index(nx, ny, i0, j, k0);
+    double xc[2], zl[2], fzl[2];
+    pair(x, c, xc);
+    if (k0 > 0) {
+      pair(x, c - plane, zl);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        zl[e] = neighbour_or_boundary<double>(bc.m[4], xc[e], x,
+                                              c + e + (nz - 1) * plane);
+    }
+    pair(A.fz, c, fzl);
+    double wxy[2] = {1.0, 1.0};
+    if (volume_rows) {
+      wxy[0] = w.x[i0] * w.y[j];
+      wxy[1] = w.x[i0 + 1] * w.y[j];
+    }
+    for (int k = k0; k < k1; ++k) {
+      double zh[2], fzh[2], yl[2], yh[2], fyl[2], fyh[2], d[2], av[2];
+      if (k + 1 < nz) {
+        pair(x, c + plane, zh);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+          zh[e] = neighbour_or_boundary<double>(bc.m[5], xc[e], x,
+                                                c + e - (nz - 1) * plane);
+      }
+      pair(A.fz, c + plane, fzh);
+      if (j > 0) {
+        pair(x, c - nx, yl);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+          yl[e] = neighbour_or_boundary<double>(bc.m[2], xc[e], x,
+                                                c + e + (ny - 1) * nx);
+      }
+      if (j + 1 < ny) {
+        pair(x, c + nx, yh);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+          yh[e] = neighbour_or_boundary<double>(bc.m[3], xc[e], x,
+                                                c + e - (ny - 1) * nx);
+      }
+      const int row = j + ny * k;
+      const int fyo = c + nx * k;
+      pair(A.fy, fyo, fyl);
+      pair(A.fy, fyo + nx, fyh);
+      pair(A.D, c, d);
+      pair(aux, c, av);
+      const double fx0 = A.fx[c + row];
+      const double fx1 = A.fx[c + row + 1];
+      const double fx2 = A.fx[c + row + 2];
+      const double xl0 = i0 > 0 ? x[c - 1]
+                                : neighbour_or_boundary<double>(
+                                      bc.m[0], xc[0], x, c + (nx - 1));
+      const double xh1 = i0 + 2 < nx
+                             ? x[c + 2]
+                             : neighbour_or_boundary<double>(
+                                   bc.m[1], xc[1], x, c + 1 - (nx - 1));
+      const double xl[2] = {xl0, xc[0]};
+      const double xh[2] = {xc[1], xh1};
+      const double fxl[2] = {fx0, fx1};
+      const double fxh[2] = {fx1, fx2};
+      Pair o;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        double v = d[e] * xc[e];
+        v -= fxl[e] * xl[e];
+        v -= fxh[e] * xh[e];
+        v -= fyl[e] * yl[e];
+        v -= fyh[e] * yh[e];
+        v -= fzl[e] * zl[e];
+        v -= fzh[e] * zh[e];
+        if (volume_rows) v = v / (wxy[e] * w.z[k]);
+        if constexpr (kMode == kFgArnoldi)
+          o.v[e] = v + -shift * av[e];
+        else
+          o.v[e] = 1.0 * av[e] + -1.0 * v;
+      }
+      *reinterpret_cast<Pair*>(out + c) = o;
+      if constexpr (kMode == kFgArnoldi) {
+#pragma unroll
+        for (int a = 0; a < kNd; ++a) {
+          double bv[2];
+          pair(basis + a * vstride, c, bv);
+          acc[a] += bv[0] * o.v[0];
+          acc[a] += bv[1] * o.v[1];
+        }
+      }
+      ww += o.v[0] * o.v[0];
+      ww += o.v[1] * o.v[1];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        zl[e] = xc[e];
+        xc[e] = zh[e];
+        fzl[e] = fzh[e];
+      }
+      c += plane;
+    }
+  }
+  if constexpr (kMode == kFgArnoldi) {
+#pragma unroll
+    for (int a = 0; a < kNd; ++a) {
+      const double s = block_sum(acc[a], scratch);
+      if (threadIdx.x == 0) partials[a * pstride + lb] = s;
+    }
+  }
+  const double s = block_sum(ww, scratch);
+  if (threadIdx.x == 0) partials[wacc * pstride + lb] = s;
+}
+
+// k_fg_dots over pairs of cells of a row (t tiles (nx / 2) x ny columns),
+// in the partial layout of k_fg_stencil2.
+template <int kNd>
+__global__ void k_fg_dots2(Tiles t, const double* wv, const double* basis,
+                           int64_t vstride, int row0, int acc0,
+                           double* partials, int64_t pstride) {
+  static_assert(kNd >= 1 && kNd <= kFgDots, "dots of one pass");
+  struct alignas(16) Pair {
+    double v[2];
+  };
+  __shared__ double scratch[16];
+  const int lb = xcd_block32(blockIdx.x, gridDim.x);
+  const int gxy = t.gx * t.gy;
+  const int zb = lb / gxy;
+  int ip, j, k_unused;
+  tile_xyz(t, lb - zb * gxy, ip, j, k_unused);
+  const int nx = 2 * t.nx, ny = t.ny, nz = t.nz;
+  const int k0 = zb * t.per;
+  const int k1 = min(nz, k0 + t.per);
+  double acc[kNd];
+#pragma unroll
+  for (int a = 0; a < kNd; ++a) acc[a] = 0.0;
+  if (ip < t.nx && j < ny) {
+    const int plane = nx * ny;
+    int c = cell_index(nx, ny, 2 * ip, j, k0);
+    for (int k = k0; k < k1; ++k) {
+      const Pair o = *reinterpret_cast<const Pair*>(wv + c);
+#pragma unroll
+      for (int a = 0; a < kNd; ++a) {
+        const Pair b = *reinterpret_cast<const Pair*>(
+            basis + (row0 + a) * vstride + c);
+        acc[a] += b.v[0] * o.v[0];
+        acc[a] += b.v[1] * o.v[1];
+      }
+      c += plane;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < kNd; ++a) {
+    const double s = block_sum(acc[a], scratch);
+    if (threadIdx.x == 0) partials[(acc0 + a) * pstride + lb] = s;
+  }
+}
+
 // Scales of one basis combination, passed by value.
 struct FgScales {
   double s[kFgMaxRestart + 1];

@@ -268,6 +268,7 @@ struct hf_plan {
   int tail_first = 0;  // first level handled by the tail kernel
   bool row_pairs = true;   // pair kernels on even rows (HF_ROW_PAIRS)
   bool cheb_first = true;  // b-and-D-only first Chebyshev stage (HF_CHEB_FIRST)
+  bool krylov_pairs = true;  // FGMRES level-0 kernels over row pairs (HF_KRYLOV_PAIRS)
   std::vector<CycleOp> program;
   std::vector<ExecItem> exec;
   std::vector<TailOp> tail_ops_host;
@@ -538,6 +539,15 @@ bool defect_restrict_enabled() {
 // parity tests compare the two).
 bool row_pairs_enabled() {
   const char* env = std::getenv("HF_ROW_PAIRS");
+  return env == nullptr || std::atoi(env) != 0;
+}
+
+// HF_KRYLOV_PAIRS=0: the FGMRES operator, residual and dot kernels one
+// cell per lane on every grid (the pair forms add each lane's two products
+// in cell order, so their inner products round differently; the switch
+// lets the tests compare the two).
+bool krylov_pairs_enabled() {
+  const char* env = std::getenv("HF_KRYLOV_PAIRS");
   return env == nullptr || std::atoi(env) != 0;
 }
 
@@ -1416,29 +1426,37 @@ int fg_back_substitute(const std::vector<double>& h, size_t rows,
   return 0;
 }
 
-// k_fg_stencil<kFgArnoldi, nd> and k_fg_dots<nd> for a dot count nd in
-// [1, kFgDots] known at run time.
+// k_fg_stencil<kFgArnoldi, nd> and k_fg_dots<nd> (their row-pair forms
+// when `pairs`) for a dot count nd in [1, kFgDots] known at run time.
 template <int N = 1, class... Args>
-void launch_fg_arnoldi(int nd, int64_t blocks, hipStream_t stream,
+void launch_fg_arnoldi(bool pairs, int nd, int64_t blocks, hipStream_t stream,
                        Args... args) {
   if constexpr (N <= kFgDots) {
     if (nd == N) {
-      HF_LAUNCH((k_fg_stencil<kFgArnoldi, N>), blocks, kBlock, 0, stream,
-                args...);
+      if (pairs)
+        HF_LAUNCH((k_fg_stencil2<kFgArnoldi, N>), blocks, kBlock, 0, stream,
+                  args...);
+      else
+        HF_LAUNCH((k_fg_stencil<kFgArnoldi, N>), blocks, kBlock, 0, stream,
+                  args...);
       return;
     }
-    launch_fg_arnoldi<N + 1>(nd, blocks, stream, args...);
+    launch_fg_arnoldi<N + 1>(pairs, nd, blocks, stream, args...);
   }
 }
 
 template <int N = 1, class... Args>
-void launch_fg_dots(int nd, int64_t blocks, hipStream_t stream, Args... args) {
+void launch_fg_dots(bool pairs, int nd, int64_t blocks, hipStream_t stream,
+                    Args... args) {
   if constexpr (N <= kFgDots) {
     if (nd == N) {
-      HF_LAUNCH(k_fg_dots<N>, blocks, kBlock, 0, stream, args...);
+      if (pairs)
+        HF_LAUNCH(k_fg_dots2<N>, blocks, kBlock, 0, stream, args...);
+      else
+        HF_LAUNCH(k_fg_dots<N>, blocks, kBlock, 0, stream, args...);
       return;
     }
-    launch_fg_dots<N + 1>(nd, blocks, stream, args...);
+    launch_fg_dots<N + 1>(pairs, nd, blocks, stream, args...);
   }
 }
 
@@ -1449,7 +1467,10 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
   const int64_t N = p->cells;
   const Lv<double> A = krylov_view(p);
   const Level& L0 = p->levels[0];
-  const Tiles& tz = L0.tz;
+  // the level-0 stencils and dots run over pairs of cells of a row when
+  // rows have even length (one partial per block of the layout used)
+  const bool pairs = p->krylov_pairs && L0.n[0] % 2 == 0;
+  const Tiles& tz = pairs ? L0.tzp : L0.tz;
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
   const int vrows = ctl->volume_rows ? 1 : 0;
@@ -1525,10 +1546,16 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
                            int* term) -> hf_status {
     {
       HF_PROF(p, P_TRUE_RESIDUAL);
-      HF_LAUNCH((k_fg_stencil<kFgResidual, 0>), tz.blocks, kBlock, 0,
-                p->stream, tz, A, p->bc, vec, rhs, out,
-                static_cast<const double*>(nullptr), int64_t{0}, 0, 0.0, vrows,
-                wd, p->fg_partials, pst);
+      if (pairs)
+        HF_LAUNCH((k_fg_stencil2<kFgResidual, 0>), tz.blocks, kBlock, 0,
+                  p->stream, tz, A, p->bc, vec, rhs, out,
+                  static_cast<const double*>(nullptr), int64_t{0}, 0, 0.0,
+                  vrows, wd, p->fg_partials, pst);
+      else
+        HF_LAUNCH((k_fg_stencil<kFgResidual, 0>), tz.blocks, kBlock, 0,
+                  p->stream, tz, A, p->bc, vec, rhs, out,
+                  static_cast<const double*>(nullptr), int64_t{0}, 0, 0.0,
+                  vrows, wd, p->fg_partials, pst);
     }
     bool bad = false;
     HF_TRY(reduce(1, &bad));
@@ -1587,15 +1614,19 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
                         int acc0) {
     for (size_t r = row0; r < row0 + count; r += kFgDots) {
       const int nd = static_cast<int>(std::min<size_t>(kFgDots, row0 + count - r));
-      launch_fg_dots(nd, tz.blocks, p->stream, tz, wv,
+      launch_fg_dots(pairs, nd, tz.blocks, p->stream, tz, wv,
                      static_cast<const double*>(p->fg_v), N,
                      static_cast<int>(r), acc0 + static_cast<int>(r - row0),
                      p->fg_partials, pst);
     }
   };
   auto norm_of = [&](const double* wv, double* norm, bool* bad) -> hf_status {
-    HF_LAUNCH(k_fg_dots<1>, tz.blocks, kBlock, 0, p->stream, tz, wv, wv,
-              int64_t{0}, 0, 0, p->fg_partials, pst);
+    if (pairs)
+      HF_LAUNCH(k_fg_dots2<1>, tz.blocks, kBlock, 0, p->stream, tz, wv, wv,
+                int64_t{0}, 0, 0, p->fg_partials, pst);
+    else
+      HF_LAUNCH(k_fg_dots<1>, tz.blocks, kBlock, 0, p->stream, tz, wv, wv,
+                int64_t{0}, 0, 0, p->fg_partials, pst);
     HF_TRY(reduce(1, bad));
     *norm = std::sqrt(p->fg_host[0]);
     return ok();
@@ -1671,7 +1702,8 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
       const size_t dots = col + 1;
       {
         HF_PROF(p, P_OPERATOR);
-        launch_fg_arnoldi(static_cast<int>(std::min<size_t>(dots, kFgDots)),
+        launch_fg_arnoldi(pairs,
+                          static_cast<int>(std::min<size_t>(dots, kFgDots)),
                           tz.blocks, p->stream, tz, A, p->bc,
                           static_cast<const double*>(Z(col)),
                           static_cast<const double*>(V(col)), ax,
@@ -1923,6 +1955,7 @@ hf_status hf_plan_create(const hf_grid_desc* grid, const hf_policy* policy,
   const int64_t threshold = tail_threshold();
   p->row_pairs = row_pairs_enabled();
   p->cheb_first = cheb_first_enabled();
+  p->krylov_pairs = krylov_pairs_enabled();
   p->tail_first = static_cast<int>(p->levels.size());
   int64_t lds_doubles = 0;
   // line levels (every level of a line plan) are smoothed by the line

@@ -229,3 +229,43 @@ def test_production_solve_same_on_per_cell_route(torch_cuda, monkeypatch,
     assert out["1"][1] == out["0"][1]
     assert np.array_equal(out["1"][0], out["0"][0]), \
         np.max(np.abs(out["1"][0] - out["0"][0]))
+
+
+@pytest.mark.parametrize("volume_rows", [False, True])
+def test_fgmres_row_pair_kernels(torch_cuda, monkeypatch, volume_rows):
+    """FGMRES with its level-0 operator, residual and dot kernels over pairs
+    of cells of a row against the one-cell-per-lane kernels: the pair forms
+    add each lane's two products in cell order, so the inner products round
+    differently; the same iteration count and the solution to 1e-12."""
+    torch = torch_cuda
+    import hundun_flow as H
+    case = hf_cases.compressible_flame(n=(40, 24, 20), seed=77,
+                                       acoustic_cfl=4.0)
+    masks = [_t(torch, case[k]) for k in ("cell_active", "x_active",
+                                          "y_active", "z_active")]
+    out = {}
+    for on in ("1", "0"):
+        monkeypatch.setenv("HF_KRYLOV_PAIRS", on)
+        plan = H.Plan(case["n"], case["faces"], case["bc"], H.default_policy(
+            pre_sweeps=1, post_sweeps=2, point_smoother=1, f_cycle=1,
+            diagonal_shortcut_ratio=0.0, null_space_constant=0))
+        _, bx, by, bz = plan.assemble(_t(torch, case["mobility"]))
+        d, b = plan.assemble_pressure_system(
+            bx, by, bz, [_t(torch, p) for p in case["phi"]],
+            _t(torch, case["rho"]), _t(torch, case["rho_n"]),
+            _t(torch, case["drho_dp"]), case["bdf"],
+            rho_nm1=_t(torch, case["rho_nm1"]),
+            source=_t(torch, case["source"]))
+        b = b * masks[0].to(b.dtype)
+        plan.set_coefficients(d, bx, by, bz, *masks)
+        x, res = plan.solve_fgmres(b, rtol=1e-10, restart=12,
+                                   volume_rows=volume_rows)
+        out[on] = (x.cpu().numpy(), res.iterations, res.termination)
+        plan.close()
+    rel = (np.linalg.norm(out["1"][0] - out["0"][0])
+           / np.linalg.norm(out["0"][0]))
+    _say(f"[fgmres pairs] volume_rows={volume_rows} "
+         f"it={out['1'][1]}/{out['0'][1]} rel={rel:.2e}")
+    assert out["1"][2] == "converged" and out["0"][2] == "converged"
+    assert out["1"][1] == out["0"][1]
+    assert rel <= 1e-12, rel
