@@ -635,22 +635,56 @@ struct FgScales {
 // dst = src + sum_{row < count} s[row] * basis_row, the terms added in row
 // order (add_scaled_basis, src/solver_krylov.cpp:510-555); with scale != 0
 // the sum is then multiplied by post (the normalisation of a basis vector,
-// which must stay finite: a non-finite element sets *flag to 1).
+// which must stay finite: a non-finite element sets *flag to 1).  pairs
+// (an even vector length, so every basis row starts 16-byte aligned): a
+// thread forms two consecutive elements with 16-byte accesses
+// (ch.chunk = 2 kBlock); else one element per thread (ch.chunk = kBlock).
+struct alignas(16) FgPair {
+  double v[2];
+};
+
+__device__ __forceinline__ double fg_combine_one(const double* src,
+                                                 const double* basis,
+                                                 int64_t vstride,
+                                                 const FgScales& s, int count,
+                                                 int64_t c) {
+  double v = src[c];
+  for (int row = 0; row < count; ++row) v = v + s.s[row] * basis[row * vstride + c];
+  return v;
+}
+
 __global__ void k_fg_combine(Chunk ch, double* dst, const double* src,
                              const double* basis, int64_t vstride,
                              FgScales s, int count, double post, int scale,
-                             double* flag) {
+                             double* flag, int pairs) {
   int lb, lo, hi;
   chunk_range(ch, lb, lo, hi);
   bool bad = false;
-  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
-    double v = src[c];
-    for (int row = 0; row < count; ++row) v = v + s.s[row] * basis[row * vstride + c];
-    if (scale) {
-      v = post * v;
-      bad = bad || !isfinite(v);
+  const int step = pairs ? 2 : 1;
+  for (int c = lo + step * static_cast<int>(threadIdx.x); c < hi;
+       c += step * kBlock) {
+    if (pairs && c + 1 < hi) {
+      FgPair v = *reinterpret_cast<const FgPair*>(src + c);
+      for (int row = 0; row < count; ++row) {
+        const FgPair b =
+            *reinterpret_cast<const FgPair*>(basis + row * vstride + c);
+        v.v[0] = v.v[0] + s.s[row] * b.v[0];
+        v.v[1] = v.v[1] + s.s[row] * b.v[1];
+      }
+      if (scale) {
+        v.v[0] = post * v.v[0];
+        v.v[1] = post * v.v[1];
+        bad = bad || !isfinite(v.v[0]) || !isfinite(v.v[1]);
+      }
+      *reinterpret_cast<FgPair*>(dst + c) = v;
+    } else {
+      double v = fg_combine_one(src, basis, vstride, s, count, c);
+      if (scale) {
+        v = post * v;
+        bad = bad || !isfinite(v);
+      }
+      dst[c] = v;
     }
-    dst[c] = v;
   }
   if (bad) *flag = 1.0;
 }
@@ -664,17 +698,40 @@ template <class T>
 __global__ void k_fg_combine_entry(Chunk ch, double* dst, const double* src,
                                    const double* basis, int64_t vstride,
                                    FgScales s, int count, double post,
-                                   double* flag, const uint8_t* act, T* b) {
+                                   double* flag, const uint8_t* act, T* b,
+                                   int pairs) {
+  struct alignas(2 * sizeof(T)) BPair {
+    T v[2];
+  };
   int lb, lo, hi;
   chunk_range(ch, lb, lo, hi);
   bool bad = false;
-  for (int c = lo + threadIdx.x; c < hi; c += kBlock) {
-    double v = src[c];
-    for (int row = 0; row < count; ++row) v = v + s.s[row] * basis[row * vstride + c];
-    v = post * v;
-    bad = bad || !isfinite(v);
-    dst[c] = v;
-    b[c] = act[c] != 0 ? st<T>(v) : st<T>(0.0);
+  const int step = pairs ? 2 : 1;
+  for (int c = lo + step * static_cast<int>(threadIdx.x); c < hi;
+       c += step * kBlock) {
+    if (pairs && c + 1 < hi) {
+      FgPair v = *reinterpret_cast<const FgPair*>(src + c);
+      for (int row = 0; row < count; ++row) {
+        const FgPair bs =
+            *reinterpret_cast<const FgPair*>(basis + row * vstride + c);
+        v.v[0] = v.v[0] + s.s[row] * bs.v[0];
+        v.v[1] = v.v[1] + s.s[row] * bs.v[1];
+      }
+      v.v[0] = post * v.v[0];
+      v.v[1] = post * v.v[1];
+      bad = bad || !isfinite(v.v[0]) || !isfinite(v.v[1]);
+      *reinterpret_cast<FgPair*>(dst + c) = v;
+      const uint16_t fl = *reinterpret_cast<const uint16_t*>(act + c);
+      BPair bo;
+      bo.v[0] = (fl & 0xffu) != 0 ? st<T>(v.v[0]) : st<T>(0.0);
+      bo.v[1] = (fl >> 8) != 0 ? st<T>(v.v[1]) : st<T>(0.0);
+      *reinterpret_cast<BPair*>(b + c) = bo;
+    } else {
+      double v = post * fg_combine_one(src, basis, vstride, s, count, c);
+      bad = bad || !isfinite(v);
+      dst[c] = v;
+      b[c] = act[c] != 0 ? st<T>(v) : st<T>(0.0);
+    }
   }
   if (bad) *flag = 1.0;
 }

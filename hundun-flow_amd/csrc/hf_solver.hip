@@ -1473,6 +1473,12 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
   const Tiles& tz = pairs ? L0.tzp : L0.tz;
   const Chunk ch = p->chunk0;
   const int nbv = chunk_blocks(ch);
+  // the basis combinations take two elements per thread when the vector
+  // length is even (every basis row then starts 16-byte aligned)
+  const int comb_pairs = N % 2 == 0 ? 1 : 0;
+  Chunk ch2 = ch;
+  ch2.chunk = comb_pairs ? 2 * kBlock : kBlock;
+  const int nbv2 = chunk_blocks(ch2);
   const int vrows = ctl->volume_rows ? 1 : 0;
   const Widths wd = widths(p);
   const int64_t pst = tz.blocks;
@@ -1591,16 +1597,17 @@ hf_status fgmres_impl(hf_plan* p, const double* rhs, double* solution,
     if (scale && entry_in_combine) {
       dispatch(L0.prec, [&](auto t) {
         using T = decltype(t);
-        HF_LAUNCH(k_fg_combine_entry<T>, nbv, kBlock, 0, p->stream, ch, dst,
+        HF_LAUNCH(k_fg_combine_entry<T>, nbv2, kBlock, 0, p->stream, ch2, dst,
                   src, basis, N, k, static_cast<int>(count), post,
                   p->fg_scalars + kFgFlag,
-                  static_cast<const uint8_t*>(L0.act), static_cast<T*>(L0.b));
+                  static_cast<const uint8_t*>(L0.act), static_cast<T*>(L0.b),
+                  comb_pairs);
       });
       entered_next = true;
     } else {
-      HF_LAUNCH(k_fg_combine, nbv, kBlock, 0, p->stream, ch, dst, src, basis,
+      HF_LAUNCH(k_fg_combine, nbv2, kBlock, 0, p->stream, ch2, dst, src, basis,
                 N, k, static_cast<int>(count), post, scale,
-                p->fg_scalars + kFgFlag);
+                p->fg_scalars + kFgFlag, comb_pairs);
     }
     if (scale) {
       flag_pending = true;
