@@ -295,7 +295,7 @@ __global__ void k_row_ratio_max(int nx, int ny, int nz, const double* D,
 constexpr int kFgDots = 16;       // accumulators of one Arnoldi pass
 constexpr int kFgMaxRestart = 64;
 
-enum FgMode : int { kFgArnoldi = 0, kFgResidual = 1 };
+enum FgMode : int { kFgArnoldi = 0, kFgResidual = 1, kFgOpDot = 2 };
 
 // Level-0 FP64 action y = A' x, A' = A or A divided by the geometric cell
 // volume (the per-volume rows of the cold system), z-marching as
@@ -440,6 +440,8 @@ __global__ void k_fg_dots(Tiles t, const double* wv, const double* basis,
 // cell's operator value is formed as in k_fg_stencil; a thread adds its two
 // cells' products to its accumulators in cell order.  t tiles (nx / 2) x ny
 // columns in z-chunks; partials as in k_fg_stencil, one per block of t.
+// kFgOpDot (the PCG operator, k_stencil_z<double, kZOpDot> over pairs):
+// out = A' x and accumulator wacc holds x . out; aux is not read.
 template <int kMode, int kNd>
 __global__ void k_fg_stencil2(Tiles t, Lv<double> A, Bc bc, const double* x,
                               const double* aux, double* out,
@@ -520,7 +522,7 @@ __global__ void k_fg_stencil2(Tiles t, Lv<double> A, Bc bc, const double* x,
       pair(A.fy, fyo, fyl);
       pair(A.fy, fyo + nx, fyh);
       pair(A.D, c, d);
-      pair(aux, c, av);
+      if constexpr (kMode != kFgOpDot) pair(aux, c, av);
       const double fx0 = A.fx[c + row];
       const double fx1 = A.fx[c + row + 1];
       const double fx2 = A.fx[c + row + 2];
@@ -548,8 +550,10 @@ __global__ void k_fg_stencil2(Tiles t, Lv<double> A, Bc bc, const double* x,
         if (volume_rows) v = v / (wxy[e] * w.z[k]);
         if constexpr (kMode == kFgArnoldi)
           o.v[e] = v + -shift * av[e];
-        else
+        else if constexpr (kMode == kFgResidual)
           o.v[e] = 1.0 * av[e] + -1.0 * v;
+        else
+          o.v[e] = v;
       }
       *reinterpret_cast<Pair*>(out + c) = o;
       if constexpr (kMode == kFgArnoldi) {
@@ -561,8 +565,13 @@ __global__ void k_fg_stencil2(Tiles t, Lv<double> A, Bc bc, const double* x,
           acc[a] += bv[1] * o.v[1];
         }
       }
-      ww += o.v[0] * o.v[0];
-      ww += o.v[1] * o.v[1];
+      if constexpr (kMode == kFgOpDot) {
+        ww += xc[0] * o.v[0];
+        ww += xc[1] * o.v[1];
+      } else {
+        ww += o.v[0] * o.v[0];
+        ww += o.v[1] * o.v[1];
+      }
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         zl[e] = xc[e];

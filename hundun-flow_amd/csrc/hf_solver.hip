@@ -2467,6 +2467,9 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
                           const hf_solve_control* ctl, hf_solve_result* res) {
   const Lv<double> A = krylov_view(p);
   const Tiles& tr = p->levels[0].tz;  // z-marching Krylov stencils
+  // the operator over pairs of cells of a row on rows of even length
+  const bool op_pairs = p->krylov_pairs && p->levels[0].n[0] % 2 == 0;
+  const Tiles& L0p = p->levels[0].tzp;
   // the PCG update stores the preconditioner's entry when level 0 is a grid
   // level (the tail and line-relaxed entries also zero x; a Chebyshev level 0
   // reads no x in its first stage, and mg_apply projects its b)
@@ -2590,14 +2593,22 @@ static hf_status pcg_impl(hf_plan* p, const double* rhs, double* solution,
       return finish(fail(HF_NUMERICAL_FAILURE, kDetailBreakdown),
                     HF_TERM_BREAKDOWN);
     };
-    {
+    if (op_pairs) {
+      // q = A p over pairs of cells of a row (p.q rounds in another order)
+      HF_PROF(p, P_OPERATOR);
+      HF_LAUNCH((k_fg_stencil2<kFgOpDot, 0>), L0p.blocks, kBlock, 0,
+                p->stream, L0p, A, p->bc, static_cast<const double*>(pp),
+                static_cast<const double*>(nullptr), q,
+                static_cast<const double*>(nullptr), int64_t{0}, 0, 0.0, 0,
+                widths(p), p->partials, int64_t{0});
+    } else {
       HF_PROF(p, P_OPERATOR);
       HF_LAUNCH((k_stencil_z<double, kZOpDot>), tr.blocks, kBlock, 0,
                 p->stream, tr, A, p->bc, static_cast<const double*>(pp),
                 static_cast<const double*>(nullptr), q, p->partials, 0);
     }
     res->operator_applies++;
-    finalize(p, tr.blocks, 1, F_ALPHA, 0);
+    finalize(p, op_pairs ? L0p.blocks : tr.blocks, 1, F_ALPHA, 0);
     // an iteration that verifies on schedule replaces r with the true
     // residual, so its update forms x alone and its step checks wait for the
     // true residual's read

@@ -696,3 +696,30 @@ def test_row_pair_kernels_match_per_cell(torch_cuda, monkeypatch, name,
         plan.close()
     assert np.array_equal(out["1"], out["0"]), \
         np.max(np.abs(out["1"] - out["0"]))
+
+
+@pytest.mark.parametrize("name", ["step", "mixed_wide", "periodic_xy"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pcg_row_pair_operator(torch_cuda, monkeypatch, name, precision):
+    """The PCG operator over pairs of cells of a row (q = A p bit for bit;
+    p.q summed in another order) against the one-cell-per-lane operator:
+    the same iteration count and the solution to 1e-12."""
+    torch = torch_cuda
+    _say(f"[pcg pairs] case={name} precision={precision}")
+    case = CASES[name]()
+    pol = _policy(name, precision=precision)
+    b = _t(torch, case["rhs"])  # consistent with a constant null space
+    out = {}
+    for on in ("1", "0"):
+        monkeypatch.setenv("HF_KRYLOV_PAIRS", on)
+        plan = _gpu_plan(torch, case, **pol)
+        x, res = plan.solve_pcg(b, rtol=1e-10,
+                                flexible=precision != "fp64")
+        out[on] = (x.cpu().numpy(), res.iterations, res.termination)
+        plan.close()
+    rel = (np.linalg.norm(out["1"][0] - out["0"][0])
+           / np.linalg.norm(out["0"][0]))
+    _say(f"  it={out['1'][1]}/{out['0'][1]} rel={rel:.2e}")
+    assert out["1"][2] == "converged" and out["0"][2] == "converged"
+    assert out["1"][1] == out["0"][1]
+    assert rel <= 1e-12, rel
